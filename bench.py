@@ -1,0 +1,260 @@
+#!/usr/bin/env python
+"""Benchmark of the explicit MPM step on one MI355X.
+
+Workload (the metric's configuration): 3D granular column collapse,
+Drucker-Prager (phi 30 deg, psi 0, c 0), MUSL, cubic B-splines, domain
+1.0 x 0.5 x 0.5 m at h = 0.01 m (101 x 51 x 51 nodes), column
+0.2 x 0.5 x 0.4 m filled at 27 particles per cell with +-10% seeded jitter
+(1,080,000 particles), gravity, slip walls, dt = 1e-5 s.
+
+A step is one full MUSL update of all particles: spatial sort, P2G, node
+BCs, first map + re-map, node BCs, second map + return map. Inputs are
+resident in HBM before the timed region. The timed loop drives the C ABI of
+libpmpm_hip.so on one stream with HIP events around every entry point; the
+dominant kernel's roofline uses its own event time.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-updates/s at 1M DP particles; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per particle of each entry point (FP32, 3D, cubic,
+# Drucker-Prager, MUSL); derivation in DESIGN.md.
+BYTES_PER_PARTICLE = {
+    "partition": 76,     # hash: x 12 + key 4; 3 radix passes x (key 4 + key/val 8 in + 8 out); ranges 4
+    "p2g": 64,           # sorted 4, x 12, v 12, m 4, V 4, stress 24, + node flush
+    "musl_map": 64,      # sorted 4, x 12, v 12, m 4 in; x 12, v 12 out (+ node flush)
+    "musl_g2p": 248,     # in: sorted 4 x 12 F 36 V0 4 stress 24 eps 24 m 4 colour 1 flags 1 id 4
+                         # out: F 36 V 4 stress 24 eps 24 L 36 m 4 V0 4 colour 1 flags 1 id 4
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--per-axis", type=int, default=3, help="particles per cell per axis (3 -> 27 ppc)")
+    ap.add_argument("--cell-size", type=float, default=0.01)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+class Workload:
+    """Config-2 column collapse held in torch tensors on the GPU, driven through
+    the C ABI (ctypes)."""
+
+    def __init__(self, per_axis=3, h=0.01, seed=0):
+        import torch
+        from pyroclastmpm_amd import capi
+        from tests import scenarios as S
+
+        self.torch, self.capi = torch, capi
+        spec = S.column(h=h, per_axis=per_axis, dt=1e-5, jitter=0.1, seed=seed)
+        self.spec = spec
+        pos = spec.positions.astype(np.float32)
+        self.n = n = pos.shape[0]
+        dev = torch.device("cuda")
+        self.grid = capi.make_grid(spec.origin, spec.end, h, 3)
+        nn = self.nn = self.grid.num_cells_total
+        # particles.cpp:355-358: (Real)pow(h, D) / (Real)ppc
+        vol = np.float32(np.float32(float(np.float32(h)) ** 3) / np.float32(per_axis ** 3))
+
+        def f32(shape, fill=0.0):
+            return torch.full(shape, fill, dtype=torch.float32, device=dev)
+
+        def set_(fill_x):
+            d = {}
+            d["x"] = torch.from_numpy(np.ascontiguousarray(pos.T)).to(dev).clone() if fill_x else f32((3, n))
+            d["v"] = f32((3, n))
+            d["stress"] = f32((6, n))
+            F = f32((9, n))
+            F[0] = 1.0
+            F[4] = 1.0
+            F[8] = 1.0
+            d["F"] = F
+            d["L"] = f32((9, n))
+            d["mass"] = f32((n,), float(np.float32(1500.0) * vol))
+            d["volume"] = f32((n,), float(vol))
+            d["volume0"] = f32((n,), float(vol))
+            d["color"] = torch.zeros(n, dtype=torch.uint8, device=dev)
+            d["flags"] = torch.full((n,), capi.PF_ACTIVE, dtype=torch.uint8, device=dev)
+            d["id"] = torch.arange(n, dtype=torch.int32, device=dev)
+            d["eps_e"] = f32((6, n))
+            d["stress_ref"] = f32((6, n))
+            return d
+
+        self.sets = [set_(True), set_(False)]
+        self.cur = 0
+        self.nodes_t = {k: f32((nn,) if k == "mass" else (3 * nn,)) for k in ["mass", "mom", "mom_nt", "vel_nt",
+                                                                                "dvel"]}
+        self.sp_t = {k: torch.zeros(n, dtype=torch.int32, device=dev) for k in ["hash", "hash_sorted", "sorted"]}
+        self.sp_t["cell_start"] = torch.zeros(nn, dtype=torch.int32, device=dev)
+        self.sp_t["cell_end"] = torch.zeros(nn, dtype=torch.int32, device=dev)
+        self.ctx = capi.Context(n, nn)
+        self.nodes = capi.Nodes(*[self.nodes_t[k].data_ptr() for k in ["mass", "mom", "mom_nt", "vel_nt", "dvel"]])
+        self.spatial = capi.Spatial(self.sp_t["hash"].data_ptr(), self.sp_t["hash_sorted"].data_ptr(),
+                                    self.sp_t["sorted"].data_ptr(), self.sp_t["cell_start"].data_ptr(),
+                                    self.sp_t["cell_end"].data_ptr(), None)
+        self.views = [self._view(s) for s in self.sets]
+        prm = capi.Params()
+        prm.dim, prm.shape_fn, prm.dt, prm.alpha = 3, capi.SF["cubic"], 1e-5, 0.99
+        prm.small_mass_cutoff, prm.node_types_built, prm.wrap_quirk = 1e-9, 1, 1
+        p = spec.materials[0][1]
+        E, nu = np.float32(p["E"]), np.float32(p["pois"])
+        K = E / (np.float32(3.0) * (np.float32(1.0) - np.float32(2.0) * nu))
+        G = E / (np.float32(2.0) * (np.float32(1.0) + nu))
+        prm.num_materials = 1
+        m = prm.materials[0]
+        m.kind, m.flags = capi.MAT_DRUCKER_PRAGER, capi.MATF_UPDATE_HISTORY | capi.MATF_APEX_GUARD
+        for i, val in enumerate([K, G, p["M"], p["Pt"], 0.0, 0.0]):
+            m.p[i] = float(val)
+        prm.num_bcs = 2
+        prm.bcs[0].kind = capi.BC_GRAVITY
+        prm.bcs[0].f[2] = -9.81
+        prm.bcs[1].kind = capi.BC_NODE_DOMAIN
+        for a in range(3):
+            prm.bcs[1].i[a] = 1
+            prm.bcs[1].i[3 + a] = 1
+        self.prm = prm
+        self.stream = torch.cuda.current_stream().cuda_stream
+
+    def _view(self, d):
+        return self.capi.Particles(self.n, *[d[k].data_ptr() for k in ["x", "v", "stress", "F", "L", "mass",
+                                                                          "volume", "volume0", "color", "flags",
+                                                                          "id", "eps_e", "stress_ref"]])
+
+    def calls(self):
+        """The step as (name, thunk) entry-point calls, in order."""
+        L, capi = self.capi.lib(), self.capi
+        a, b = self.views[self.cur], self.views[self.cur ^ 1]
+        g, prm, sp, nd, s = C.byref(self.grid), C.byref(self.prm), C.byref(self.spatial), C.byref(self.nodes), \
+            self.stream
+        nb = self.prm.num_bcs
+        return [
+            ("partition", lambda: L.pmpm_partition(self.ctx.ptr, 3, g, a.x, self.n, sp, s)),
+            ("p2g", lambda: L.pmpm_p2g(prm, g, C.byref(a), sp, nd, s)),
+            ("grid_bc", lambda: L.pmpm_grid_bc(prm, g, nd, 0, nb, 0, s)),
+            ("musl_map", lambda: L.pmpm_musl_map(prm, g, C.byref(a), C.byref(b), sp, nd, s)),
+            ("grid_bc_remap", lambda: L.pmpm_grid_bc(prm, g, nd, 0, nb, 1, s)),
+            ("musl_g2p", lambda: L.pmpm_musl_g2p(prm, g, C.byref(a), C.byref(b), sp, nd, s)),
+        ]
+
+    def step(self, events=None):
+        for i, (name, fn) in enumerate(self.calls()):
+            if events is not None:
+                events[i][0].record()
+            rc = fn()
+            if rc != 0:
+                raise RuntimeError(f"{name} failed with code {rc}")
+            if events is not None:
+                events[i][1].record()
+        self.cur ^= 1
+
+    def fused_step(self):
+        L = self.capi.lib()
+        a, b = self.views[self.cur], self.views[self.cur ^ 1]
+        rc = L.pmpm_step(self.ctx.ptr, C.byref(self.prm), C.byref(self.grid), 1, C.byref(a), C.byref(b),
+                         C.byref(self.spatial), C.byref(self.nodes), self.stream)
+        if rc != 0:
+            raise RuntimeError(f"pmpm_step failed with code {rc}")
+        self.cur ^= 1
+
+
+def cpu_baseline(spec, steps):
+    """The oracle (FP32, one thread, the reference's node-gather loop
+    structure) on the same workload for `steps` steps after initialisation."""
+    from oracle import _oracle
+    from tests import scenarios as S
+
+    sim = S.build_oracle(_oracle, spec, "f32")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    dt = time.perf_counter() - t0
+    n = len(spec.positions)
+    return {"value": n * steps / dt, "unit": "particle-updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle FP32 single thread, {steps} steps of the full {n}-particle workload, "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.set_device(0)
+    wl = Workload(args.per_axis, args.cell_size)
+    n = wl.n
+    print(f"bench: column collapse, {n} particles, grid {list(wl.grid.num_cells)}, warmup {args.warmup}, "
+          f"steps {args.steps}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        wl.step()
+    torch.cuda.synchronize()
+    ncalls = len(wl.calls())
+    ev = [[[torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)] for _ in range(ncalls)]
+          for _ in range(args.steps)]
+    t_start, t_end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_start.record()
+    for k in range(args.steps):
+        wl.step(ev[k])
+    t_end.record()
+    torch.cuda.synchronize()
+    total_ms = t_start.elapsed_time(t_end)
+    names = [c[0] for c in wl.calls()]
+    per_call = {nm: sum(ev[k][i][0].elapsed_time(ev[k][i][1]) for k in range(args.steps)) / args.steps
+                for i, nm in enumerate(names)}
+    ms_per_step = total_ms / args.steps
+    value = n * args.steps / (total_ms / 1e3)
+    print("bench: per-entry-point mean ms: " + json.dumps({k: round(v, 4) for k, v in per_call.items()}),
+          file=sys.stderr, flush=True)
+    # dominant single-kernel entry point
+    single = {k: v for k, v in per_call.items() if k in ("musl_g2p", "musl_map")}
+    dom = max(single, key=single.get)
+    achieved = BYTES_PER_PARTICLE[dom] * n / (single[dom] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_particle": BYTES_PER_PARTICLE[dom], "kernel_ms": round(single[dom], 4)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(wl.spec, args.cpu_steps)
+        except Exception as e:  # the baseline is reported, never required
+            cpu = {"value": None, "error": str(e)}
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "particle-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "3D Drucker-Prager MUSL column collapse, cubic B-spline, "
+                               f"{n} particles (27 per cell), grid 101x51x51, dt 1e-5",
+                   "particles": n, "cell_size": args.cell_size, "solver": "MUSL", "shape_function": "cubic",
+                   "material": "Drucker-Prager phi=30 psi=0 c=0", "per_entry_point_ms": per_call},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,251 @@
+/* pmpm.h -- C ABI of the MI355X explicit-MPM hot path (libpmpm_hip.so).
+ *
+ * All pointers are device pointers unless stated. All work is enqueued
+ * asynchronously on the given HIP stream (NULL = default stream); nothing in
+ * an entry point synchronises the device unless it says so. Every entry point
+ * returns PMPM_OK (0) or a negative PMPM_ERR_* code and never exits the
+ * process. There are no process globals: time step, shape function, grid and
+ * material parameters travel in the arguments, so independent contexts can run
+ * side by side (one per scene).
+ *
+ * HBM layout (structure of arrays, component-major):
+ *   particle vector field  float[dim][n]        v[a*n + i]
+ *   particle D x D tensor  float[dim*dim][n]    F[(a*dim+b)*n + i] = F(a,b)
+ *   particle stress        float[6][n]          xx, yy, zz, xy, xz, yz
+ *                                               (symmetric Cauchy stress; the
+ *                                               reference's 3x3 in all dims)
+ *   node vector field      float[dim][nn]       x-fastest node index
+ * Particles are stored in "slots"; id[slot] is the particle's original index.
+ * Solvers keep slots physically sorted by cell (the G2P kernels write their
+ * output in cell order), and the host API maps back through id[].
+ */
+#ifndef PMPM_H_
+#define PMPM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *pmpm_stream_t; /* == hipStream_t */
+
+enum {
+  PMPM_OK = 0,
+  PMPM_ERR_INVALID = -1,     /* bad argument (dimension, shape function, size) */
+  PMPM_ERR_HIP = -2,         /* a HIP runtime call failed */
+  PMPM_ERR_WORKSPACE = -3,   /* context workspace too small; call pmpm_context_reserve */
+  PMPM_ERR_UNSUPPORTED = -4  /* valid request this build does not implement */
+};
+
+enum { PMPM_SF_LINEAR = 0, PMPM_SF_QUADRATIC = 1, PMPM_SF_CUBIC = 2 };
+enum { PMPM_MAT_NONE = 0, PMPM_MAT_LINEAR_ELASTIC = 1, PMPM_MAT_NEWTON_FLUID = 2,
+       PMPM_MAT_LOCAL_RHEO = 3, PMPM_MAT_DRUCKER_PRAGER = 4 };
+enum { PMPM_BC_NONE = 0, PMPM_BC_GRAVITY = 1, PMPM_BC_NODE_DOMAIN = 2, PMPM_BC_RIGID_SDF = 4 };
+
+/* material flags (DP) */
+enum { PMPM_MATF_UPDATE_HISTORY = 1, PMPM_MATF_VELGRAD_IS_INCREMENT = 2,
+       PMPM_MATF_APEX_GUARD = 4, PMPM_MATF_HAS_STRESS_REF = 8 };
+/* particle flag bits */
+enum { PMPM_PF_RIGID = 1, PMPM_PF_ACTIVE = 2 };
+
+#define PMPM_MAX_MATERIALS 8
+#define PMPM_MAX_BCS 8
+
+/* Background grid (types_common.h:250-291 Grid). num_cells = nodes per axis. */
+typedef struct pmpm_grid {
+  float origin[3];
+  float cell_size;
+  float inv_cell_size;
+  int32_t num_cells[3];
+  int32_t num_cells_total;
+} pmpm_grid;
+
+/* Particle state (particles.h:181-232 ParticlesContainer fields, as SoA). */
+typedef struct pmpm_particles {
+  int32_t n;
+  float *x;          /* [dim][n] positions */
+  float *v;          /* [dim][n] velocities */
+  float *stress;     /* [6][n] */
+  float *F;          /* [dim*dim][n] deformation gradient */
+  float *L;          /* [dim*dim][n] velocity gradient (may be NULL: not stored) */
+  float *mass;       /* [n] */
+  float *volume;     /* [n] current volume */
+  float *volume0;    /* [n] original volume */
+  uint8_t *color;    /* [n] material id */
+  uint8_t *flags;    /* [n] PMPM_PF_* */
+  int32_t *id;       /* [n] original particle index of each slot */
+  float *eps_e;      /* [6][n] elastic strain history (Drucker-Prager) or NULL */
+  float *stress_ref; /* [6][n] reference stress (Drucker-Prager) or NULL */
+} pmpm_particles;
+
+/* Node state (nodes.h:113-140 NodesContainer fields, as SoA). */
+typedef struct pmpm_nodes {
+  float *mass;    /* [nn] */
+  float *mom;     /* [dim][nn] moments */
+  float *mom_nt;  /* [dim][nn] forward moments (USL), re-mapped moments (MUSL) */
+  float *vel_nt;  /* [dim][nn] derived: mom_nt / mass (0 where mass <= cutoff) */
+  float *dvel;    /* [dim][nn] derived: (mom_nt - mom) / mass */
+} pmpm_nodes;
+
+/* Spatial partition (spatialpartition.h:77-132 SpatialPartition fields). */
+typedef struct pmpm_spatial {
+  uint32_t *hash;        /* [n] cell hash per slot (unsorted) */
+  uint32_t *hash_sorted; /* [n] */
+  int32_t *sorted;       /* [n] slot of the j-th particle in (hash, slot) order */
+  int32_t *cell_start;   /* [num_cells_total] */
+  int32_t *cell_end;     /* [num_cells_total]; empty cells hold 0 in both */
+  int32_t *bins;         /* [dim][n] integer cell coordinates per slot, or NULL */
+} pmpm_spatial;
+
+typedef struct pmpm_material {
+  int32_t kind;  /* PMPM_MAT_* */
+  int32_t flags; /* PMPM_MATF_* */
+  /* LINEAR_ELASTIC: lame, shear, bulk
+   * NEWTON_FLUID:   viscosity, bulk_modulus, gamma
+   * LOCAL_RHEO:     shear, lame, bulk, rho_c, mu_s, mu_2, I0, EPS
+   * DRUCKER_PRAGER: bulk, shear, M, Pt, eta_overline, Vs */
+  float p[16];
+} pmpm_material;
+
+typedef struct pmpm_bc {
+  int32_t kind; /* PMPM_BC_* */
+  /* NODE_DOMAIN: face0_mode[3], face1_mode[3] (0 fixed, 1 slip, other none) */
+  int32_t i[6];
+  /* GRAVITY: gravity[3] applied this step
+   * RIGID_SDF: half[3], center[3], velocity[3], rotation[9] (row-major,
+   *            body->world), band, advance (1 = move body by velocity*dt) */
+  float f[24];
+} pmpm_bc;
+
+/* Everything one step needs besides the buffers. */
+typedef struct pmpm_params {
+  int32_t dim;              /* 1, 2, 3 */
+  int32_t shape_fn;         /* PMPM_SF_* */
+  float dt;
+  float alpha;              /* FLIP/PIC blend (usl.h:85-114) */
+  float small_mass_cutoff;  /* nodes.h:140, 1e-9 */
+  int32_t node_types_built; /* 0 reproduces a NodesContainer built before the
+                               cubic spline was selected (all node types 0) */
+  int32_t wrap_quirk;       /* 1: P2G bounds-checks only the linear node index,
+                               as the reference does (usl_inline.h:122-126) */
+  int32_t num_materials;
+  pmpm_material materials[PMPM_MAX_MATERIALS];
+  int32_t num_bcs;
+  pmpm_bc bcs[PMPM_MAX_BCS];
+} pmpm_params;
+
+typedef struct pmpm_context pmpm_context;
+
+/* ---- context / workspace -------------------------------------------------
+ * A context owns the scratch memory of the spatial sort for up to
+ * max_particles particles and max_cells cells on the current device. */
+int pmpm_context_create(pmpm_context **ctx);
+int pmpm_context_destroy(pmpm_context *ctx);
+int pmpm_context_reserve(pmpm_context *ctx, int32_t max_particles, int32_t max_cells);
+/* Particles found outside [0, num_cells_total) by the last pmpm_spatial
+ * (synchronises the stream). The reference writes out of bounds there
+ * (spatialpartition_inline.h:102-113); here they sort last and join no cell. */
+int pmpm_context_out_of_grid(pmpm_context *ctx, pmpm_stream_t stream, int32_t *count);
+
+/* ---- spatial binning: replaces SpatialPartition::calculate_hash,
+ * sort_hashes and bin_particles (spatialpartition.cpp:90-136,
+ * spatialpartition_inline.h:49-115). Stable LSD radix sort of (hash, slot):
+ * sorted[] equals thrust::stable_sort_by_key over slots 0..n-1. */
+int pmpm_partition(pmpm_context *ctx, int32_t dim, const pmpm_grid *grid, const float *x,
+                   int32_t n, pmpm_spatial *part, pmpm_stream_t stream);
+
+/* ---- shape functions: replaces calculate_shape_function
+ * (shapefunction.cpp:70-190). Materialises psi [n][nsn] and dpsi [n][nsn][dim]
+ * in the reference's layout and window order, per slot. The solver kernels
+ * never call this: they recompute weights in registers. */
+int pmpm_shape_function(int32_t dim, int32_t shape_fn, int32_t node_types_built,
+                        const pmpm_grid *grid, const float *x, int32_t n, float *psi,
+                        float *dpsi, pmpm_stream_t stream);
+
+/* ---- P2G: replaces USL::P2G / usl_p2g_kernel (usl.cpp:118-161,
+ * usl_inline.h:82-180). Clears the node arrays, scatters mass, momentum and
+ * force, then writes mom_nt = mom + force*dt (nodes with mass <= cutoff are
+ * left at 0). Needs a partition of the same positions. */
+int pmpm_p2g(const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles *p,
+             const pmpm_spatial *part, pmpm_nodes *nodes, pmpm_stream_t stream);
+
+/* ---- node boundary conditions: replaces the apply_on_nodes_moments hooks of
+ * Gravity, NodeDomain (gravity.cpp:65-88, nodedomain.cpp:42-64) and the
+ * analytic box body, applied in list order to prm->bcs[bc_begin, bc_end).
+ * remap_pass = 1 applies only kinematic constraints (MUSL second map).
+ * Always recomputes vel_nt and dvel afterwards. */
+int pmpm_grid_bc(const pmpm_params *prm, const pmpm_grid *grid, pmpm_nodes *nodes,
+                 int32_t bc_begin, int32_t bc_end, int32_t remap_pass, pmpm_stream_t stream);
+
+/* ---- rigid-particle level set contact: replaces
+ * RigidBodyLevelSet::apply_on_nodes_moments (rigidbodylevelset.cpp:92-275,
+ * rigidbodylevelset_inline.h:65-343): rigid particle velocities from the
+ * body rotation, nearest-rigid-particle stick contact on the nodes, then the
+ * rigid particles advance. rotation (row-major D x D, host-side animation)
+ * and com are HOST pointers. dpsi_orig is the materialised dpsi of
+ * pmpm_shape_function in ORIGINAL particle order: the reference indexes it
+ * with the linear window size 2^D whatever the spline
+ * (rigidbodylevelset_inline.h:147-148), so the stored table is what it reads. */
+int pmpm_rigid_levelset(const pmpm_params *prm, const pmpm_grid *grid, pmpm_particles *p,
+                        const pmpm_spatial *part, pmpm_nodes *nodes, const float *rotation,
+                        const float *com, const float *dpsi_orig, float *rigid_dx,
+                        pmpm_stream_t stream);
+/* rigid_dx (optional, [dim][n]): when given, the rigid particles are not moved
+ * here; their displacement v*dt is stored (0 for the others) and applied after
+ * G2P with pmpm_rigid_advance, so that G2P still weights with the positions
+ * the step started from (the reference's stored psi). */
+int pmpm_rigid_advance(int32_t dim, int32_t n, float *x, const float *rigid_dx,
+                       const int32_t *sorted, pmpm_stream_t stream);
+
+/* ---- utility: dst[c*n + id[i]] = src[c*n + i] for c < ncomp (slot order ->
+ * original order), or the inverse when inverse != 0. No reference
+ * counterpart: the reference never reorders particle storage. */
+int pmpm_permute(int32_t n, int32_t ncomp, int32_t elem_bytes, const void *src,
+                 const int32_t *id, void *dst, int32_t inverse, pmpm_stream_t stream);
+
+/* ---- G2P fused with the return maps: replaces USL::G2P + Solver::stress_update
+ * (usl_inline.h:245-332, solver.cpp:110-121, linearelastic_inline.h:40-107,
+ * newtonfluid_inline.h:49-96, localrheo_inline.h:74-157,
+ * dp_rheo_inline.h:123-290). Reads particle j of the cell order from slot
+ * part->sorted[j] of `in` and writes it to slot j of `out` when out->F !=
+ * in->F (the particles end up sorted by cell), or back to its own slot when
+ * the F buffers are the same (in place). Weights use in->x. */
+int pmpm_g2p(const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles *in,
+             pmpm_particles *out, const pmpm_spatial *part, const pmpm_nodes *nodes,
+             pmpm_stream_t stream);
+
+/* ---- MUSL (no reference implementation; semantics of test_musl.h:37-156,
+ * musl.cuh:39-72). First map + re-map, fused: velocities and positions from
+ * the grid, then mom_nt := sum N m v with the new velocities (mass cutoff as
+ * P2G). Writes x, v of particle j to slot j of `out` (out != in required). */
+int pmpm_musl_map(const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles *in,
+                  pmpm_particles *out, const pmpm_spatial *part, pmpm_nodes *nodes,
+                  pmpm_stream_t stream);
+/* Second map: velocity gradient, F, volume from the re-mapped moments, then
+ * the return maps (weights from in->x, the positions the step started from);
+ * slot placement as pmpm_g2p. */
+int pmpm_musl_g2p(const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles *in,
+                  pmpm_particles *out, const pmpm_spatial *part, const pmpm_nodes *nodes,
+                  pmpm_stream_t stream);
+
+/* ---- stand-alone return map: replaces Material::stress_update for one
+ * material id (linearelastic.cpp:63-90 etc.), in place, from p->L. */
+int pmpm_stress_update(const pmpm_params *prm, int32_t mat_id, pmpm_particles *p,
+                       pmpm_stream_t stream);
+
+/* ---- one full step: USL::solve (usl.cpp:69-115) or its MUSL counterpart
+ * for configurations without a rigid-particle level set: partition, P2G,
+ * node BCs, G2P + return maps. Reads `a`, writes `b` (cell-sorted). */
+int pmpm_step(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid, int32_t musl,
+              const pmpm_particles *a, pmpm_particles *b, pmpm_spatial *part,
+              pmpm_nodes *nodes, pmpm_stream_t stream);
+
+/* Build identification string (static storage). */
+const char *pmpm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PMPM_H_ */

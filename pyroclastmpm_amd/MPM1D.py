@@ -1,0 +1,5 @@
+"""1-D module of the PyroclastMPM API (the reference's `pyroclastmpm.MPM1D`)."""
+from ._core import MPM1D as _m
+
+globals().update({k: getattr(_m, k) for k in dir(_m) if not k.startswith("__")})
+del _m

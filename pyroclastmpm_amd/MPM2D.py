@@ -1,0 +1,5 @@
+"""2-D module of the PyroclastMPM API (the reference's `pyroclastmpm.MPM2D`)."""
+from ._core import MPM2D as _m
+
+globals().update({k: getattr(_m, k) for k in dir(_m) if not k.startswith("__")})
+del _m

@@ -1,0 +1,47 @@
+// context.hpp -- private definition of pmpm_context and host-side helpers
+// shared by the .hip translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pmpm.h"
+
+struct pmpm_context {
+  int device = 0;
+  int32_t cap_particles = 0;
+  int32_t cap_cells = 0;
+  uint32_t *keys_tmp = nullptr;  // radix sort ping-pong keys
+  int32_t *vals_tmp = nullptr;   // radix sort ping-pong values
+  uint32_t *hist = nullptr;      // [RADIX][blocks] digit histograms
+  size_t hist_len = 0;
+  uint32_t *counters = nullptr;  // [4]: out-of-grid particles, ...
+};
+
+namespace pmpm {
+
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortBlock * kSortItems;
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+#define PMPM_HIP_CHECK(expr)                              \
+  do {                                                    \
+    hipError_t e_ = (expr);                               \
+    if (e_ != hipSuccess) return PMPM_ERR_HIP;            \
+  } while (0)
+
+#define PMPM_LAUNCH_CHECK()                               \
+  do {                                                    \
+    hipError_t e_ = hipGetLastError();                    \
+    if (e_ != hipSuccess) return PMPM_ERR_HIP;            \
+  } while (0)
+
+// Fill n floats/ints with zero (a kernel, not hipMemsetAsync, so the step can
+// be captured in a HIP graph).
+int zero_fill(void *ptr, int64_t n_words, hipStream_t s);
+
+}  // namespace pmpm

@@ -1,0 +1,379 @@
+// device.hpp -- device-side building blocks shared by the gfx950 kernels:
+// B-spline weights per axis, symmetric 3x3 stress algebra and the four
+// return maps. Header-only, included by the .hip translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pmpm.h"
+
+namespace pmpm {
+
+// ---------------------------------------------------------------------------
+// Spline traits. SPAN = node extent per cell (tile halo), ENTRIES = nodes one
+// particle touches per axis. NLO = first node offset relative to the cell.
+// ---------------------------------------------------------------------------
+template <int SF> struct Spline;
+template <> struct Spline<PMPM_SF_LINEAR> { static constexpr int SPAN = 2, ENTRIES = 2, NLO = 0; };
+template <> struct Spline<PMPM_SF_QUADRATIC> { static constexpr int SPAN = 4, ENTRIES = 3, NLO = -1; };
+template <> struct Spline<PMPM_SF_CUBIC> { static constexpr int SPAN = 4, ENTRIES = 4, NLO = -1; };
+
+// Boundary-modified cubic B-spline pieces (cubic.cuh:28-197): coefficients of
+// c0 + c1 r + c2 r^2 + c3 r^3 for node type 0..4 (0 = untyped -> 0) and piece
+// k covering r in [k-2, k-1).
+__device__ __constant__ static const float kCubic[5][4][4] = {
+    {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}},
+    {{4.f / 3.f, 2.f, 1.f, 1.f / 6.f}, {1.f, 1.f, 0.f, -1.f / 6.f}, {1.f, -1.f, 0.f, 1.f / 6.f}, {4.f / 3.f, -2.f, 1.f, -1.f / 6.f}},
+    {{0, 0, 0, 0}, {2.f / 3.f, 0.f, -1.f, -1.f / 3.f}, {2.f / 3.f, 0.f, -1.f, 0.5f}, {4.f / 3.f, -2.f, 1.f, -1.f / 6.f}},
+    {{4.f / 3.f, 2.f, 1.f, 1.f / 6.f}, {2.f / 3.f, 0.f, -1.f, -0.5f}, {2.f / 3.f, 0.f, -1.f, 0.5f}, {4.f / 3.f, -2.f, 1.f, -1.f / 6.f}},
+    {{4.f / 3.f, 2.f, 1.f, 1.f / 6.f}, {2.f / 3.f, 0.f, -1.f, -0.5f}, {2.f / 3.f, 0.f, -1.f, 1.f / 3.f}, {0, 0, 0, 0}}};
+
+__device__ inline int cubic_node_type(int id, int n, int built) {
+  if (!built) return 0;
+  if (id == 0 || id == n - 1) return 1;
+  if (id == 1) return 2;
+  return 3;  // the reference never assigns type 4 (nodes.cpp:160)
+}
+
+__device__ inline void cubic_eval(float r, int type, float inv_h, float &N, float &dN) {
+  int k;
+  if (r >= -2.f && r < -1.f) k = 0;
+  else if (r >= -1.f && r < 0.f) k = 1;
+  else if (r >= 0.f && r < 1.f) k = 2;
+  else if (r >= 1.f && r < 2.f) k = 3;
+  else { N = 0.f; dN = 0.f; return; }
+  const float *c = kCubic[type][k];
+  N = ((c[3] * r + c[2]) * r + c[1]) * r + c[0];
+  dN = inv_h * ((3.f * c[3] * r + 2.f * c[2]) * r + c[1]);
+}
+
+__device__ inline void linear_eval(float r, float inv_h, float &N, float &dN) {
+  const float a = fabsf(r);
+  N = (a >= 1.f) ? 0.f : 1.f - a;
+  dN = (r > 1.f || r < -1.f || r == 0.f) ? 0.f : (r > 0.f ? -inv_h : inv_h);
+}
+
+__device__ inline void quad_eval(float r, float inv_h, float &N, float &dN) {
+  const float a = fabsf(r);
+  if (a < 0.5f) { N = 0.75f - r * r; dN = -2.f * r * inv_h; }
+  else if (a < 1.5f) { const float b = 1.5f - a; N = 0.5f * b * b; dN = (r > 0.f ? -b : b) * inv_h; }
+  else { N = 0.f; dN = 0.f; }
+}
+
+// Any-spline, any-offset evaluation (used by the materialised shape function
+// and by the wrap-around slow path): r = relative coordinate, type = cubic
+// node type.
+template <int SF> __device__ inline void spline_eval(float r, int type, float inv_h, float &N, float &dN) {
+  if (SF == PMPM_SF_LINEAR) linear_eval(r, inv_h, N, dN);
+  else if (SF == PMPM_SF_QUADRATIC) quad_eval(r, inv_h, N, dN);
+  else cubic_eval(r, type, inv_h, N, dN);
+}
+
+// Per-axis weights of one particle. t = (x - origin) * inv_h, bin = trunc(t).
+// first = offset (relative to bin) of entry 0; N[e], dN[e] for the ENTRIES
+// nodes bin+first+e. The relative coordinate (t - bin) - o equals the
+// reference's t - (bin + o) bit for bit (t - bin is exact).
+template <int SF>
+__device__ inline void axis_weights(float t, int bin, int n_axis, int types_built, float inv_h,
+                                    int &first, float *N, float *dN) {
+  const float frac = t - (float)bin;
+  if (SF == PMPM_SF_LINEAR) {
+    first = 0;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) linear_eval(frac - (float)e, inv_h, N[e], dN[e]);
+  } else if (SF == PMPM_SF_QUADRATIC) {
+    first = (frac < 0.5f) ? -1 : 0;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) quad_eval(frac - (float)(first + e), inv_h, N[e], dN[e]);
+  } else {
+    first = -1;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int o = e - 1;
+      cubic_eval(frac - (float)o, cubic_node_type(bin + o, n_axis, types_built), inv_h, N[e], dN[e]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Symmetric 3x3 (stress, elastic strain): xx yy zz xy xz yz.
+// ---------------------------------------------------------------------------
+struct Sym {
+  float xx, yy, zz, xy, xz, yz;
+};
+__device__ inline float sym_at(const Sym &s, int i, int j) {
+  if (i == j) return i == 0 ? s.xx : (i == 1 ? s.yy : s.zz);
+  const int k = i + j;  // 1 -> xy, 2 -> xz, 3 -> yz
+  return k == 1 ? s.xy : (k == 2 ? s.xz : s.yz);
+}
+__device__ inline Sym sym_zero() { return Sym{0.f, 0.f, 0.f, 0.f, 0.f, 0.f}; }
+__device__ inline Sym load_sym(const float *a, int n, int i) {
+  return Sym{a[i], a[n + i], a[2 * n + i], a[3 * n + i], a[4 * n + i], a[5 * n + i]};
+}
+__device__ inline void store_sym(float *a, int n, int i, const Sym &s) {
+  a[i] = s.xx; a[n + i] = s.yy; a[2 * n + i] = s.zz;
+  a[3 * n + i] = s.xy; a[4 * n + i] = s.xz; a[5 * n + i] = s.yz;
+}
+__device__ inline float sym_frob2(const Sym &s) {
+  return s.xx * s.xx + s.yy * s.yy + s.zz * s.zz + 2.f * (s.xy * s.xy + s.xz * s.xz + s.yz * s.yz);
+}
+
+// ---------------------------------------------------------------------------
+// Return maps. L is the velocity gradient padded to 3x3 row-major (L[3*i+j]),
+// entries outside the D x D block are 0. dt = time step.
+// ---------------------------------------------------------------------------
+
+// linearelastic_inline.h:40-107: sigma_DxD += lambda*deps + 2G*deps,
+// deps = 0.5 (L + L^T) dt (the reference's lambda*deps*I is a matrix product).
+template <int D>
+__device__ inline void rm_linear_elastic(const pmpm_material &m, float dt, const float *L, Sym &s) {
+  const float lam = m.p[0], twoG = 2.f * m.p[1];
+  float de[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) de[i][j] = 0.5f * (L[3 * i + j] + L[3 * j + i]) * dt;
+  s.xx += lam * de[0][0] + twoG * de[0][0];
+  if (D > 1) { s.yy += lam * de[1][1] + twoG * de[1][1]; s.xy += lam * de[0][1] + twoG * de[0][1]; }
+  if (D > 2) {
+    s.zz += lam * de[2][2] + twoG * de[2][2];
+    s.xz += lam * de[0][2] + twoG * de[0][2];
+    s.yz += lam * de[1][2] + twoG * de[1][2];
+  }
+}
+
+// newtonfluid_inline.h:49-96: p = -K((rho/rho0)^gamma - 1) on the D x D block,
+// tau = (2 mu / 3) dev(D); the stress is replaced, not incremented.
+template <int D>
+__device__ inline void rm_newton_fluid(const pmpm_material &m, const float *L, float mass, float vol,
+                                       float vol0, Sym &s) {
+  const float visc = m.p[0], K = m.p[1], gam = m.p[2];
+  const float rho = mass / vol, rho0 = mass / vol0;
+  const float pres = -K * (powf(rho / rho0, gam) - 1.f);
+  float Dm[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Dm[i][j] = 0.5f * (L[3 * i + j] + L[3 * j + i]);
+  const float tr = Dm[0][0] + (D > 1 ? Dm[1][1] : 0.f) + (D > 2 ? Dm[2][2] : 0.f);
+  const float third = (float)((1.0 / 3.0) * (double)tr);
+  const float fac = (float)(2.0 * (double)visc / 3.0);
+  Sym o = sym_zero();
+  o.xx = pres + (Dm[0][0] - third) * fac;
+  if (D > 1) { o.yy = pres + (Dm[1][1] - third) * fac; o.xy = Dm[0][1] * fac; }
+  if (D > 2) { o.zz = pres + (Dm[2][2] - third) * fac; o.xz = Dm[0][2] * fac; o.yz = Dm[1][2] * fac; }
+  s = o;
+}
+
+// localrheo_inline.h:39-157: hypoelastic Jaumann trial, zero stress below the
+// critical density or in tension, otherwise the rate-dependent radial return
+// with the reference's hard-coded grain density 2450 and diameter 0.005.
+__device__ inline double negative_root(double a, double b, double c) {
+  if (b > 0) return (-b - sqrt(b * b - 4 * a * c)) / (2 * a);
+  return (2 * c) / (-b + sqrt(b * b - 4 * a * c));
+}
+__device__ inline void rm_local_rheo(const pmpm_material &m, float dt, const float *L, float mass,
+                                     float vol, Sym &s) {
+  const float G = m.p[0], lam = m.p[1], rho_c = m.p[3], mu_s = m.p[4], mu_2 = m.p[5], I0 = m.p[6];
+  const float rho = mass / vol;
+  float Dm[3][3], W[3][3], S[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      Dm[i][j] = 0.5f * (L[3 * i + j] + L[3 * j + i]);
+      W[i][j] = 0.5f * (L[3 * i + j] - L[3 * j + i]);
+      S[i][j] = sym_at(s, i, j);
+    }
+  const float trD = Dm[0][0] + Dm[1][1] + Dm[2][2];
+  float st[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      float sw = 0.f, ws = 0.f;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) { sw += S[i][k] * W[k][j]; ws += W[i][k] * S[k][j]; }
+      st[i][j] = S[i][j] + dt * (2.f * G * Dm[i][j] + (i == j ? lam * trD : 0.f) + (-sw + ws));
+    }
+  const float ptr = (float)(-(1.0 / 3.0) * (double)(st[0][0] + st[1][1] + st[2][2]));
+  float s0[3][3];
+  float f2 = 0.f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      s0[i][j] = st[i][j] + (i == j ? ptr : 0.f);
+      f2 += s0[i][j] * s0[i][j];
+    }
+  const float tau = (float)sqrt(0.5 * (double)f2);
+  if (ptr < 0.f || rho < rho_c) { s = sym_zero(); return; }
+  const float S0 = mu_s * ptr;
+  float scale = 1.f;
+  if (!(tau <= S0)) {
+    const float S2 = mu_2 * ptr;
+    const float al = G * I0 * dt * sqrtf(ptr / 2450.0f) / 0.005f;
+    const float B = -(S2 + tau + al);
+    const float H = S2 * tau + S0 * al;
+    const float tn = (float)negative_root(1.0, (double)B, (double)H);
+    scale = tn / tau;
+  }
+  s.xx = scale * s0[0][0] - ptr;
+  s.yy = scale * s0[1][1] - ptr;
+  s.zz = scale * s0[2][2] - ptr;
+  s.xy = scale * s0[0][1];
+  s.xz = scale * s0[0][2];
+  s.yz = scale * s0[1][2];
+}
+
+// dp_rheo_inline.h:43-290 (+ dp_rheo.cpp:45-70): co-rotated elastic-strain
+// trial, perfectly plastic Drucker-Prager cone (eta = M/sqrt3, tension cut Pt,
+// dilatancy eta_bar) with the closed-form cone and apex returns.
+template <int D>
+__device__ inline void rm_drucker_prager(const pmpm_material &m, float dt, const float *L,
+                                         const Sym &sref, Sym &s, Sym &eps) {
+  const float K = m.p[0], G = m.p[1], M = m.p[2], Pt = m.p[3], etab = m.p[4];
+  float de[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  float ep[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) ep[i][j] = (i < D && j < D) ? sym_at(eps, i, j) : 0.f;
+  if (m.flags & PMPM_MATF_VELGRAD_IS_INCREMENT) {
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) de[i][j] = L[3 * i + j];
+  } else {
+    float W[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        de[i][j] = 0.5f * (L[3 * i + j] + L[3 * j + i]) * dt;
+        W[i][j] = 0.5f * (L[3 * i + j] - L[3 * j + i]) * dt;
+      }
+    float c[3][3];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        float we = 0.f, ew = 0.f;
+#pragma unroll
+        for (int k = 0; k < D; ++k) { we += W[i][k] * ep[k][j]; ew += ep[i][k] * W[k][j]; }
+        c[i][j] = -we + ew;
+      }
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) de[i][j] += c[i][j];
+  }
+  const float pref = (float)((1.0 / 3.0) * (double)(sref.xx + sref.yy + sref.zz));
+  float sr[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) sr[i][j] = sym_at(sref, i, j) - (i == j ? pref : 0.f);
+  float et[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) et[i][j] = (i < D && j < D) ? ep[i][j] + de[i][j] : 0.f;
+  const float ev = et[0][0] + (D > 1 ? et[1][1] : 0.f) + (D > 2 ? et[2][2] : 0.f);
+  const float ptr = pref + K * ev;
+  const float ev3 = (1.f / 3.f) * ev;
+  float str[3][3];
+  float f2 = 0.f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      float dev = et[i][j] - (i == j ? ev3 : 0.f);
+      if (D < 3 && i == 2 && j == 2) dev = 0.f;
+      str[i][j] = sr[i][j] + 2.f * G * dev;
+      f2 += str[i][j] * str[i][j];
+    }
+  const float J2 = 0.5f * f2;
+  const float eta = 0.57735026918962576f * M;
+  const float sqJ2 = sqrtf(J2);
+  const float Phi = sqJ2 + eta * (ptr - Pt);
+  if (Phi <= 0.f) {
+    s = Sym{str[0][0] + ptr, str[1][1] + ptr, str[2][2] + ptr, str[0][1], str[0][2], str[1][2]};
+    if (m.flags & PMPM_MATF_UPDATE_HISTORY)
+      eps = Sym{et[0][0], D > 1 ? et[1][1] : 0.f, D > 2 ? et[2][2] : 0.f, D > 1 ? et[0][1] : 0.f,
+                D > 2 ? et[0][2] : 0.f, D > 2 ? et[1][2] : 0.f};
+    return;
+  }
+  // cone (dp_rheo_inline.h:50-65)
+  float dg = Phi / (K * eta * etab + G);
+  float pn = ptr - K * etab * dg;
+  float f = 1.f - (G * dg) / sqJ2;
+  float sn[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) sn[i][j] = f * str[i][j];
+  if (sqJ2 - G * dg < 0.f) {  // apex (dp_rheo_inline.h:67-78)
+    if ((m.flags & PMPM_MATF_APEX_GUARD) && etab == 0.f) {
+      pn = Pt;
+    } else {
+      const float dga = (ptr - Pt) / (etab * K);
+      pn = ptr - K * etab * dga;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) sn[i][j] = 0.f;
+  }
+  // update_plasticity (dp_rheo_inline.h:80-96)
+  s = Sym{sn[0][0] + pn, sn[1][1] + pn, sn[2][2] + pn, sn[0][1], sn[0][2], sn[1][2]};
+  if (m.flags & PMPM_MATF_UPDATE_HISTORY) {
+    const float i2G = 1.f / (2.f * G);
+    const float vol = (pn - pref) / (3.f * K);
+    float e[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) e[i][j] = (sn[i][j] - sr[i][j]) * i2G + (i == j ? vol : 0.f);
+    eps = Sym{e[0][0], D > 1 ? e[1][1] : 0.f, D > 2 ? e[2][2] : 0.f, D > 1 ? e[0][1] : 0.f,
+              D > 2 ? e[0][2] : 0.f, D > 2 ? e[1][2] : 0.f};
+  }
+}
+
+// Dispatch on the material of the particle's colour (solver.cpp:110-121: each
+// material updates only particles whose colour equals its list index).
+template <int D>
+__device__ inline void return_map(const pmpm_params &prm, int color, uint8_t pflags, const float *L,
+                                  float mass, float vol, float vol0, const Sym &sref, Sym &s, Sym &eps) {
+  if (color >= prm.num_materials) return;
+  const pmpm_material &m = prm.materials[color];
+  const bool active = (pflags & PMPM_PF_ACTIVE) != 0;
+  switch (m.kind) {
+    case PMPM_MAT_LINEAR_ELASTIC:
+      if (active) rm_linear_elastic<D>(m, prm.dt, L, s);
+      break;
+    case PMPM_MAT_NEWTON_FLUID:
+      if (active) rm_newton_fluid<D>(m, L, mass, vol, vol0, s);
+      break;
+    case PMPM_MAT_LOCAL_RHEO:
+      rm_local_rheo(m, prm.dt, L, mass, vol, s);
+      break;
+    case PMPM_MAT_DRUCKER_PRAGER:
+      rm_drucker_prager<D>(m, prm.dt, L, sref, s, eps);
+      break;
+    default:
+      break;
+  }
+}
+
+template <int D> __device__ inline float det_dd(const float *F) {
+  if (D == 1) return F[0];
+  if (D == 2) return F[0] * F[3] - F[1] * F[2];
+  return F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+         F[2] * (F[3] * F[7] - F[4] * F[6]);
+}
+
+__device__ inline int64_t ceil_div64(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace pmpm

@@ -1,0 +1,266 @@
+// partition.hip -- spatial binning on gfx950: cell hash per particle, stable
+// LSD radix sort of (hash, slot), cell start/end ranges. Replaces the
+// reference's calculate_hashes / thrust::stable_sort_by_key / bin_particles
+// (spatialpartition_inline.h:49-115, spatialpartition.cpp:77-136).
+#include <hip/hip_runtime.h>
+
+#include <new>
+
+#include "context.hpp"
+
+namespace pmpm {
+
+__global__ void zero_kernel(uint32_t *p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+int zero_fill(void *ptr, int64_t n_words, hipStream_t s) {
+  if (n_words <= 0) return PMPM_OK;
+  int64_t blocks = ceil_div(n_words, 256);
+  if (blocks > 16384) blocks = 16384;
+  hipLaunchKernelGGL(zero_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (uint32_t *)ptr, n_words);
+  PMPM_LAUNCH_CHECK();
+  return PMPM_OK;
+}
+
+// bins = trunc((x - origin) * inv_h) per axis (int cast truncates toward 0, as
+// Eigen's cast<int>), hash = x + y*nx + z*nx*ny. Hashes outside
+// [0, num_cells_total) are counted and clamped to num_cells_total so they sort
+// last and belong to no cell.
+template <int D>
+__global__ void hash_kernel(const float *__restrict__ x, int n, pmpm_grid g, uint32_t *__restrict__ hash,
+                            int32_t *__restrict__ bins, uint32_t *__restrict__ out_of_grid) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int b[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    const float t = (x[a * n + i] - g.origin[a]) * g.inv_cell_size;
+    b[a] = (int)t;
+  }
+  int64_t h = b[0];
+  if (D > 1) h += (int64_t)b[1] * g.num_cells[0];
+  if (D > 2) h += (int64_t)b[2] * g.num_cells[0] * g.num_cells[1];
+  uint32_t key;
+  if (h < 0 || h >= g.num_cells_total) {
+    key = (uint32_t)g.num_cells_total;
+    atomicAdd(out_of_grid, 1u);
+  } else {
+    key = (uint32_t)h;
+  }
+  hash[i] = key;
+  if (bins) {
+#pragma unroll
+    for (int a = 0; a < D; ++a) bins[a * n + i] = b[a];
+  }
+}
+
+// Per-block digit histogram, digit-major layout hist[digit * nblocks + block].
+__global__ __launch_bounds__(kSortBlock) void radix_hist_kernel(const uint32_t *__restrict__ keys, int n, int shift,
+                                                                uint32_t *__restrict__ hist, int nblocks) {
+  __shared__ uint32_t h[kRadix];
+  const int t = threadIdx.x;
+  h[t] = 0u;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+#pragma unroll 4
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t i = base + (int64_t)r * kSortBlock + t;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & (kRadix - 1)], 1u);
+  }
+  __syncthreads();
+  hist[(int64_t)t * nblocks + blockIdx.x] = h[t];
+}
+
+// Exclusive scan of len entries with one workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void scan_kernel(uint32_t *__restrict__ data, int64_t len) {
+  __shared__ uint32_t part[1024];
+  const int t = threadIdx.x;
+  const int64_t chunk = (len + 1023) / 1024;
+  const int64_t b = t * chunk;
+  const int64_t e = (b + chunk < len) ? b + chunk : len;
+  uint32_t s = 0;
+  for (int64_t i = b; i < e; ++i) s += data[i];
+  part[t] = s;
+  __syncthreads();
+  // Hillis-Steele inclusive scan of the 1024 partial sums
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = (t >= off) ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = (t > 0) ? part[t - 1] : 0u;
+  for (int64_t i = b; i < e; ++i) {
+    const uint32_t v = data[i];
+    data[i] = run;
+    run += v;
+  }
+}
+
+// Stable scatter of one radix pass. Within a round of 256 keys (thread order
+// == input order) a lane's rank among same-digit lanes of its wave comes from
+// 8 ballots; per-wave counts are combined per digit in LDS.
+__global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
+    const uint32_t *__restrict__ keys_in, const int32_t *__restrict__ vals_in, uint32_t *__restrict__ keys_out,
+    int32_t *__restrict__ vals_out, int n, int shift, const uint32_t *__restrict__ hist, int nblocks) {
+  __shared__ uint32_t base[kRadix];
+  __shared__ uint32_t wcnt[kSortBlock / 64][kRadix];
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int w = t >> 6;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  base[t] = hist[(int64_t)t * nblocks + blockIdx.x];
+  const int64_t tile = (int64_t)blockIdx.x * kSortTile;
+  for (int r = 0; r < kSortItems; ++r) {
+#pragma unroll
+    for (int q = 0; q < kSortBlock / 64; ++q) wcnt[q][t] = 0u;
+    __syncthreads();
+    const int64_t i = tile + (int64_t)r * kSortBlock + t;
+    const bool valid = i < n;
+    const uint32_t key = valid ? keys_in[i] : 0u;
+    const uint32_t d = (key >> shift) & (kRadix - 1);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < kRadixBits; ++bit) {
+      const bool on = (d >> bit) & 1u;
+      const uint64_t m = __ballot(on);
+      peers &= on ? m : ~m;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+    if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    {
+      uint32_t run = base[t];
+#pragma unroll
+      for (int q = 0; q < kSortBlock / 64; ++q) {
+        const uint32_t c = wcnt[q][t];
+        wcnt[q][t] = run;
+        run += c;
+      }
+      base[t] = run;
+    }
+    __syncthreads();
+    if (valid) {
+      const uint32_t pos = wcnt[w][d] + rank;
+      keys_out[pos] = key;
+      vals_out[pos] = vals_in ? vals_in[i] : (int32_t)i;
+    }
+    __syncthreads();
+  }
+}
+
+// Cell ranges over the sorted hashes (spatialpartition_inline.h:85-115);
+// empty cells keep the 0 of the preceding clear (spatialpartition.cpp:81-82).
+__global__ void cell_range_kernel(const uint32_t *__restrict__ hs, int n, uint32_t ncells,
+                                  int32_t *__restrict__ cstart, int32_t *__restrict__ cend) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t h = hs[j];
+  if (h >= ncells) return;
+  if (j == 0 || hs[j - 1] != h) cstart[h] = j;
+  if (j == n - 1 || hs[j + 1] != h) cend[h] = j + 1;
+}
+
+}  // namespace pmpm
+
+using namespace pmpm;
+
+extern "C" int pmpm_context_create(pmpm_context **ctx) {
+  if (!ctx) return PMPM_ERR_INVALID;
+  pmpm_context *c = new (std::nothrow) pmpm_context();
+  if (!c) return PMPM_ERR_INVALID;
+  if (hipGetDevice(&c->device) != hipSuccess) { delete c; return PMPM_ERR_HIP; }
+  if (hipMalloc(&c->counters, 4 * sizeof(uint32_t)) != hipSuccess) { delete c; return PMPM_ERR_HIP; }
+  if (hipMemset(c->counters, 0, 4 * sizeof(uint32_t)) != hipSuccess) { (void)hipFree(c->counters); delete c; return PMPM_ERR_HIP; }
+  *ctx = c;
+  return PMPM_OK;
+}
+
+extern "C" int pmpm_context_destroy(pmpm_context *ctx) {
+  if (!ctx) return PMPM_OK;
+  (void)hipFree(ctx->keys_tmp);
+  (void)hipFree(ctx->vals_tmp);
+  (void)hipFree(ctx->hist);
+  (void)hipFree(ctx->counters);
+  delete ctx;
+  return PMPM_OK;
+}
+
+extern "C" int pmpm_context_reserve(pmpm_context *ctx, int32_t max_particles, int32_t max_cells) {
+  if (!ctx || max_particles < 0 || max_cells < 0) return PMPM_ERR_INVALID;
+  if (max_particles > ctx->cap_particles) {
+    (void)hipFree(ctx->keys_tmp);
+    (void)hipFree(ctx->vals_tmp);
+    (void)hipFree(ctx->hist);
+    ctx->keys_tmp = nullptr; ctx->vals_tmp = nullptr; ctx->hist = nullptr; ctx->cap_particles = 0;
+    const size_t np = (size_t)(max_particles > 0 ? max_particles : 1);
+    const size_t nblocks = (size_t)ceil_div((int64_t)np, kSortTile);
+    ctx->hist_len = (size_t)kRadix * nblocks;
+    PMPM_HIP_CHECK(hipMalloc(&ctx->keys_tmp, np * sizeof(uint32_t)));
+    PMPM_HIP_CHECK(hipMalloc(&ctx->vals_tmp, np * sizeof(int32_t)));
+    PMPM_HIP_CHECK(hipMalloc(&ctx->hist, ctx->hist_len * sizeof(uint32_t)));
+    ctx->cap_particles = max_particles;
+  }
+  if (max_cells > ctx->cap_cells) ctx->cap_cells = max_cells;
+  return PMPM_OK;
+}
+
+extern "C" int pmpm_context_out_of_grid(pmpm_context *ctx, pmpm_stream_t stream, int32_t *count) {
+  if (!ctx || !count) return PMPM_ERR_INVALID;
+  uint32_t v = 0;
+  PMPM_HIP_CHECK(hipMemcpyAsync(&v, ctx->counters, sizeof(uint32_t), hipMemcpyDeviceToHost, (hipStream_t)stream));
+  PMPM_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  *count = (int32_t)v;
+  return PMPM_OK;
+}
+
+extern "C" int pmpm_partition(pmpm_context *ctx, int32_t dim, const pmpm_grid *grid, const float *x, int32_t n,
+                              pmpm_spatial *part, pmpm_stream_t stream) {
+  if (!ctx || !grid || !part || dim < 1 || dim > 3 || n < 0) return PMPM_ERR_INVALID;
+  if (n > 0 && (!x || !part->hash || !part->hash_sorted || !part->sorted)) return PMPM_ERR_INVALID;
+  if (!part->cell_start || !part->cell_end || grid->num_cells_total <= 0) return PMPM_ERR_INVALID;
+  if (n > ctx->cap_particles) return PMPM_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  const pmpm_grid g = *grid;
+  const uint32_t ncells = (uint32_t)g.num_cells_total;
+  int rc;
+  if ((rc = zero_fill(ctx->counters, 1, s)) != PMPM_OK) return rc;
+  if ((rc = zero_fill(part->cell_start, ncells, s)) != PMPM_OK) return rc;
+  if ((rc = zero_fill(part->cell_end, ncells, s)) != PMPM_OK) return rc;
+  if (n == 0) return PMPM_OK;
+  const unsigned blocks = (unsigned)ceil_div(n, 256);
+  switch (dim) {
+    case 1: hipLaunchKernelGGL(hash_kernel<1>, dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, ctx->counters); break;
+    case 2: hipLaunchKernelGGL(hash_kernel<2>, dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, ctx->counters); break;
+    default: hipLaunchKernelGGL(hash_kernel<3>, dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, ctx->counters); break;
+  }
+  PMPM_LAUNCH_CHECK();
+  // keys are in [0, ncells] (ncells = out-of-grid bucket)
+  int bits = 1;
+  while (bits < 32 && ((uint64_t)1 << bits) <= (uint64_t)ncells) ++bits;
+  const int passes = (bits + kRadixBits - 1) / kRadixBits;
+  const int nblocks = (int)ceil_div(n, kSortTile);
+  const uint32_t *kin = part->hash;
+  const int32_t *vin = nullptr;  // identity on the first pass
+  for (int p = 0; p < passes; ++p) {
+    const bool to_dest = ((passes - 1 - p) % 2) == 0;
+    uint32_t *kout = to_dest ? part->hash_sorted : ctx->keys_tmp;
+    int32_t *vout = to_dest ? part->sorted : ctx->vals_tmp;
+    const int shift = p * kRadixBits;
+    hipLaunchKernelGGL(radix_hist_kernel, dim3(nblocks), dim3(kSortBlock), 0, s, kin, n, shift, ctx->hist, nblocks);
+    PMPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, ctx->hist, (int64_t)kRadix * nblocks);
+    PMPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(radix_scatter_kernel, dim3(nblocks), dim3(kSortBlock), 0, s, kin, vin, kout, vout, n, shift,
+                       ctx->hist, nblocks);
+    PMPM_LAUNCH_CHECK();
+    kin = kout;
+    vin = vout;
+  }
+  hipLaunchKernelGGL(cell_range_kernel, dim3(blocks), dim3(256), 0, s, part->hash_sorted, n, ncells, part->cell_start,
+                     part->cell_end);
+  PMPM_LAUNCH_CHECK();
+  return PMPM_OK;
+}

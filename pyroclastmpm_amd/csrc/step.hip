@@ -1,0 +1,40 @@
+// step.hip -- one full USL / MUSL step through the C ABI (usl.cpp:69-115):
+// partition -> P2G -> node BCs -> G2P + return maps, all enqueued on one
+// stream with no host synchronisation, so the sequence can be captured in a
+// HIP graph by the caller.
+#include <hip/hip_runtime.h>
+
+#include "context.hpp"
+
+namespace pmpm {
+int do_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &p, const pmpm_spatial &part,
+           const pmpm_nodes &nodes, hipStream_t s);
+int do_g2p(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
+           const pmpm_spatial &part, const pmpm_nodes &nodes, int mode, hipStream_t s);
+int do_musl_map(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
+                const pmpm_spatial &part, const pmpm_nodes &nodes, hipStream_t s);
+int launch_grid(const pmpm_params &prm, const pmpm_grid &g, const pmpm_nodes &nodes, int finalize, int b0, int b1,
+                int remap, hipStream_t s);
+}  // namespace pmpm
+
+using namespace pmpm;
+
+extern "C" int pmpm_step(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid, int32_t musl,
+                         const pmpm_particles *a, pmpm_particles *b, pmpm_spatial *part, pmpm_nodes *nodes,
+                         pmpm_stream_t stream) {
+  if (!ctx || !prm || !grid || !a || !b || !part || !nodes) return PMPM_ERR_INVALID;
+  if (prm->dim < 1 || prm->dim > 3 || prm->shape_fn < 0 || prm->shape_fn > 2) return PMPM_ERR_INVALID;
+  if (prm->num_bcs < 0 || prm->num_bcs > PMPM_MAX_BCS) return PMPM_ERR_INVALID;
+  if (a->n != b->n || a->x == b->x) return PMPM_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = pmpm_partition(ctx, prm->dim, grid, a->x, a->n, part, stream);
+  if (rc != PMPM_OK) return rc;
+  if ((rc = do_p2g(*prm, *grid, *a, *part, *nodes, s)) != PMPM_OK) return rc;
+  if ((rc = launch_grid(*prm, *grid, *nodes, 1, 0, prm->num_bcs, 0, s)) != PMPM_OK) return rc;
+  if (!musl) return do_g2p(*prm, *grid, *a, *b, *part, *nodes, 0, s);
+  if ((rc = do_musl_map(*prm, *grid, *a, *b, *part, *nodes, s)) != PMPM_OK) return rc;
+  if ((rc = launch_grid(*prm, *grid, *nodes, 0, 0, prm->num_bcs, 1, s)) != PMPM_OK) return rc;
+  return do_g2p(*prm, *grid, *a, *b, *part, *nodes, 2, s);
+}
+
+extern "C" const char *pmpm_version(void) { return "pyroclastmpm_amd 0.1 (gfx950)"; }

@@ -1,0 +1,206 @@
+"""Scenario specs shared by the GPU parity tests, the smoke entry point and the
+benchmark: one description, built either on the product's PyroclastMPM-style
+API or on the CPU oracle (test infrastructure)."""
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+
+@dataclass
+class Spec:
+    dim: int
+    origin: list
+    end: list
+    cell_size: float
+    shape_function: str
+    dt: float
+    ppc: int
+    positions: np.ndarray
+    velocities: Optional[np.ndarray] = None
+    colors: Optional[np.ndarray] = None
+    stresses: Optional[np.ndarray] = None
+    rigid_positions: Optional[np.ndarray] = None
+    materials: List[tuple] = field(default_factory=list)  # (kind, params dict)
+    bcs: List[tuple] = field(default_factory=list)        # (kind, params dict)
+    musl: bool = False
+    alpha: float = 0.99
+    wrap_quirk: bool = True
+
+
+# ------------------------------------------------------------------ geometry
+def create_circle(center, radius, cell_size, ppc=4):
+    """Lattice disc of examples/two_disc_impact/run.py:49-75 (spacing h/(ppc/2),
+    half-spacing offset, r^2 + 5e-5 tolerance)."""
+    center = np.asarray(center, dtype=np.float64)
+    start, end = center - radius, center + radius
+    spacing = cell_size / (ppc / 2)
+    tol = 0.00005
+    x = np.arange(start[0], end[0] + spacing, spacing) + 0.5 * spacing
+    y = np.arange(start[1], end[1] + spacing, spacing) + 0.5 * spacing
+    xv, yv = np.meshgrid(x, y)
+    pts = np.stack([xv.ravel(), yv.ravel()], axis=1)
+    mask = (pts[:, 0] - center[0]) ** 2 + (pts[:, 1] - center[1]) ** 2 < radius ** 2 + tol
+    return pts[mask]
+
+
+def box_lattice(lo, hi, spacing, jitter=0.0, seed=0):
+    """Lattice of points at half-spacing offsets inside [lo, hi], optional
+    uniform jitter of +-jitter*spacing per axis."""
+    axes = [np.arange(a + 0.5 * spacing, b - 1e-12, spacing) for a, b in zip(lo, hi)]
+    grids = np.meshgrid(*axes, indexing="ij")
+    pts = np.stack([g.ravel() for g in grids], axis=1)
+    if jitter > 0:
+        rng = np.random.default_rng(seed)
+        pts = pts + rng.uniform(-jitter * spacing, jitter * spacing, size=pts.shape)
+    return pts
+
+
+def rot_y(deg):
+    t = math.radians(deg)
+    return [math.cos(t), 0.0, math.sin(t), 0.0, 1.0, 0.0, -math.sin(t), 0.0, math.cos(t)]
+
+
+def dp_params(phi_deg, cohesion, psi_deg=0.0):
+    sp = math.sin(math.radians(phi_deg))
+    M = 6 * sp / (3 - sp)
+    Pt = cohesion * math.cos(math.radians(phi_deg)) / sp if sp > 0 else 0.0
+    return M, Pt
+
+
+# ------------------------------------------------------------------ scenarios
+def two_disc(alpha=0.99, musl=False):
+    """Config 1: 2D two-disc elastic impact, linear, USL (416 particles)."""
+    h = 0.05
+    c1 = create_circle([0.25, 0.25], 0.2, h, 4)
+    c2 = create_circle([0.75, 0.75], 0.2, h, 4)
+    pos = np.vstack([c1, c2])
+    vel = np.vstack([np.full(c1.shape, 0.1), np.full(c2.shape, -0.1)])
+    return Spec(dim=2, origin=[0.0, 0.0], end=[1.0, 1.0], cell_size=h, shape_function="linear", dt=1e-3, ppc=4,
+                positions=pos, velocities=vel,
+                materials=[("LinearElastic", {"density": 1000.0, "E": 1000.0, "pois": 0.3})],
+                bcs=[("NodeDomain", {"face0_mode": [0, 0], "face1_mode": [0, 0]})], alpha=alpha, musl=musl)
+
+
+def column(h=0.01, per_axis=3, phi=30.0, cohesion=0.0, E=1e6, dt=1e-5, jitter=0.1, seed=0,
+           domain=(1.0, 0.5, 0.5), col=(0.2, 0.5, 0.4), shape_function="cubic", musl=True):
+    """Config 2: 3D granular column collapse, Drucker-Prager, MUSL, cubic."""
+    spacing = h / per_axis
+    pos = box_lattice([0, 0, 0], col, spacing, jitter=jitter, seed=seed)
+    M, Pt = dp_params(phi, cohesion)
+    return Spec(dim=3, origin=[0.0, 0.0, 0.0], end=list(domain), cell_size=h, shape_function=shape_function, dt=dt,
+                ppc=per_axis ** 3, positions=pos,
+                materials=[("DP_rheo", {"density": 1500.0, "E": E, "pois": 0.3, "M": M, "Pt": Pt,
+                                        "dilatancy_angle": 0.0})],
+                bcs=[("Gravity", {"gravity": [0.0, 0.0, -9.81]}),
+                     ("NodeDomain", {"face0_mode": [1, 1, 1], "face1_mode": [1, 1, 1]})],
+                musl=musl)
+
+
+def scoop(h=0.0125, per_axis=2, domain=(2.0, 1.0, 1.0), bed=0.4, dt=5e-6, seed=0):
+    """Config 3: 3D rigid box scoop into a mu(I) bed, quadratic B-spline, USL."""
+    spacing = h / per_axis
+    pos = box_lattice([0, 0, 0], [domain[0], domain[1], bed], spacing, jitter=0.0, seed=seed)
+    return Spec(dim=3, origin=[0.0, 0.0, 0.0], end=list(domain), cell_size=h, shape_function="quadratic", dt=dt,
+                ppc=per_axis ** 3, positions=pos,
+                materials=[("LocalGranularRheology", {"density": 1500.0, "E": 1e6, "pois": 0.3, "I0": 0.279,
+                                                      "mu_s": 0.38, "mu_2": 0.64, "rho_c": 900.0,
+                                                      "particle_diameter": 0.001, "particle_density": 2450.0})],
+                bcs=[("Gravity", {"gravity": [0.0, 0.0, -9.81]}),
+                     ("RigidBodySDF", {"half_extents": [0.15, 0.4, 0.025], "center": [0.3, 0.5, 0.45],
+                                       "velocity": [1.0, 0.0, -0.2], "rotation": rot_y(15.0),
+                                       "band": h}),
+                     ("NodeDomain", {"face0_mode": [1, 1, 1], "face1_mode": [1, 1, 1]})])
+
+
+# ------------------------------------------------------------------ builders
+def _mk_material(pm, kind, p):
+    if kind == "LinearElastic":
+        return pm.LinearElastic(p["density"], p["E"], p.get("pois", 0.0))
+    if kind == "NewtonFluid":
+        return pm.NewtonFluid(p["density"], p["viscosity"], p.get("bulk_modulus", 0.0), p.get("gamma", 7.0))
+    if kind == "LocalGranularRheology":
+        return pm.LocalGranularRheology(p["density"], p["E"], p["pois"], p["I0"], p["mu_s"], p["mu_2"], p["rho_c"],
+                                        p["particle_diameter"], p["particle_density"])
+    if kind == "DP_rheo":
+        m = pm.DP_rheo(p["density"], p["E"], p["pois"], p["M"], p.get("lam", 0.0), p.get("kap", 0.0),
+                       p.get("Vs", 0.0), p.get("R", 0.0), p.get("Pt", 0.0), p.get("beta", 0.0),
+                       p.get("dilatancy_angle", 0.0), p.get("solid_volume_fraction", 1.0))
+        m.apex_guard = bool(p.get("apex_guard", 1))
+        return m
+    raise ValueError(kind)
+
+
+def _mk_bc(pm, kind, p, dim):
+    if kind == "Gravity":
+        return pm.Gravity(p["gravity"], bool(p.get("is_ramp", 0)), int(p.get("ramp_step", 0)),
+                          p.get("gravity_end", [0.0] * dim))
+    if kind == "NodeDomain":
+        return pm.NodeDomain(p["face0_mode"], p["face1_mode"])
+    if kind == "RigidBodyLevelSet":
+        b = pm.RigidBodyLevelSet()
+        if p.get("mode", 0) == 1:
+            b.set_mode_loop_rotate(p["euler_angles_per_second"], p.get("COM", [0.0] * dim))
+        return b
+    if kind == "RigidBodySDF":
+        return pm.RigidBodySDF(p["half_extents"], p["center"], p["velocity"], p.get("rotation"), p.get("band", 0.0))
+    raise ValueError(kind)
+
+
+def build_product(spec: Spec):
+    import importlib
+
+    pm = importlib.import_module(f"pyroclastmpm_amd.MPM{spec.dim}D")
+    pm.set_globals(spec.dt, spec.ppc, spec.shape_function, "./output/")
+    pm.set_global_step(0)
+    nodes = pm.NodesContainer(spec.origin, spec.end, spec.cell_size)
+    particles = pm.ParticlesContainer(spec.positions, spec.velocities, spec.colors, spec.rigid_positions)
+    if spec.stresses is not None:
+        particles.stresses = spec.stresses
+    mats = [_mk_material(pm, k, p) for k, p in spec.materials]
+    bcs = [_mk_bc(pm, k, p, spec.dim) for k, p in spec.bcs]
+    cls = pm.MUSL if spec.musl else pm.USL
+    solver = cls(particles, nodes, mats, bcs, spec.alpha)
+    solver.wrap_quirk = spec.wrap_quirk
+    return pm, solver
+
+
+def build_oracle(mod, spec: Spec, prec="f32"):
+    cls = getattr(mod, f"Sim{spec.dim}D_{prec}")
+    s = cls(spec.origin, spec.end, spec.cell_size, spec.shape_function, spec.dt, spec.ppc)
+    s.set_particles(spec.positions, spec.velocities, spec.colors, spec.rigid_positions)
+    if spec.stresses is not None:
+        s.set("stresses", spec.stresses)
+    for k, p in spec.materials:
+        s.add_material(k, dict(p))
+    for k, p in spec.bcs:
+        s.add_bc(k, dict(p))
+    s.musl = spec.musl
+    s.alpha = spec.alpha
+    s.initialize()
+    return s
+
+
+FIELDS = ["positions", "velocities", "stresses", "F", "volumes"]
+
+
+def product_state(solver):
+    p = solver.particles
+    return {"positions": np.asarray(p.positions, dtype=np.float64),
+            "velocities": np.asarray(p.velocities, dtype=np.float64),
+            "stresses": np.asarray(p.stresses, dtype=np.float64),
+            "F": np.asarray(p.F, dtype=np.float64),
+            "volumes": np.asarray(p.volumes, dtype=np.float64)}
+
+
+def oracle_state(sim):
+    return {k: np.asarray(sim.get(k), dtype=np.float64) for k in FIELDS}
+
+
+def max_rel_err(a, b):
+    """max |a - b| / max |b| (field-scaled), 0 for an all-zero reference."""
+    scale = float(np.max(np.abs(b))) if b.size else 0.0
+    if scale == 0.0:
+        return float(np.max(np.abs(a))) if a.size else 0.0
+    return float(np.max(np.abs(a - b))) / scale

@@ -1,0 +1,250 @@
+"""GPU parity of the HIP path against the CPU oracle (same seeded inputs).
+
+Bars (stated per test):
+  * integer outputs (bins, hashes, sort order, cell ranges): bit-exact;
+  * linear-spline weights: bit-exact; quadratic / cubic: |diff| <= 2e-6
+    (the GPU evaluates the polynomials in FP32, the reference in FP64);
+  * single transfers: field-scaled relative error <= 2e-5 (FP32 atomics
+    change the summation order);
+  * multi-step runs: field-scaled relative error <= 1e-4 after the stated
+    number of steps against the FP32 oracle.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from tests.conftest import oracle_sim
+from tests import scenarios as S
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _banner(msg):
+    print(f"\n[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import pyroclastmpm_amd
+    assert pyroclastmpm_amd.device_count() > 0
+
+
+# --------------------------------------------------------------- binning
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_partition_bit_exact(oracle_mod, dim):
+    from pyroclastmpm_amd import capi
+    rng = np.random.default_rng(7 + dim)
+    n = {1: 5000, 2: 60000, 3: 200000}[dim]
+    h = {1: 0.01, 2: 0.02, 3: 0.025}[dim]
+    end = [1.0] * dim
+    x = rng.uniform(0.0, 0.999, size=(n, dim)).astype(np.float32)
+    # particles exactly on cell faces exercise the truncating cast
+    x[: n // 10] = (np.floor(x[: n // 10] / h) * h).astype(np.float32)
+    _banner(f"partition dim={dim} n={n}")
+    g = capi.make_grid([0, 0, 0], end + [0] * (3 - dim), h, dim)
+    ctx = capi.Context(n, g.num_cells_total)
+    dev = torch.device("cuda")
+    xs = torch.from_numpy(np.ascontiguousarray(x.T)).to(dev)
+    t = {k: torch.zeros(n, dtype=torch.int32, device=dev) for k in ["hash", "hash_sorted", "sorted"]}
+    cs = torch.zeros(g.num_cells_total, dtype=torch.int32, device=dev)
+    ce = torch.zeros_like(cs)
+    bins = torch.zeros(dim * n, dtype=torch.int32, device=dev)
+    sp = capi.Spatial(t["hash"].data_ptr(), t["hash_sorted"].data_ptr(), t["sorted"].data_ptr(), cs.data_ptr(),
+                      ce.data_ptr(), bins.data_ptr())
+    capi.check(capi.lib().pmpm_partition(ctx.ptr, dim, C.byref(g), xs.data_ptr(), n, C.byref(sp), None))
+    torch.cuda.synchronize()
+    s = oracle_sim(oracle_mod, dim, "f32", [0.0] * dim, end, h, "linear", 0.1, 1)
+    s.set_particles(x.astype(np.float64))
+    s.partition()
+    np.testing.assert_array_equal(bins.cpu().numpy().reshape(dim, n).T, s.get("bins"))
+    np.testing.assert_array_equal(t["hash"].cpu().numpy().astype(np.int64), s.get("hash_unsorted"))
+    np.testing.assert_array_equal(t["hash_sorted"].cpu().numpy().astype(np.int64), s.get("hash_sorted"))
+    np.testing.assert_array_equal(t["sorted"].cpu().numpy(), s.get("sorted_index"))
+    np.testing.assert_array_equal(cs.cpu().numpy(), s.get("cell_start"))
+    np.testing.assert_array_equal(ce.cpu().numpy(), s.get("cell_end"))
+
+
+# --------------------------------------------------------------- weights
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("sf", ["linear", "quadratic", "cubic"])
+def test_shape_functions(oracle_mod, dim, sf):
+    import importlib
+    pm = importlib.import_module(f"pyroclastmpm_amd.MPM{dim}D")
+    rng = np.random.default_rng(3)
+    h = 0.1
+    x = rng.uniform(0.0, 1.0, size=(300, dim))
+    x[:20] = np.round(x[:20] / h) * h  # on nodes
+    _banner(f"shape functions dim={dim} sf={sf}")
+    pm.set_globals(0.1, 1, sf, "./output/")
+    nodes = pm.NodesContainer([0.0] * dim, [1.0] * dim, h)
+    parts = pm.ParticlesContainer(x)
+    psi, dpsi = pm.calculate_shape_function(nodes, parts)
+    s = oracle_sim(oracle_mod, dim, "f32", [0.0] * dim, [1.0] * dim, h, sf, 0.1, 1)
+    s.set_particles(x)
+    s.partition()
+    s.calculate_shape_function()
+    opsi = s.get("psi").reshape(psi.shape)
+    odpsi = s.get("dpsi").reshape(dpsi.shape)
+    if sf == "linear":
+        np.testing.assert_array_equal(psi, opsi.astype(np.float32))
+        np.testing.assert_array_equal(dpsi, odpsi.astype(np.float32))
+    else:
+        np.testing.assert_allclose(psi, opsi, rtol=0, atol=2e-6)
+        np.testing.assert_allclose(dpsi, odpsi, rtol=0, atol=2e-6 / h)
+
+
+# --------------------------------------------------------------- reference goldens
+def _stepwise(dim, musl):
+    import importlib
+    import json
+    pm = importlib.import_module(f"pyroclastmpm_amd.MPM{dim}D")
+    G = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_unit_goldens.json")))
+    g = G["usl_stepwise"]
+    d = g[str(dim)]
+    pm.set_globals(g["dt"], g["ppc"], "linear", "output")
+    pm.set_global_step(0)
+    particles = pm.ParticlesContainer(np.array([d["pos"], d["pos"]]), np.ones((2, dim)))
+    nodes = pm.NodesContainer([0.0] * dim, [1.0] * dim, g["cell_size"])
+    cls = pm.MUSL if musl else pm.USL
+    solver = cls(particles, nodes, [pm.LinearElastic(1000, 0.1, 0.1)])
+    solver.particles.masses = g["masses"]
+    solver.particles.volumes = g["volumes"]
+    solver.particles.volumes_original = g["volumes"]
+    st = np.zeros((2, 3, 3))
+    st[0] = 1.0
+    solver.particles.stresses = st
+    solver.P2G()
+    return G, solver, d
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_usl_stepwise_golden(dim):
+    _banner(f"USL stepwise golden dim={dim}")
+    G, solver, d = _stepwise(dim, False)
+    tol = G["usl_stepwise"]["tol"]
+    np.testing.assert_allclose(solver.nodes.masses[:2], d["node_mass"], atol=tol)
+    np.testing.assert_allclose(solver.nodes.moments[:2], d["node_moment"], atol=tol)
+    solver.G2P()
+    p = solver.particles
+    np.testing.assert_allclose(p.volumes[0], d["volume"], atol=tol)
+    np.testing.assert_allclose(p.positions[0], d["position"], atol=tol)
+    np.testing.assert_allclose(p.velocities[0], d["velocity"], atol=tol)
+    np.testing.assert_allclose(np.diagonal(p.F[0]), d["F_diag"], atol=tol)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_musl_stepwise_golden(dim):
+    _banner(f"MUSL stepwise golden dim={dim}")
+    G, solver, _ = _stepwise(dim, True)
+    d = G["musl_solve"][str(dim)]
+    tol = G["musl_solve"]["tol"]
+    solver.G2P_double_mapping()
+    np.testing.assert_allclose(solver.particles.velocities[0], d["velocity"], atol=tol)
+    np.testing.assert_allclose(solver.particles.positions[0], d["position"], atol=tol)
+    solver.P2G_double_mapping()
+    np.testing.assert_allclose(solver.nodes.moments_nt[:2], d["node_moment_nt"], atol=tol)
+    solver.G2P()
+    np.testing.assert_allclose(solver.particles.volumes[0], d["volume"], atol=tol)
+    np.testing.assert_allclose(np.diagonal(solver.particles.F[0]), d["F_diag"], atol=tol)
+
+
+# --------------------------------------------------------------- full steps
+def _compare_run(oracle_mod, spec, steps, tol, label):
+    _banner(f"{label}: n={len(spec.positions)} steps={steps}")
+    pm, solver = S.build_product(spec)
+    sim = S.build_oracle(oracle_mod, spec, "f32")
+    solver.solve_nsteps(steps)
+    for _ in range(steps):
+        sim.step()
+    got, ref = S.product_state(solver), S.oracle_state(sim)
+    errs = {k: S.max_rel_err(got[k], ref[k]) for k in S.FIELDS}
+    print(f"[gpu-test] {label} field-scaled errors: {errs}", flush=True)
+    for k, e in errs.items():
+        assert e <= tol, (label, k, e)
+    return solver, sim
+
+
+def test_two_disc_usl_parity(oracle_mod):
+    _compare_run(oracle_mod, S.two_disc(), 100, 1e-4, "two-disc USL linear LE")
+
+
+def test_two_disc_musl_parity(oracle_mod):
+    _compare_run(oracle_mod, S.two_disc(musl=True), 100, 1e-4, "two-disc MUSL linear LE")
+
+
+def test_column_dp_musl_cubic_parity(oracle_mod):
+    spec = S.column(h=0.05, per_axis=3, dt=1e-4, domain=(1.0, 0.5, 0.5))
+    _compare_run(oracle_mod, spec, 10, 1e-4, "column DP MUSL cubic")
+
+
+def test_column_dp_usl_quadratic_parity(oracle_mod):
+    spec = S.column(h=0.05, per_axis=2, dt=1e-4, shape_function="quadratic", musl=False)
+    _compare_run(oracle_mod, spec, 10, 1e-4, "column DP USL quadratic")
+
+
+def test_scoop_mui_sdf_parity(oracle_mod):
+    spec = S.scoop(h=0.05, per_axis=2, dt=1e-4)
+    _compare_run(oracle_mod, spec, 5, 1e-4, "scoop mu(I) quadratic + SDF box")
+
+
+def test_newtonian_fluid_parity(oracle_mod):
+    pos = S.box_lattice([0.0, 0.0], [0.4, 0.6], 0.025)
+    spec = S.Spec(dim=2, origin=[0.0, 0.0], end=[1.0, 1.0], cell_size=0.05, shape_function="cubic", dt=1e-4, ppc=4,
+                  positions=pos,
+                  materials=[("NewtonFluid", {"density": 1000.0, "viscosity": 0.002, "bulk_modulus": 2e4,
+                                              "gamma": 7.0})],
+                  bcs=[("Gravity", {"gravity": [0.0, -9.81]}),
+                       ("NodeDomain", {"face0_mode": [1, 1], "face1_mode": [1, 1]})])
+    _compare_run(oracle_mod, spec, 20, 1e-4, "Newtonian dam break 2D cubic")
+
+
+def test_rigid_levelset_parity(oracle_mod):
+    disc = S.create_circle([0.5, 0.45], 0.2, 0.05, 4)
+    rod = np.stack([np.linspace(0.3, 0.7, 17), np.full(17, 0.7)], axis=1)
+    spec = S.Spec(dim=2, origin=[0.0, 0.0], end=[1.0, 1.0], cell_size=0.05, shape_function="linear", dt=1e-3, ppc=4,
+                  positions=disc, velocities=np.tile([0.0, 0.5], (len(disc), 1)), rigid_positions=rod,
+                  materials=[("LinearElastic", {"density": 1000.0, "E": 1000.0, "pois": 0.3})],
+                  bcs=[("RigidBodyLevelSet", {"mode": 1, "euler_angles_per_second": [0.5, 0.0],
+                                              "COM": [0.5, 0.7]})])
+    _compare_run(oracle_mod, spec, 20, 1e-4, "rigid level set 2D rotate")
+
+
+def test_vibrating_bar_1d_parity(oracle_mod):
+    L = 25.0
+    pos = (np.arange(0.5, 50.0) * 0.5)[:, None]
+    vel = (0.1 * np.sin(np.pi * pos[:, 0] / (2 * L)))[:, None]
+    spec = S.Spec(dim=1, origin=[0.0], end=[26.0], cell_size=1.0, shape_function="linear", dt=0.01, ppc=2,
+                  positions=pos, velocities=vel,
+                  materials=[("LinearElastic", {"density": 1.0, "E": 100.0, "pois": 0.0})],
+                  bcs=[("NodeDomain", {"face0_mode": [0], "face1_mode": [2]})])
+    _compare_run(oracle_mod, spec, 100, 1e-4, "1D bar")
+
+
+# --------------------------------------------------------------- acceptance
+def test_two_disc_energy_acceptance():
+    """test_two_disks.py:147,157: KE0 = 2.51 +- 10%, |dKE|/KE0 < 0.2 over the
+    reference run length (run(3000) = 3001 steps)."""
+    import json
+    G = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_unit_goldens.json")))
+    acc = G["two_disc_acceptance"]
+    spec = S.two_disc()
+    _banner("two-disc KE acceptance 3001 steps")
+    pm, solver = S.build_product(spec)
+    m = np.asarray(solver.particles.masses, dtype=np.float64)
+
+    def ke():
+        v = np.asarray(solver.particles.velocities, dtype=np.float64)
+        return float(np.sum(0.5 * m * np.sum(v * v, axis=1)))
+
+    ke0 = ke()
+    solver.run(3000, 100)
+    ke1 = ke()
+    assert abs(ke0 - acc["KE0"]) <= acc["KE0_rel_tol"] * acc["KE0"], ke0
+    assert abs(ke1 - ke0) / ke0 < acc["max_rel_KE_drift"], (ke0, ke1)
