@@ -147,9 +147,9 @@ class Workload:
         nb = self.prm.num_bcs
         return [
             ("partition", lambda: L.pmpm_partition(self.ctx.ptr, 3, g, a.x, self.n, sp, s)),
-            ("p2g", lambda: L.pmpm_p2g(prm, g, C.byref(a), sp, nd, s)),
+            ("p2g", lambda: L.pmpm_p2g(self.ctx.ptr, prm, g, C.byref(a), sp, nd, s)),
             ("grid_bc", lambda: L.pmpm_grid_bc(prm, g, nd, 0, nb, 0, s)),
-            ("musl_map", lambda: L.pmpm_musl_map(prm, g, C.byref(a), C.byref(b), sp, nd, s)),
+            ("musl_map", lambda: L.pmpm_musl_map(self.ctx.ptr, prm, g, C.byref(a), C.byref(b), sp, nd, s)),
             ("grid_bc_remap", lambda: L.pmpm_grid_bc(prm, g, nd, 0, nb, 1, s)),
             ("musl_g2p", lambda: L.pmpm_musl_g2p(prm, g, C.byref(a), C.byref(b), sp, nd, s)),
         ]

@@ -167,9 +167,11 @@ int pmpm_shape_function(int32_t dim, int32_t shape_fn, int32_t node_types_built,
 /* ---- P2G: replaces USL::P2G / usl_p2g_kernel (usl.cpp:118-161,
  * usl_inline.h:82-180). Clears the node arrays, scatters mass, momentum and
  * force, then writes mom_nt = mom + force*dt (nodes with mass <= cutoff are
- * left at 0). Needs a partition of the same positions. */
-int pmpm_p2g(const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles *p,
-             const pmpm_spatial *part, pmpm_nodes *nodes, pmpm_stream_t stream);
+ * left at 0). Needs a partition of the same positions. The context holds the
+ * per-tile partial sums (allocated on the first call for a larger grid). */
+int pmpm_p2g(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid,
+             const pmpm_particles *p, const pmpm_spatial *part, pmpm_nodes *nodes,
+             pmpm_stream_t stream);
 
 /* ---- node boundary conditions: replaces the apply_on_nodes_moments hooks of
  * Gravity, NodeDomain (gravity.cpp:65-88, nodedomain.cpp:42-64) and the
@@ -219,10 +221,11 @@ int pmpm_g2p(const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles
 /* ---- MUSL (no reference implementation; semantics of test_musl.h:37-156,
  * musl.cuh:39-72). First map + re-map, fused: velocities and positions from
  * the grid, then mom_nt := sum N m v with the new velocities (mass cutoff as
- * P2G). Writes x, v of particle j to slot j of `out` (out != in required). */
-int pmpm_musl_map(const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles *in,
-                  pmpm_particles *out, const pmpm_spatial *part, pmpm_nodes *nodes,
-                  pmpm_stream_t stream);
+ * P2G) and the node velocities from it. Writes x, v of particle j to slot j
+ * of `out` (out != in required). */
+int pmpm_musl_map(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid,
+                  const pmpm_particles *in, pmpm_particles *out, const pmpm_spatial *part,
+                  pmpm_nodes *nodes, pmpm_stream_t stream);
 /* Second map: velocity gradient, F, volume from the re-mapped moments, then
  * the return maps (weights from in->x, the positions the step started from);
  * slot placement as pmpm_g2p. */

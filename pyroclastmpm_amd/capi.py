@@ -71,14 +71,14 @@ PROTOTYPES = {
     "pmpm_context_out_of_grid": (C.c_int, [vp, vp, P(C.c_int32)]),
     "pmpm_partition": (C.c_int, [vp, C.c_int32, P(Grid), vp, C.c_int32, P(Spatial), vp]),
     "pmpm_shape_function": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, P(Grid), vp, C.c_int32, vp, vp, vp]),
-    "pmpm_p2g": (C.c_int, [P(Params), P(Grid), P(Particles), P(Spatial), P(Nodes), vp]),
+    "pmpm_p2g": (C.c_int, [vp, P(Params), P(Grid), P(Particles), P(Spatial), P(Nodes), vp]),
     "pmpm_grid_bc": (C.c_int, [P(Params), P(Grid), P(Nodes), C.c_int32, C.c_int32, C.c_int32, vp]),
     "pmpm_rigid_levelset": (C.c_int, [P(Params), P(Grid), P(Particles), P(Spatial), P(Nodes), P(C.c_float),
                                       P(C.c_float), vp, vp, vp]),
     "pmpm_rigid_advance": (C.c_int, [C.c_int32, C.c_int32, vp, vp, vp, vp]),
     "pmpm_permute": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, C.c_int32, vp]),
     "pmpm_g2p": (C.c_int, [P(Params), P(Grid), P(Particles), P(Particles), P(Spatial), P(Nodes), vp]),
-    "pmpm_musl_map": (C.c_int, [P(Params), P(Grid), P(Particles), P(Particles), P(Spatial), P(Nodes), vp]),
+    "pmpm_musl_map": (C.c_int, [vp, P(Params), P(Grid), P(Particles), P(Particles), P(Spatial), P(Nodes), vp]),
     "pmpm_musl_g2p": (C.c_int, [P(Params), P(Grid), P(Particles), P(Particles), P(Spatial), P(Nodes), vp]),
     "pmpm_stress_update": (C.c_int, [P(Params), C.c_int32, P(Particles), vp]),
     "pmpm_step": (C.c_int, [vp, P(Params), P(Grid), C.c_int32, P(Particles), P(Particles), P(Spatial), P(Nodes),

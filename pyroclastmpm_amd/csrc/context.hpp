@@ -16,7 +16,19 @@ struct pmpm_context {
   uint32_t *hist = nullptr;      // [RADIX][blocks] digit histograms
   size_t hist_len = 0;
   uint32_t *counters = nullptr;  // [4]: out-of-grid particles, ...
+  // P2G tile partials: per tile, the LDS node block of each accumulated field
+  // (plain stores; the grid kernel sums the <= 2^D tiles covering a node)
+  float *partials = nullptr;
+  size_t partials_cap = 0;       // floats
+  int32_t *tile_flags = nullptr;  // 1 = tile had particles this launch
+  size_t tile_flags_cap = 0;
 };
+
+namespace pmpm {
+// Grows the tile-partial workspace (hipMalloc on first use of a larger grid;
+// run one step before capturing a graph).
+int ensure_partials(pmpm_context *ctx, size_t floats, size_t tiles);
+}  // namespace pmpm
 
 namespace pmpm {
 

@@ -50,7 +50,7 @@ __device__ inline void cubic_eval(float r, int type, float inv_h, float &N, floa
 
 __device__ inline void linear_eval(float r, float inv_h, float &N, float &dN) {
   const float a = fabsf(r);
-  N = (a >= 1.f) ? 0.f : 1.f - a;
+  N = (a >= 1.f) ? 0.f : __fsub_rn(1.f, a);
   dN = (r > 1.f || r < -1.f || r == 0.f) ? 0.f : (r > 0.f ? -inv_h : inv_h);
 }
 
@@ -70,6 +70,15 @@ template <int SF> __device__ inline void spline_eval(float r, int type, float in
   else cubic_eval(r, type, inv_h, N, dN);
 }
 
+// t = (x - origin) * inv_h rounded after each operation, exactly as the
+// reference evaluates it (no fused multiply-add: the cell index and the
+// relative coordinates must come out bit-identical).
+__device__ inline float cell_coord(float x, float origin, float inv_h) {
+  // the product must not fuse with a later subtraction of the cell index
+#pragma clang fp contract(off)
+  return (x - origin) * inv_h;
+}
+
 // Per-axis weights of one particle. t = (x - origin) * inv_h, bin = trunc(t).
 // first = offset (relative to bin) of entry 0; N[e], dN[e] for the ENTRIES
 // nodes bin+first+e. The relative coordinate (t - bin) - o equals the
@@ -77,21 +86,21 @@ template <int SF> __device__ inline void spline_eval(float r, int type, float in
 template <int SF>
 __device__ inline void axis_weights(float t, int bin, int n_axis, int types_built, float inv_h,
                                     int &first, float *N, float *dN) {
-  const float frac = t - (float)bin;
+  const float frac = __fsub_rn(t, (float)bin);
   if (SF == PMPM_SF_LINEAR) {
     first = 0;
 #pragma unroll
-    for (int e = 0; e < 2; ++e) linear_eval(frac - (float)e, inv_h, N[e], dN[e]);
+    for (int e = 0; e < 2; ++e) linear_eval(__fsub_rn(frac, (float)e), inv_h, N[e], dN[e]);
   } else if (SF == PMPM_SF_QUADRATIC) {
     first = (frac < 0.5f) ? -1 : 0;
 #pragma unroll
-    for (int e = 0; e < 3; ++e) quad_eval(frac - (float)(first + e), inv_h, N[e], dN[e]);
+    for (int e = 0; e < 3; ++e) quad_eval(__fsub_rn(frac, (float)(first + e)), inv_h, N[e], dN[e]);
   } else {
     first = -1;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int o = e - 1;
-      cubic_eval(frac - (float)o, cubic_node_type(bin + o, n_axis, types_built), inv_h, N[e], dN[e]);
+      cubic_eval(__fsub_rn(frac, (float)o), cubic_node_type(bin + o, n_axis, types_built), inv_h, N[e], dN[e]);
     }
   }
 }

@@ -699,7 +699,7 @@ template <int D> void Solver<D>::step_composed() {
   pmpm_spatial sp = particles.spatial_view(particles.work);
   pmpm_nodes nv = nodes.view();
   check(pmpm_partition(ctx, D, &nodes.grid, a.x, N, &sp, stream), "pmpm_partition");
-  check(pmpm_p2g(&prm, &nodes.grid, &a, &sp, &nv, stream), "pmpm_p2g");
+  check(pmpm_p2g(ctx, &prm, &nodes.grid, &a, &sp, &nv, stream), "pmpm_p2g");
   DevBuf<float> rigid_dx(N * D);
   int begin = 0;
   for (int k = 0; k < prm.num_bcs; ++k) {
@@ -725,7 +725,7 @@ template <int D> void Solver<D>::step_composed() {
   if (!musl) {
     check(pmpm_g2p(&prm, &nodes.grid, &a, &b, &sp, &nv, stream), "pmpm_g2p");
   } else {
-    check(pmpm_musl_map(&prm, &nodes.grid, &a, &b, &sp, &nv, stream), "pmpm_musl_map");
+    check(pmpm_musl_map(ctx, &prm, &nodes.grid, &a, &b, &sp, &nv, stream), "pmpm_musl_map");
     check(pmpm_grid_bc(&prm, &nodes.grid, &nv, 0, prm.num_bcs, 1, stream), "pmpm_grid_bc");
     check(pmpm_musl_g2p(&prm, &nodes.grid, &a, &b, &sp, &nv, stream), "pmpm_musl_g2p");
   }
@@ -770,11 +770,12 @@ template <int D> void Solver<D>::partition() {
 }
 template <int D> void Solver<D>::P2G() {
   partition();
+  pmpm_context *ctx = particles.ctx();
   pmpm_params prm = params();
   pmpm_particles a = particles.view(particles.cur_index());
   pmpm_spatial sp = particles.spatial_view(particles.work);
   pmpm_nodes nv = nodes.view();
-  check(pmpm_p2g(&prm, &nodes.grid, &a, &sp, &nv, stream), "pmpm_p2g");
+  check(pmpm_p2g(ctx, &prm, &nodes.grid, &a, &sp, &nv, stream), "pmpm_p2g");
   synchronize();
 }
 template <int D> void Solver<D>::apply_boundary_conditions(bool remap_pass) {
@@ -799,6 +800,7 @@ template <int D> void Solver<D>::G2P() {
 // positions of the step start are kept for the second map's weights.
 template <int D> void Solver<D>::G2P_double_mapping() {
   const int N = particles.num_particles;
+  pmpm_context *ctx = particles.ctx();
   pmpm_params prm = params();
   particles.ensure_second_set();
   pmpm_particles a = particles.view(particles.cur_index());
@@ -806,7 +808,7 @@ template <int D> void Solver<D>::G2P_double_mapping() {
   pmpm_spatial sp = particles.spatial_view(particles.work);
   pmpm_nodes nv = nodes.view();
   check(pmpm_grid_bc(&prm, &nodes.grid, &nv, 0, 0, 0, stream), "pmpm_grid_bc");
-  check(pmpm_musl_map(&prm, &nodes.grid, &a, &b, &sp, &nv, stream), "pmpm_musl_map");
+  check(pmpm_musl_map(ctx, &prm, &nodes.grid, &a, &b, &sp, &nv, stream), "pmpm_musl_map");
   x_orig_.resize((size_t)N * D);
   hip_check(hipMemcpyAsync(x_orig_.get(), a.x, (size_t)N * D * 4, hipMemcpyDeviceToDevice, stream), "copy");
   // written in cell order j: scatter back to the slots

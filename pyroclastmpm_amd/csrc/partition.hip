@@ -36,7 +36,7 @@ __global__ void hash_kernel(const float *__restrict__ x, int n, pmpm_grid g, uin
   int b[3] = {0, 0, 0};
 #pragma unroll
   for (int a = 0; a < D; ++a) {
-    const float t = (x[a * n + i] - g.origin[a]) * g.inv_cell_size;
+    const float t = __fmul_rn(__fsub_rn(x[a * n + i], g.origin[a]), g.inv_cell_size);
     b[a] = (int)t;
   }
   int64_t h = b[0];
@@ -73,30 +73,33 @@ __global__ __launch_bounds__(kSortBlock) void radix_hist_kernel(const uint32_t *
   hist[(int64_t)t * nblocks + blockIdx.x] = h[t];
 }
 
-// Exclusive scan of len entries with one workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void scan_kernel(uint32_t *__restrict__ data, int64_t len) {
-  __shared__ uint32_t part[1024];
+// Exclusive scan of each digit's column hist[d][0..nblocks) (one workgroup
+// per digit); the column total goes to totals[d].
+__global__ __launch_bounds__(256) void scan_columns_kernel(uint32_t *__restrict__ hist, int nblocks,
+                                                           uint32_t *__restrict__ totals) {
+  __shared__ uint32_t part[256];
   const int t = threadIdx.x;
-  const int64_t chunk = (len + 1023) / 1024;
-  const int64_t b = t * chunk;
-  const int64_t e = (b + chunk < len) ? b + chunk : len;
+  uint32_t *col = hist + (int64_t)blockIdx.x * nblocks;
+  const int chunk = (nblocks + 255) / 256;
+  const int b = t * chunk;
+  const int e = min(b + chunk, nblocks);
   uint32_t s = 0;
-  for (int64_t i = b; i < e; ++i) s += data[i];
+  for (int i = b; i < e; ++i) s += col[i];
   part[t] = s;
   __syncthreads();
-  // Hillis-Steele inclusive scan of the 1024 partial sums
-  for (int off = 1; off < 1024; off <<= 1) {
+  for (int off = 1; off < 256; off <<= 1) {
     const uint32_t v = (t >= off) ? part[t - off] : 0u;
     __syncthreads();
     part[t] += v;
     __syncthreads();
   }
   uint32_t run = (t > 0) ? part[t - 1] : 0u;
-  for (int64_t i = b; i < e; ++i) {
-    const uint32_t v = data[i];
-    data[i] = run;
+  for (int i = b; i < e; ++i) {
+    const uint32_t v = col[i];
+    col[i] = run;
     run += v;
   }
+  if (t == 255) totals[blockIdx.x] = part[255];
 }
 
 // Stable scatter of one radix pass. Within a round of 256 keys (thread order
@@ -104,14 +107,26 @@ __global__ __launch_bounds__(1024) void scan_kernel(uint32_t *__restrict__ data,
 // 8 ballots; per-wave counts are combined per digit in LDS.
 __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
     const uint32_t *__restrict__ keys_in, const int32_t *__restrict__ vals_in, uint32_t *__restrict__ keys_out,
-    int32_t *__restrict__ vals_out, int n, int shift, const uint32_t *__restrict__ hist, int nblocks) {
+    int32_t *__restrict__ vals_out, int n, int shift, const uint32_t *__restrict__ hist, int nblocks,
+    const uint32_t *__restrict__ totals) {
   __shared__ uint32_t base[kRadix];
   __shared__ uint32_t wcnt[kSortBlock / 64][kRadix];
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int w = t >> 6;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  base[t] = hist[(int64_t)t * nblocks + blockIdx.x];
+  // digit bases: exclusive scan of the 256 column totals
+  base[t] = totals[t];
+  __syncthreads();
+  for (int off = 1; off < kRadix; off <<= 1) {
+    const uint32_t v = (t >= off) ? base[t - off] : 0u;
+    __syncthreads();
+    base[t] += v;
+    __syncthreads();
+  }
+  const uint32_t digit_base = (t > 0) ? base[t - 1] : 0u;
+  __syncthreads();
+  base[t] = digit_base + hist[(int64_t)t * nblocks + blockIdx.x];
   const int64_t tile = (int64_t)blockIdx.x * kSortTile;
   for (int r = 0; r < kSortItems; ++r) {
 #pragma unroll
@@ -178,8 +193,28 @@ extern "C" int pmpm_context_create(pmpm_context **ctx) {
   return PMPM_OK;
 }
 
+int pmpm::ensure_partials(pmpm_context *ctx, size_t floats, size_t tiles) {
+  if (floats > ctx->partials_cap) {
+    (void)hipFree(ctx->partials);
+    ctx->partials = nullptr;
+    ctx->partials_cap = 0;
+    PMPM_HIP_CHECK(hipMalloc(&ctx->partials, floats * sizeof(float)));
+    ctx->partials_cap = floats;
+  }
+  if (tiles > ctx->tile_flags_cap) {
+    (void)hipFree(ctx->tile_flags);
+    ctx->tile_flags = nullptr;
+    ctx->tile_flags_cap = 0;
+    PMPM_HIP_CHECK(hipMalloc(&ctx->tile_flags, tiles * sizeof(int32_t)));
+    ctx->tile_flags_cap = tiles;
+  }
+  return PMPM_OK;
+}
+
 extern "C" int pmpm_context_destroy(pmpm_context *ctx) {
   if (!ctx) return PMPM_OK;
+  (void)hipFree(ctx->partials);
+  (void)hipFree(ctx->tile_flags);
   (void)hipFree(ctx->keys_tmp);
   (void)hipFree(ctx->vals_tmp);
   (void)hipFree(ctx->hist);
@@ -197,7 +232,7 @@ extern "C" int pmpm_context_reserve(pmpm_context *ctx, int32_t max_particles, in
     ctx->keys_tmp = nullptr; ctx->vals_tmp = nullptr; ctx->hist = nullptr; ctx->cap_particles = 0;
     const size_t np = (size_t)(max_particles > 0 ? max_particles : 1);
     const size_t nblocks = (size_t)ceil_div((int64_t)np, kSortTile);
-    ctx->hist_len = (size_t)kRadix * nblocks;
+    ctx->hist_len = (size_t)kRadix * nblocks + kRadix;  // + digit totals
     PMPM_HIP_CHECK(hipMalloc(&ctx->keys_tmp, np * sizeof(uint32_t)));
     PMPM_HIP_CHECK(hipMalloc(&ctx->vals_tmp, np * sizeof(int32_t)));
     PMPM_HIP_CHECK(hipMalloc(&ctx->hist, ctx->hist_len * sizeof(uint32_t)));
@@ -251,10 +286,11 @@ extern "C" int pmpm_partition(pmpm_context *ctx, int32_t dim, const pmpm_grid *g
     const int shift = p * kRadixBits;
     hipLaunchKernelGGL(radix_hist_kernel, dim3(nblocks), dim3(kSortBlock), 0, s, kin, n, shift, ctx->hist, nblocks);
     PMPM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, ctx->hist, (int64_t)kRadix * nblocks);
+    uint32_t *totals = ctx->hist + (size_t)kRadix * nblocks;
+    hipLaunchKernelGGL(scan_columns_kernel, dim3(kRadix), dim3(256), 0, s, ctx->hist, nblocks, totals);
     PMPM_LAUNCH_CHECK();
     hipLaunchKernelGGL(radix_scatter_kernel, dim3(nblocks), dim3(kSortBlock), 0, s, kin, vin, kout, vout, n, shift,
-                       ctx->hist, nblocks);
+                       ctx->hist, nblocks, totals);
     PMPM_LAUNCH_CHECK();
     kin = kout;
     vin = vout;

@@ -7,14 +7,14 @@
 #include "context.hpp"
 
 namespace pmpm {
-int do_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &p, const pmpm_spatial &part,
-           const pmpm_nodes &nodes, hipStream_t s);
+int do_p2g(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &p,
+           const pmpm_spatial &part, const pmpm_nodes &nodes, hipStream_t s);
 int do_g2p(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
            const pmpm_spatial &part, const pmpm_nodes &nodes, int mode, hipStream_t s);
-int do_musl_map(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
-                const pmpm_spatial &part, const pmpm_nodes &nodes, hipStream_t s);
-int launch_grid(const pmpm_params &prm, const pmpm_grid &g, const pmpm_nodes &nodes, int finalize, int b0, int b1,
-                int remap, hipStream_t s);
+int do_musl_map(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in,
+                const pmpm_particles &out, const pmpm_spatial &part, const pmpm_nodes &nodes, hipStream_t s);
+int launch_grid(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const pmpm_nodes &nodes, int finalize,
+                int b0, int b1, int remap, int gather, hipStream_t s);
 }  // namespace pmpm
 
 using namespace pmpm;
@@ -29,11 +29,11 @@ extern "C" int pmpm_step(pmpm_context *ctx, const pmpm_params *prm, const pmpm_g
   hipStream_t s = (hipStream_t)stream;
   int rc = pmpm_partition(ctx, prm->dim, grid, a->x, a->n, part, stream);
   if (rc != PMPM_OK) return rc;
-  if ((rc = do_p2g(*prm, *grid, *a, *part, *nodes, s)) != PMPM_OK) return rc;
-  if ((rc = launch_grid(*prm, *grid, *nodes, 1, 0, prm->num_bcs, 0, s)) != PMPM_OK) return rc;
+  if ((rc = do_p2g(ctx, *prm, *grid, *a, *part, *nodes, s)) != PMPM_OK) return rc;
+  if ((rc = launch_grid(ctx, *prm, *grid, *nodes, 1, 0, prm->num_bcs, 0, 1, s)) != PMPM_OK) return rc;
   if (!musl) return do_g2p(*prm, *grid, *a, *b, *part, *nodes, 0, s);
-  if ((rc = do_musl_map(*prm, *grid, *a, *b, *part, *nodes, s)) != PMPM_OK) return rc;
-  if ((rc = launch_grid(*prm, *grid, *nodes, 0, 0, prm->num_bcs, 1, s)) != PMPM_OK) return rc;
+  if ((rc = do_musl_map(ctx, *prm, *grid, *a, *b, *part, *nodes, s)) != PMPM_OK) return rc;
+  if ((rc = launch_grid(ctx, *prm, *grid, *nodes, 0, 0, prm->num_bcs, 1, 2, s)) != PMPM_OK) return rc;
   return do_g2p(*prm, *grid, *a, *b, *part, *nodes, 2, s);
 }
 

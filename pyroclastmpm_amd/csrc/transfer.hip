@@ -22,10 +22,13 @@
 
 namespace pmpm {
 
+// Cell tile of one P2G workgroup (128 cells) and its thread count: every cell
+// is shared by kTileThreads / 128 = 8 waves ("rounds").
+constexpr int kTileThreads = 1024;
 template <int D> struct Tile;
-template <> struct Tile<3> { static constexpr int X = 16, Y = 4, Z = 4; };
-template <> struct Tile<2> { static constexpr int X = 16, Y = 16, Z = 1; };
-template <> struct Tile<1> { static constexpr int X = 256, Y = 1, Z = 1; };
+template <> struct Tile<3> { static constexpr int X = 16, Y = 4, Z = 2; };
+template <> struct Tile<2> { static constexpr int X = 16, Y = 8, Z = 1; };
+template <> struct Tile<1> { static constexpr int X = 128, Y = 1, Z = 1; };
 
 __device__ inline int64_t lin_index(const pmpm_grid &g, const int *b, int D) {
   int64_t h = b[0];
@@ -51,7 +54,7 @@ __device__ inline void particle_weights(const pmpm_params &prm, const pmpm_grid 
   for (int a = 0; a < 3; ++a) {
     if (a < D) {
       xp[a] = x[a * n + slot];
-      w.t[a] = (xp[a] - g.origin[a]) * g.inv_cell_size;
+      w.t[a] = cell_coord(xp[a], g.origin[a], g.inv_cell_size);
       w.bin[a] = (int)w.t[a];
       axis_weights<SF>(w.t[a], w.bin[a], g.num_cells[a], prm.node_types_built, g.inv_cell_size, w.first[a], w.N[a],
                        w.dN[a]);
@@ -123,8 +126,8 @@ __device__ inline void entry_weight(const PW<D, SF> &w, const int *off, const in
   float N[3] = {1.f, 1.f, 1.f}, dN[3] = {0.f, 0.f, 0.f};
 #pragma unroll
   for (int a = 0; a < D; ++a) {
-    const float frac = w.t[a] - (float)w.bin[a];
-    spline_eval<SF>(frac - (float)off[a], types[a], inv_h, N[a], dN[a]);
+    const float frac = __fsub_rn(w.t[a], (float)w.bin[a]);
+    spline_eval<SF>(__fsub_rn(frac, (float)off[a]), types[a], inv_h, N[a], dN[a]);
   }
   wgt = (N[0] * N[1]) * N[2];
   gr[0] = (dN[0] * N[1]) * N[2];
@@ -186,19 +189,40 @@ __device__ inline void scatter_entry_global(const pmpm_params &prm, const pmpm_g
   }
 }
 
+// LDS row pitch: at least the tile's node extent and 16 mod 32, so the two
+// cell rows of one half-wave (16 lanes each) fall into opposite bank halves.
+__host__ __device__ constexpr int lds_pitch(int nx, int rows_per_wave) {
+  return rows_per_wave <= 1 ? nx : ((nx + 15) / 32) * 32 + 16;
+}
+
 // MODE 0: P2G of mass, momentum and force (force accumulates in mom_nt).
 // MODE 1: MUSL first map (x, v of particle j written to slot j of `out`) and
 //         re-map of momentum into mom_nt.
+// Thread t of a workgroup owns cell (t % CELLS) of the tile and walks the
+// particles k = t / CELLS, t / CELLS + ROUNDS, ... of that cell, so every wave
+// holds 64 distinct cells (conflict-free LDS atomics at a fixed window offset)
+// and a cell's particles spread over ROUNDS waves.
+// The finished LDS block is written with plain stores to this tile's slice of
+// `partials` (in-grid nodes; flags[tile] says whether the slice is live) and
+// the grid kernel sums the <= 2^D slices covering each node. LDS positions
+// outside the grid hold the reference's linear-index wrap-around deposits
+// (weighted with the wrapped node's types); they are flushed with one global
+// atomic per tile and field into the node arrays.
 template <int D, int SF, int MODE>
-__global__ __launch_bounds__(256) void p2g_tile_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles in,
-                                                       pmpm_particles out, pmpm_spatial part, pmpm_nodes nodes,
-                                                       int tiles_x, int tiles_y) {
+__global__ __launch_bounds__(kTileThreads) void p2g_tile_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles in,
+                                                                pmpm_particles out, pmpm_spatial part,
+                                                                pmpm_nodes nodes, int tiles_x, int tiles_y,
+                                                                float *__restrict__ partials,
+                                                                int32_t *__restrict__ flags) {
   using SP = Spline<SF>;
-  constexpr int TX = Tile<D>::X, TY = Tile<D>::Y, TZ = Tile<D>::Z;
-  constexpr int NX = TX + SP::SPAN - 1;
-  constexpr int NY = D > 1 ? TY + SP::SPAN - 1 : 1;
-  constexpr int NZ = D > 2 ? TZ + SP::SPAN - 1 : 1;
-  constexpr int NT = NX * NY * NZ;
+  using T = Tile<D>;
+  constexpr int CELLS = T::X * T::Y * T::Z;
+  constexpr int ROUNDS = kTileThreads / CELLS;
+  constexpr int NX = T::X + SP::SPAN - 1;
+  constexpr int NY = D > 1 ? T::Y + SP::SPAN - 1 : 1;
+  constexpr int NZ = D > 2 ? T::Z + SP::SPAN - 1 : 1;
+  constexpr int NXP = lds_pitch(NX, D > 1 ? 64 / T::X : 1);
+  constexpr int NT = NXP * NY * NZ;  // LDS nodes incl. padding
   constexpr int NV = MODE == 0 ? 1 + 2 * D : D;
   constexpr int E = SP::ENTRIES;
   constexpr int EY = D > 1 ? E : 1;
@@ -208,13 +232,15 @@ __global__ __launch_bounds__(256) void p2g_tile_kernel(pmpm_params prm, pmpm_gri
   const int n = in.n;
   const int nn = g.num_cells_total;
   const int t = threadIdx.x;
+  const int cl = t % CELLS;
+  const int round = t / CELLS;
   const int bx = blockIdx.x % tiles_x;
   const int by = (blockIdx.x / tiles_x) % tiles_y;
   const int bz = blockIdx.x / (tiles_x * tiles_y);
-  const int c[3] = {bx * TX + t % TX, by * TY + (t / TX) % TY, bz * TZ + t / (TX * TY)};
-  const int n0[3] = {bx * TX + SP::NLO, D > 1 ? by * TY + SP::NLO : 0, D > 2 ? bz * TZ + SP::NLO : 0};
+  const int c[3] = {bx * T::X + cl % T::X, by * T::Y + (cl / T::X) % T::Y, bz * T::Z + cl / (T::X * T::Y)};
+  const int n0[3] = {bx * T::X + SP::NLO, D > 1 ? by * T::Y + SP::NLO : 0, D > 2 ? bz * T::Z + SP::NLO : 0};
 
-  for (int i = t; i < NV * NT; i += 256) tile[i] = 0.f;
+  for (int i = t; i < NV * NT; i += kTileThreads) tile[i] = 0.f;
 
   bool in_grid = c[0] < g.num_cells[0];
   if (D > 1) in_grid = in_grid && c[1] < g.num_cells[1];
@@ -225,9 +251,12 @@ __global__ __launch_bounds__(256) void p2g_tile_kernel(pmpm_params prm, pmpm_gri
     ce = part.cell_end[cell];
     cs = part.cell_start[cell];
   }
-  if (!__syncthreads_or(ce > cs)) return;
+  if (!__syncthreads_or(ce - cs > round)) {
+    if (t == 0) flags[blockIdx.x] = 0;
+    return;
+  }
 
-  for (int j = cs; j < ce; ++j) {
+  for (int j = cs + round; j < ce; j += ROUNDS) {
     const int slot = part.sorted[j];
     PW<D, SF> w;
     float xp[3];
@@ -256,7 +285,14 @@ __global__ __launch_bounds__(256) void p2g_tile_kernel(pmpm_params prm, pmpm_gri
         mv[a] = m * vn;
       }
     }
-    const bool own_cell = (w.bin[0] == c[0]) && (D < 2 || w.bin[1] == c[1]) && (D < 3 || w.bin[2] == c[2]);
+    bool fast = true;  // own cell and the whole window inside the grid
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      fast = fast && w.bin[a] == c[a] && w.bin[a] + w.first[a] >= 0 && w.bin[a] + w.first[a] + E - 1 < g.num_cells[a];
+    int base = w.bin[0] + w.first[0] - n0[0];
+    if (D > 1) base += NXP * (w.bin[1] + w.first[1] - n0[1]);
+    if (D > 2) base += NXP * NY * (w.bin[2] + w.first[2] - n0[2]);
+    const bool own = (w.bin[0] == c[0]) && (D < 2 || w.bin[1] == c[1]) && (D < 3 || w.bin[2] == c[2]);
 #pragma unroll
     for (int ey = 0; ey < EY; ++ey) {
 #pragma unroll
@@ -266,34 +302,48 @@ __global__ __launch_bounds__(256) void p2g_tile_kernel(pmpm_params prm, pmpm_gri
         const float dyx = w.dN[1][ey] * w.N[0][ex];
 #pragma unroll
         for (int ez = 0; ez < EZ; ++ez) {
-          const int e[3] = {ex, ey, ez};
-          int nb[3] = {0, 0, 0};
-          bool inside = true;
+          const int li = base + ex + NXP * ey + NXP * NY * ez;
+          float wgt = nxy * w.N[2][ez];
+          float g0 = dxy * w.N[2][ez];
+          float g1 = dyx * w.N[2][ez];
+          float g2 = w.dN[2][ez] * nxy;
+          if (!fast) {
+            const int e3[3] = {ex, ey, ez};
+            if (!own) {  // particle outside its listed cell: straight to global memory
+              scatter_entry_global<D, SF, MODE>(prm, g, nodes, nn, w, e3, m, mv, A);
+              continue;
+            }
+            int nb[3] = {0, 0, 0}, off[3] = {0, 0, 0};
+            bool inside = true;
 #pragma unroll
-          for (int a = 0; a < D; ++a) {
-            nb[a] = w.bin[a] + w.first[a] + e[a];
-            inside = inside && nb[a] >= 0 && nb[a] < g.num_cells[a];
+            for (int a = 0; a < D; ++a) {
+              off[a] = w.first[a] + e3[a];
+              nb[a] = w.bin[a] + off[a];
+              inside = inside && nb[a] >= 0 && nb[a] < g.num_cells[a];
+            }
+            if (!inside) {
+              // the reference bound-checks only the linear node index
+              const int64_t lin = lin_index(g, nb, D);
+              if (!prm.wrap_quirk || lin < 0 || lin >= nn) continue;
+              int ids[3], types[3] = {0, 0, 0};
+              node_ids<D>(g, lin, ids);
+#pragma unroll
+              for (int a = 0; a < D; ++a) types[a] = cubic_node_type(ids[a], g.num_cells[a], prm.node_types_built);
+              float gr[3];
+              entry_weight<D, SF>(w, off, types, g.inv_cell_size, wgt, gr);
+              g0 = gr[0];
+              g1 = gr[1];
+              g2 = gr[2];
+            }
           }
-          if (!(inside && own_cell)) {
-            scatter_entry_global<D, SF, MODE>(prm, g, nodes, nn, w, e, m, mv, A);
-            continue;
-          }
-          int li = nb[0] - n0[0];
-          if (D > 1) li += NX * (nb[1] - n0[1]);
-          if (D > 2) li += NX * NY * (nb[2] - n0[2]);
-          const float wgt = nxy * w.N[2][ez];
           if (MODE == 0) {
-            float gr[3];
-            gr[0] = dxy * w.N[2][ez];
-            gr[1] = dyx * w.N[2][ez];
-            gr[2] = (w.dN[2][ez] * w.N[0][ex]) * w.N[1][ey];
             atomicAdd(&tile[li], wgt * m);
 #pragma unroll
             for (int a = 0; a < D; ++a) {
               atomicAdd(&tile[(1 + a) * NT + li], wgt * mv[a]);
-              float f = 0.f;
-#pragma unroll
-              for (int b = 0; b < D; ++b) f += A[a][b] * gr[b];
+              float f = A[a][0] * g0;
+              if (D > 1) f += A[a][1] * g1;
+              if (D > 2) f += A[a][2] * g2;
               atomicAdd(&tile[(1 + D + a) * NT + li], f);
             }
           } else {
@@ -305,33 +355,85 @@ __global__ __launch_bounds__(256) void p2g_tile_kernel(pmpm_params prm, pmpm_gri
     }
   }
   __syncthreads();
-  for (int i = t; i < NT; i += 256) {
+  constexpr int NTN = NX * NY * NZ;
+  float *slice = partials + (int64_t)blockIdx.x * NV * NTN;
+  if (t == 0) flags[blockIdx.x] = 1;
+  for (int i = t; i < NTN; i += kTileThreads) {
     const int ix = i % NX;
     const int iy = (i / NX) % NY;
     const int iz = i / (NX * NY);
     const int nb[3] = {n0[0] + ix, n0[1] + iy, n0[2] + iz};
-    if (nb[0] < 0 || nb[0] >= g.num_cells[0]) continue;
-    if (D > 1 && (nb[1] < 0 || nb[1] >= g.num_cells[1])) continue;
-    if (D > 2 && (nb[2] < 0 || nb[2] >= g.num_cells[2])) continue;
-    const int node = (int)lin_index(g, nb, D);
+    const int li = ix + NXP * (iy + NY * iz);
+    bool inside = nb[0] >= 0 && nb[0] < g.num_cells[0];
+    if (D > 1) inside = inside && nb[1] >= 0 && nb[1] < g.num_cells[1];
+    if (D > 2) inside = inside && nb[2] >= 0 && nb[2] < g.num_cells[2];
+    if (inside) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) slice[v * NTN + i] = tile[v * NT + li];
+      continue;
+    }
+    // wrap-around deposits, reduced over the tile: one atomic per field
+    const int64_t lin = lin_index(g, nb, D);
+    if (lin < 0 || lin >= nn) continue;
+    const int node = (int)lin;
     if (MODE == 0) {
-      const float mm = tile[i];
+      const float mm = tile[li];
       if (mm != 0.f) unsafeAtomicAdd(&nodes.mass[node], mm);
 #pragma unroll
       for (int a = 0; a < D; ++a) {
-        const float pm = tile[(1 + a) * NT + i];
+        const float pm = tile[(1 + a) * NT + li];
         if (pm != 0.f) unsafeAtomicAdd(&nodes.mom[a * nn + node], pm);
-        const float f = tile[(1 + D + a) * NT + i];
+        const float f = tile[(1 + D + a) * NT + li];
         if (f != 0.f) unsafeAtomicAdd(&nodes.mom_nt[a * nn + node], f);
       }
     } else {
 #pragma unroll
       for (int a = 0; a < D; ++a) {
-        const float pm = tile[a * NT + i];
+        const float pm = tile[a * NT + li];
         if (pm != 0.f) unsafeAtomicAdd(&nodes.mom_nt[a * nn + node], pm);
       }
     }
   }
+}
+
+// Geometry of the P2G tiles for the grid kernel's gather.
+struct TileGeom {
+  int tiles[3];  // tiles per axis
+  int T[3];      // cells per tile per axis
+  int N[3];      // nodes per tile block per axis (T + SPAN - 1)
+  int nlo;       // first node offset of a tile block
+  int nv;        // fields per node in the partials
+  int gather;    // 0: none, 1: P2G (mass, momentum, force), 2: MUSL re-map (momentum)
+};
+
+// Sums the partial slices of the (at most 2 per axis) tiles whose node block
+// covers node `ids`, in a fixed tile order.
+template <int D>
+__device__ inline void gather_partials(const TileGeom &tg, const float *partials, const int32_t *flags,
+                                       const int *ids, float *acc) {
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    // tile b covers nodes [b*T + nlo, b*T + nlo + N - 1]
+    const int r = ids[a] - tg.nlo;
+    int b_hi = r / tg.T[a];
+    int b_lo = (r - tg.N[a] + 1 + tg.T[a] - 1) >= 0 ? (r - tg.N[a] + tg.T[a]) / tg.T[a] : 0;
+    if (b_hi > tg.tiles[a] - 1) b_hi = tg.tiles[a] - 1;
+    lo[a] = b_lo;
+    hi[a] = b_hi;
+  }
+  const int ntn = tg.N[0] * (D > 1 ? tg.N[1] : 1) * (D > 2 ? tg.N[2] : 1);
+  for (int bz = lo[2]; bz <= hi[2]; ++bz)
+    for (int by = lo[1]; by <= hi[1]; ++by)
+      for (int bx = lo[0]; bx <= hi[0]; ++bx) {
+        const int tile = bx + tg.tiles[0] * (by + tg.tiles[1] * bz);
+        if (!flags[tile]) continue;
+        int li = ids[0] - (bx * tg.T[0] + tg.nlo);
+        if (D > 1) li += tg.N[0] * (ids[1] - (by * tg.T[1] + tg.nlo));
+        if (D > 2) li += tg.N[0] * tg.N[1] * (ids[2] - (bz * tg.T[2] + tg.nlo));
+        const float *s = partials + (int64_t)tile * tg.nv * ntn + li;
+        for (int v = 0; v < tg.nv; ++v) acc[v] += s[(int64_t)v * ntn];
+      }
 }
 
 // Per node: cutoff + forward moment (finalize), boundary conditions
@@ -340,12 +442,32 @@ __global__ __launch_bounds__(256) void p2g_tile_kernel(pmpm_params prm, pmpm_gri
 // conditions apply (gravity already acted through the particle velocities).
 template <int D>
 __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int finalize, int bc_begin, int bc_end,
-                            int remap_pass) {
+                            int remap_pass, TileGeom tg, const float *__restrict__ partials,
+                            const int32_t *__restrict__ flags) {
   const int nn = g.num_cells_total;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nn) return;
-  const float m = nodes.mass[i];
+  float m = nodes.mass[i];
   float P[3] = {0.f, 0.f, 0.f}, Q[3] = {0.f, 0.f, 0.f};
+  if (tg.gather) {
+    // node arrays hold the wrap-around deposits; add the tile partials
+    int ids0[3];
+    node_ids<D>(g, i, ids0);
+    float acc[1 + 2 * 3] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    gather_partials<D>(tg, partials, flags, ids0, acc);
+    if (tg.gather == 1) {
+      m += acc[0];
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        nodes.mom[a * nn + i] += acc[1 + a];
+        nodes.mom_nt[a * nn + i] += acc[1 + D + a];
+      }
+      nodes.mass[i] = m;
+    } else {
+#pragma unroll
+      for (int a = 0; a < D; ++a) nodes.mom_nt[a * nn + i] += acc[a];
+    }
+  }
   if ((finalize || remap_pass) && m <= prm.small_mass_cutoff) {
     if (finalize) {
       nodes.mass[i] = 0.f;
@@ -583,7 +705,7 @@ __global__ void shape_function_kernel(pmpm_grid g, int types_built, const float 
   int bin[3] = {0, 0, 0};
 #pragma unroll
   for (int a = 0; a < D; ++a) {
-    t[a] = (x[a * n + i] - g.origin[a]) * g.inv_cell_size;
+    t[a] = cell_coord(x[a * n + i], g.origin[a], g.inv_cell_size);
     bin[a] = (int)t[a];
   }
   for (int k = 0; k < NSN; ++k) {
@@ -602,9 +724,9 @@ __global__ void shape_function_kernel(pmpm_grid g, int types_built, const float 
       float N[3] = {1.f, 1.f, 1.f}, dN[3] = {0.f, 0.f, 0.f};
 #pragma unroll
       for (int a = 0; a < D; ++a) {
-        const float frac = t[a] - (float)bin[a];
-        spline_eval<SF>(frac - (float)off[a], cubic_node_type(ids[a], g.num_cells[a], types_built), g.inv_cell_size,
-                        N[a], dN[a]);
+        const float frac = __fsub_rn(t[a], (float)bin[a]);
+        spline_eval<SF>(__fsub_rn(frac, (float)off[a]), cubic_node_type(ids[a], g.num_cells[a], types_built),
+                        g.inv_cell_size, N[a], dN[a]);
       }
       if (D == 1) { wgt = N[0]; gr[0] = dN[0]; }
       else if (D == 2) { wgt = N[0] * N[1]; gr[0] = dN[0] * N[1]; gr[1] = dN[1] * N[0]; }
@@ -638,20 +760,39 @@ __global__ void shape_function_kernel(pmpm_grid g, int types_built, const float 
     default: return PMPM_ERR_INVALID;                                             \
   }
 
+TileGeom tile_geom(int dim, int sf, const pmpm_grid &g, int gather) {
+  TileGeom tg{};
+  const int T3[3] = {16, 4, 2}, T2[3] = {16, 8, 1}, T1[3] = {128, 1, 1};
+  const int *T = dim == 3 ? T3 : (dim == 2 ? T2 : T1);
+  const int span = (sf == PMPM_SF_LINEAR) ? 2 : 4;
+  tg.nlo = (sf == PMPM_SF_LINEAR) ? 0 : -1;
+  for (int a = 0; a < 3; ++a) {
+    tg.T[a] = a < dim ? T[a] : 1;
+    tg.N[a] = a < dim ? T[a] + span - 1 : 1;
+    tg.tiles[a] = a < dim ? (int)ceil_div(g.num_cells[a], T[a]) : 1;
+  }
+  tg.gather = gather;
+  tg.nv = gather == 2 ? dim : 1 + 2 * dim;
+  return tg;
+}
+static size_t tile_count(const TileGeom &tg) { return (size_t)tg.tiles[0] * tg.tiles[1] * tg.tiles[2]; }
+
 template <int D, int SF, int MODE>
 void launch_p2g_tile(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
-                     const pmpm_spatial &part, const pmpm_nodes &nodes, hipStream_t s) {
+                     const pmpm_spatial &part, const pmpm_nodes &nodes, float *partials, int32_t *flags,
+                     hipStream_t s) {
   const int tx = (int)ceil_div(g.num_cells[0], Tile<D>::X);
   const int ty = D > 1 ? (int)ceil_div(g.num_cells[1], Tile<D>::Y) : 1;
   const int tz = D > 2 ? (int)ceil_div(g.num_cells[2], Tile<D>::Z) : 1;
-  hipLaunchKernelGGL((p2g_tile_kernel<D, SF, MODE>), dim3(tx * ty * tz), dim3(256), 0, s, prm, g, in, out, part, nodes,
-                     tx, ty);
+  hipLaunchKernelGGL((p2g_tile_kernel<D, SF, MODE>), dim3(tx * ty * tz), dim3(kTileThreads), 0, s, prm, g, in, out,
+                     part, nodes, tx, ty, partials, flags);
 }
 template <int D, int SF>
 void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
-                const pmpm_spatial &part, const pmpm_nodes &nodes, int mode, hipStream_t s) {
-  if (mode == 0) launch_p2g_tile<D, SF, 0>(prm, g, in, out, part, nodes, s);
-  else launch_p2g_tile<D, SF, 1>(prm, g, in, out, part, nodes, s);
+                const pmpm_spatial &part, const pmpm_nodes &nodes, int mode, float *partials, int32_t *flags,
+                hipStream_t s) {
+  if (mode == 0) launch_p2g_tile<D, SF, 0>(prm, g, in, out, part, nodes, partials, flags, s);
+  else launch_p2g_tile<D, SF, 1>(prm, g, in, out, part, nodes, partials, flags, s);
 }
 template <int D, int SF>
 void launch_g2p(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
@@ -668,13 +809,18 @@ void launch_shape(const pmpm_grid &g, int types_built, const float *x, int n, fl
                      x, n, psi, dpsi);
 }
 
-int launch_grid(const pmpm_params &prm, const pmpm_grid &g, const pmpm_nodes &nodes, int finalize, int b0, int b1,
-                int remap, hipStream_t s) {
+// gather: 0 = node arrays only, 1 = add the P2G tile partials, 2 = add the
+// MUSL re-map tile partials (ctx holds them).
+int launch_grid(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const pmpm_nodes &nodes, int finalize,
+                int b0, int b1, int remap, int gather, hipStream_t s) {
   const unsigned blocks = (unsigned)ceil_div(g.num_cells_total, 256);
+  const TileGeom tg = tile_geom(prm.dim, prm.shape_fn, g, gather);
+  const float *partials = gather ? ctx->partials : nullptr;
+  const int32_t *flags = gather ? ctx->tile_flags : nullptr;
   switch (prm.dim) {
-    case 1: hipLaunchKernelGGL(grid_kernel<1>, dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap); break;
-    case 2: hipLaunchKernelGGL(grid_kernel<2>, dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap); break;
-    case 3: hipLaunchKernelGGL(grid_kernel<3>, dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap); break;
+    case 1: hipLaunchKernelGGL(grid_kernel<1>, dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags); break;
+    case 2: hipLaunchKernelGGL(grid_kernel<2>, dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags); break;
+    case 3: hipLaunchKernelGGL(grid_kernel<3>, dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags); break;
     default: return PMPM_ERR_INVALID;
   }
   PMPM_LAUNCH_CHECK();
@@ -701,14 +847,23 @@ int do_zero_nodes(const pmpm_params &prm, const pmpm_grid &g, const pmpm_nodes &
   return PMPM_OK;
 }
 
-int do_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &p, const pmpm_spatial &part,
-           const pmpm_nodes &nodes, hipStream_t s) {
-  int rc = do_zero_nodes(prm, g, nodes, s);
+static int reserve_tiles(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g) {
+  const TileGeom tg = tile_geom(prm.dim, prm.shape_fn, g, 1);
+  const size_t tiles = tile_count(tg);
+  const size_t ntn = (size_t)tg.N[0] * tg.N[1] * tg.N[2];
+  return ensure_partials(ctx, tiles * tg.nv * ntn, tiles);
+}
+
+// Scatter into the tile partials (+ wrap-around deposits into the cleared
+// node arrays); a grid_kernel pass with gather = 1 completes the P2G.
+int do_p2g(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &p,
+           const pmpm_spatial &part, const pmpm_nodes &nodes, hipStream_t s) {
+  int rc = reserve_tiles(ctx, prm, g);
   if (rc != PMPM_OK) return rc;
-  if (p.n > 0) {
-    PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, p, p, part, nodes, 0, s);
-    PMPM_LAUNCH_CHECK();
-  }
+  if ((rc = do_zero_nodes(prm, g, nodes, s)) != PMPM_OK) return rc;
+  PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, p, p, part, nodes, 0, ctx->partials, ctx->tile_flags,
+                    s);
+  PMPM_LAUNCH_CHECK();
   return PMPM_OK;
 }
 
@@ -721,14 +876,16 @@ int do_g2p(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in,
   return PMPM_OK;
 }
 
-int do_musl_map(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
-                const pmpm_spatial &part, const pmpm_nodes &nodes, hipStream_t s) {
-  int rc = zero_fill(nodes.mom_nt, (int64_t)g.num_cells_total * prm.dim, s);
+// First map + re-map into the tile partials; a grid_kernel pass with
+// gather = 2 completes mom_nt.
+int do_musl_map(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in,
+                const pmpm_particles &out, const pmpm_spatial &part, const pmpm_nodes &nodes, hipStream_t s) {
+  int rc = reserve_tiles(ctx, prm, g);
   if (rc != PMPM_OK) return rc;
-  if (in.n > 0) {
-    PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, in, out, part, nodes, 1, s);
-    PMPM_LAUNCH_CHECK();
-  }
+  if ((rc = zero_fill(nodes.mom_nt, (int64_t)g.num_cells_total * prm.dim, s)) != PMPM_OK) return rc;
+  PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, in, out, part, nodes, 1, ctx->partials,
+                    ctx->tile_flags, s);
+  PMPM_LAUNCH_CHECK();
   return PMPM_OK;
 }
 
@@ -746,21 +903,21 @@ extern "C" int pmpm_shape_function(int32_t dim, int32_t shape_fn, int32_t node_t
   return PMPM_OK;
 }
 
-extern "C" int pmpm_p2g(const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles *p,
+extern "C" int pmpm_p2g(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles *p,
                         const pmpm_spatial *part, pmpm_nodes *nodes, pmpm_stream_t stream) {
-  if (!params_ok(prm) || !grid || !particles_ok(p) || !part || !nodes) return PMPM_ERR_INVALID;
+  if (!ctx || !params_ok(prm) || !grid || !particles_ok(p) || !part || !nodes) return PMPM_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
-  int rc = do_p2g(*prm, *grid, *p, *part, *nodes, s);
+  int rc = do_p2g(ctx, *prm, *grid, *p, *part, *nodes, s);
   if (rc != PMPM_OK) return rc;
   // the reference's P2G ends with the cutoff and mom_nt = mom + f dt
-  return launch_grid(*prm, *grid, *nodes, 1, 0, 0, 0, s);
+  return launch_grid(ctx, *prm, *grid, *nodes, 1, 0, 0, 0, 1, s);
 }
 
 extern "C" int pmpm_grid_bc(const pmpm_params *prm, const pmpm_grid *grid, pmpm_nodes *nodes, int32_t bc_begin,
                             int32_t bc_end, int32_t remap_pass, pmpm_stream_t stream) {
   if (!params_ok(prm) || !grid || !nodes) return PMPM_ERR_INVALID;
   if (bc_begin < 0 || bc_end > prm->num_bcs || bc_begin > bc_end) return PMPM_ERR_INVALID;
-  return launch_grid(*prm, *grid, *nodes, 0, bc_begin, bc_end, remap_pass ? 1 : 0, (hipStream_t)stream);
+  return launch_grid(nullptr, *prm, *grid, *nodes, 0, bc_begin, bc_end, remap_pass ? 1 : 0, 0, (hipStream_t)stream);
 }
 
 extern "C" int pmpm_g2p(const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles *in, pmpm_particles *out,
@@ -770,12 +927,17 @@ extern "C" int pmpm_g2p(const pmpm_params *prm, const pmpm_grid *grid, const pmp
   return do_g2p(*prm, *grid, *in, *out, *part, *nodes, 0, (hipStream_t)stream);
 }
 
-extern "C" int pmpm_musl_map(const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles *in,
-                             pmpm_particles *out, const pmpm_spatial *part, pmpm_nodes *nodes,
-                             pmpm_stream_t stream) {
-  if (!params_ok(prm) || !grid || !particles_ok(in) || !particles_ok(out) || !part || !nodes) return PMPM_ERR_INVALID;
+extern "C" int pmpm_musl_map(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid,
+                             const pmpm_particles *in, pmpm_particles *out, const pmpm_spatial *part,
+                             pmpm_nodes *nodes, pmpm_stream_t stream) {
+  if (!ctx || !params_ok(prm) || !grid || !particles_ok(in) || !particles_ok(out) || !part || !nodes)
+    return PMPM_ERR_INVALID;
   if (out->n != in->n || out->x == in->x) return PMPM_ERR_INVALID;
-  return do_musl_map(*prm, *grid, *in, *out, *part, *nodes, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  int rc = do_musl_map(ctx, *prm, *grid, *in, *out, *part, *nodes, s);
+  if (rc != PMPM_OK) return rc;
+  // mom_nt := re-mapped momentum (mass cutoff as P2G), node velocities
+  return launch_grid(ctx, *prm, *grid, *nodes, 0, 0, 0, 1, 2, s);
 }
 
 extern "C" int pmpm_musl_g2p(const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles *in,

@@ -70,13 +70,20 @@ def dp_params(phi_deg, cohesion, psi_deg=0.0):
 
 
 # ------------------------------------------------------------------ scenarios
-def two_disc(alpha=0.99, musl=False):
-    """Config 1: 2D two-disc elastic impact, linear, USL (416 particles)."""
+def two_disc(alpha=0.99, musl=False, swirl=0.0):
+    """Config 1: 2D two-disc elastic impact, linear, USL (416 particles).
+    swirl > 0 adds a radial + shear velocity field inside each disc so the
+    discs deform from the first step (used by parity checks: with rigid
+    translation alone the stress is rounding noise until the impact)."""
     h = 0.05
     c1 = create_circle([0.25, 0.25], 0.2, h, 4)
     c2 = create_circle([0.75, 0.75], 0.2, h, 4)
     pos = np.vstack([c1, c2])
     vel = np.vstack([np.full(c1.shape, 0.1), np.full(c2.shape, -0.1)])
+    if swirl:
+        ctr = np.vstack([np.tile([0.25, 0.25], (len(c1), 1)), np.tile([0.75, 0.75], (len(c2), 1))])
+        r = pos - ctr
+        vel = vel + swirl * np.stack([r[:, 0] + 0.5 * r[:, 1], r[:, 1] - 0.3 * r[:, 0]], axis=1)
     return Spec(dim=2, origin=[0.0, 0.0], end=[1.0, 1.0], cell_size=h, shape_function="linear", dt=1e-3, ppc=4,
                 positions=pos, velocities=vel,
                 materials=[("LinearElastic", {"density": 1000.0, "E": 1000.0, "pois": 0.3})],

@@ -172,11 +172,11 @@ def _compare_run(oracle_mod, spec, steps, tol, label):
 
 
 def test_two_disc_usl_parity(oracle_mod):
-    _compare_run(oracle_mod, S.two_disc(), 100, 1e-4, "two-disc USL linear LE")
+    _compare_run(oracle_mod, S.two_disc(swirl=0.5), 100, 1e-4, "two-disc USL linear LE")
 
 
 def test_two_disc_musl_parity(oracle_mod):
-    _compare_run(oracle_mod, S.two_disc(musl=True), 100, 1e-4, "two-disc MUSL linear LE")
+    _compare_run(oracle_mod, S.two_disc(musl=True, swirl=0.5), 100, 1e-4, "two-disc MUSL linear LE")
 
 
 def test_column_dp_musl_cubic_parity(oracle_mod):
@@ -206,8 +206,12 @@ def test_newtonian_fluid_parity(oracle_mod):
 
 
 def test_rigid_levelset_parity(oracle_mod):
+    # The rod sits off the node lattice: with rigid particles exactly on nodes
+    # and mid-cells, the nearest-rigid-particle choice hits exact distance ties
+    # and the oracle's own FP32 and FP64 runs already differ by 1e-3 after one
+    # step. Off the lattice the FP32/FP64 spread is about 2e-5.
     disc = S.create_circle([0.5, 0.45], 0.2, 0.05, 4)
-    rod = np.stack([np.linspace(0.3, 0.7, 17), np.full(17, 0.7)], axis=1)
+    rod = np.stack([np.linspace(0.3037, 0.6961, 17), np.full(17, 0.6913)], axis=1)
     spec = S.Spec(dim=2, origin=[0.0, 0.0], end=[1.0, 1.0], cell_size=0.05, shape_function="linear", dt=1e-3, ppc=4,
                   positions=disc, velocities=np.tile([0.0, 0.5], (len(disc), 1)), rigid_positions=rod,
                   materials=[("LinearElastic", {"density": 1000.0, "E": 1000.0, "pois": 0.3})],
