@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--cell-size", type=float, default=0.01)
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="before the workload, run 3 plain device copies of MIB MiB (calibrates HBM byte counters)")
     return ap.parse_args()
 
 
@@ -204,6 +206,15 @@ def main():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        src = torch.ones(args.copy_probe * (1 << 18), dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"bench: copy probe {args.copy_probe} MiB read + {args.copy_probe} MiB written per launch",
+              file=sys.stderr, flush=True)
+        del src, dst
     wl = Workload(args.per_axis, args.cell_size)
     n = wl.n
     print(f"bench: column collapse, {n} particles, grid {list(wl.grid.num_cells)}, warmup {args.warmup}, "
@@ -229,8 +240,9 @@ def main():
     value = n * args.steps / (total_ms / 1e3)
     print("bench: per-entry-point mean ms: " + json.dumps({k: round(v, 4) for k, v in per_call.items()}),
           file=sys.stderr, flush=True)
-    # dominant single-kernel entry point
-    single = {k: v for k, v in per_call.items() if k in ("musl_g2p", "musl_map")}
+    # dominant entry point (each is one main kernel plus, for P2G and the
+    # MUSL map, the node-gather grid kernel)
+    single = {k: v for k, v in per_call.items() if k in BYTES_PER_PARTICLE}
     dom = max(single, key=single.get)
     achieved = BYTES_PER_PARTICLE[dom] * n / (single[dom] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

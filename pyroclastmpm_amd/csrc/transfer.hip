@@ -1,18 +1,19 @@
 // transfer.hip -- particle/grid transfers on gfx950.
 //
-//   p2g_tile_kernel   one workgroup per tile of cells, one lane per cell; each
-//                     lane walks its cell's particles (cell order from the
-//                     radix sort) and accumulates mass, momentum and force into
-//                     an LDS copy of the tile's nodes. Lanes of one wave hold
-//                     different cells, so a given window offset hits distinct
-//                     LDS addresses; the tile is then flushed with one global
-//                     atomic per node and field. The same kernel in MODE 1 is
-//                     the MUSL first map + re-map.
-//   grid_kernel       per node: mass cutoff, mom_nt = mom + f dt, boundary
-//                     conditions in list order, node velocities for G2P.
+//   p2g_kernel        one workgroup per tile of cells, one lane per (cell, y
+//                     node offset); each lane walks its cell's particles (cell
+//                     order from the radix sort) and sums mass, momentum and
+//                     force for its nodes in registers, then adds them into an
+//                     LDS copy of the tile's nodes in conflict-free phases and
+//                     the tile is stored to a per-tile slice. MODE 1 is the
+//                     MUSL re-map of momentum.
+//   grid_kernel       per node: sums the tile slices covering it, mass cutoff,
+//                     mom_nt = mom + f dt, boundary conditions in list order,
+//                     node velocities for G2P.
 //   g2p_kernel        per particle (cell order): gather from the grid,
 //                     FLIP/PIC update, F and volume, then the return map of the
-//                     particle's material, written in cell order.
+//                     particle's material, written in cell order. MODE 1 is the
+//                     MUSL first map (x, v only).
 // Weights are recomputed in registers from the positions; nothing per
 // particle x node is stored.
 #include <hip/hip_runtime.h>
@@ -22,13 +23,40 @@
 
 namespace pmpm {
 
-// Cell tile of one P2G workgroup (128 cells) and its thread count: every cell
-// is shared by kTileThreads / 128 = 8 waves ("rounds").
-constexpr int kTileThreads = 1024;
-template <int D> struct Tile;
-template <> struct Tile<3> { static constexpr int X = 16, Y = 4, Z = 2; };
-template <> struct Tile<2> { static constexpr int X = 16, Y = 8, Z = 1; };
-template <> struct Tile<1> { static constexpr int X = 128, Y = 1, Z = 1; };
+// P2G workgroup. Lane = (cell, y node offset sy of the cell's node span); the
+// CX = 64 / SY cells of a wave lie along x. 3D: 8 waves, a pair per z layer
+// of 4 consecutive layers (tile CX x 1 x 4 cells); the two waves of a pair
+// take the lower and upper half of the z node offsets. 1D/2D: 4 waves, 4
+// consecutive runs along x (tile 4 CX x 1 cells).
+
+// LDS row pitch of the P2G node tile: at least nx, and an odd multiple of cx
+// so the SY rows of cx lanes that one flush instruction covers fall into
+// disjoint bank groups (64 banks of 4 bytes).
+__host__ __device__ constexpr int p2g_pitch(int nx, int cx, int sy) {
+  return sy <= 1 ? nx : (((nx + cx - 1) / cx) | 1) * cx;
+}
+
+template <int D, int SF> struct P2GGeom {
+  static constexpr int SPAN = Spline<SF>::SPAN;  // node offsets a cell's particles reach, per axis
+  static constexpr int NLO = Spline<SF>::NLO;    // the first of them, relative to the cell
+  static constexpr int SY = D > 1 ? SPAN : 1;    // lanes per cell
+  static constexpr int SZ = D > 2 ? SPAN : 1;    // z node offsets per cell
+  static constexpr int ZH = D > 2 ? 2 : 1;       // waves sharing a z layer
+  static constexpr int SZH = SZ / ZH;            // z node offsets per lane
+  static constexpr int WAVES = D > 2 ? 8 : 4;
+  static constexpr int THREADS = 64 * WAVES;
+  static constexpr int CX = 64 / SY;             // cells per wave
+  static constexpr int TX = D == 3 ? CX : CX * WAVES;
+  static constexpr int TY = 1;
+  static constexpr int TZ = D == 3 ? WAVES / ZH : 1;
+  static constexpr int NX = TX + SPAN - 1;       // tile nodes per axis
+  static constexpr int NY = D > 1 ? TY + SPAN - 1 : 1;
+  static constexpr int NZ = D > 2 ? TZ + SPAN - 1 : 1;
+  static constexpr int NXP = p2g_pitch(NX, CX, SY);
+  static constexpr int NT = NXP * NY * NZ;       // LDS nodes incl. padding
+  static constexpr int NTN = NX * NY * NZ;       // nodes of one partial slice
+  static_assert(CX * SY == 64 && SZH * ZH == SZ, "one wave = CX cells x SY lanes");
+};
 
 __device__ inline int64_t lin_index(const pmpm_grid &g, const int *b, int D) {
   int64_t h = b[0];
@@ -66,36 +94,52 @@ __device__ inline void particle_weights(const pmpm_params &prm, const pmpm_grid 
   }
 }
 
-// Grid gather with the reference's per-axis bounds check (usl_inline.h:281-293)
-// in its window order (y outer, x, z inner). Nodes below the mass cutoff carry
-// zero velocities, which is equivalent to skipping them.
+// Grid gather in the reference's window order (y outer, x, z inner). The
+// reference skips entries outside the grid on any axis (usl_inline.h:281-293);
+// here such entries get zero weights and a clamped, valid node index, which
+// leaves every sum unchanged and keeps the loop free of branches. Nodes below
+// the mass cutoff carry zero velocities, which is equivalent to skipping them.
 template <int D, int SF, bool WANT_V, bool WANT_L>
 __device__ inline void gather_grid(const pmpm_grid &g, const pmpm_nodes &nodes, int nn, const PW<D, SF> &w,
                                    float *vinc, float *dvinc, float *L) {
   constexpr int E = Spline<SF>::ENTRIES;
   constexpr int EY = D > 1 ? E : 1;
   constexpr int EZ = D > 2 ? E : 1;
+  float N[3][4], dN[3][4];
+  int off[3][4];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int stride = a == 0 ? 1 : (a == 1 ? g.num_cells[0] : g.num_cells[0] * g.num_cells[1]);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (a < D && e < E) {
+        const int i = w.bin[a] + w.first[a] + e;
+        const bool ok = i >= 0 && i < g.num_cells[a];
+        off[a][e] = min(max(i, 0), g.num_cells[a] - 1) * stride;
+        N[a][e] = ok ? w.N[a][e] : 0.f;
+        dN[a][e] = ok ? w.dN[a][e] : 0.f;
+      } else {
+        off[a][e] = 0;
+        N[a][e] = 1.f;
+        dN[a][e] = 0.f;
+      }
+    }
+  }
 #pragma unroll
   for (int ey = 0; ey < EY; ++ey) {
-    const int iy = (D > 1) ? w.bin[1] + w.first[1] + ey : 0;
-    if (D > 1 && (iy < 0 || iy >= g.num_cells[1])) continue;
 #pragma unroll
     for (int ex = 0; ex < E; ++ex) {
-      const int ix = w.bin[0] + w.first[0] + ex;
-      if (ix < 0 || ix >= g.num_cells[0]) continue;
-      const float nxy = w.N[0][ex] * w.N[1][ey];
-      const float dxy = w.dN[0][ex] * w.N[1][ey];
-      const float dyx = w.dN[1][ey] * w.N[0][ex];
+      const float nxy = N[0][ex] * N[1][ey];
+      const float dxy = dN[0][ex] * N[1][ey];
+      const float dyx = dN[1][ey] * N[0][ex];
 #pragma unroll
       for (int ez = 0; ez < EZ; ++ez) {
-        const int iz = (D > 2) ? w.bin[2] + w.first[2] + ez : 0;
-        if (D > 2 && (iz < 0 || iz >= g.num_cells[2])) continue;
-        const int node = ix + iy * g.num_cells[0] + iz * g.num_cells[0] * g.num_cells[1];
-        const float wgt = nxy * w.N[2][ez];
+        const int node = off[0][ex] + off[1][ey] + off[2][ez];
+        const float wgt = nxy * N[2][ez];
         float gr[3];
-        gr[0] = dxy * w.N[2][ez];
-        gr[1] = dyx * w.N[2][ez];
-        gr[2] = (w.dN[2][ez] * w.N[0][ex]) * w.N[1][ey];
+        gr[0] = dxy * N[2][ez];
+        gr[1] = dyx * N[2][ez];
+        gr[2] = (dN[2][ez] * N[0][ex]) * N[1][ey];
         float vnt[3];
 #pragma unroll
         for (int a = 0; a < D; ++a) vnt[a] = nodes.vel_nt[a * nn + node];
@@ -189,58 +233,161 @@ __device__ inline void scatter_entry_global(const pmpm_params &prm, const pmpm_g
   }
 }
 
-// LDS row pitch: at least the tile's node extent and 16 mod 32, so the two
-// cell rows of one half-wave (16 lanes each) fall into opposite bank halves.
-__host__ __device__ constexpr int lds_pitch(int nx, int rows_per_wave) {
-  return rows_per_wave <= 1 ? nx : ((nx + 15) / 32) * 32 + 16;
+// Weight of one particle at span offset S (node NLO + S relative to its cell)
+// along one axis, frac = t - bin in [0, 1): the same value as the window
+// weight (axis_weights), and 0 at an offset the window does not reach
+// (quadratic). `type` is the node's cubic type (shapefunction.cpp:108-122).
+// For the cubic spline (types 1-3; the caller handles unbuilt types),
+// r = frac - (S - 1) lies in piece 3 - S of the table (cubic.cuh:28-197)
+// except when frac = 1 - 2^-24, where rounding puts r on the next knot: at
+// S = 0 (r = 2) the weight is 0 as in cubic_eval; at S = 3 (r = -1) both
+// pieces meet there (the spline is C2), so the value agrees to rounding.
+template <int SF, int S>
+__device__ inline void span_eval(float frac, int type, float inv_h, float &N, float &dN) {
+  const float r = __fsub_rn(frac, (float)(Spline<SF>::NLO + S));
+  if (SF != PMPM_SF_CUBIC) {
+    spline_eval<SF>(r, 0, inv_h, N, dN);
+    return;
+  }
+  const bool t1 = type == 1, t2 = type == 2;
+  float c0, c1, c2, c3;
+  if (S == 0) {  // piece 3, identical for types 1-3
+    c0 = 4.f / 3.f; c1 = -2.f; c2 = 1.f; c3 = -1.f / 6.f;
+  } else if (S == 1) {  // piece 2
+    c0 = t1 ? 1.f : 2.f / 3.f; c1 = t1 ? -1.f : 0.f; c2 = t1 ? 0.f : -1.f; c3 = t1 ? 1.f / 6.f : 0.5f;
+  } else if (S == 2) {  // piece 1
+    c0 = t1 ? 1.f : 2.f / 3.f; c1 = t1 ? 1.f : 0.f; c2 = t1 ? 0.f : -1.f;
+    c3 = t1 ? -1.f / 6.f : (t2 ? -1.f / 3.f : -0.5f);
+  } else {  // piece 0, zero for type 2
+    c0 = t2 ? 0.f : 4.f / 3.f; c1 = t2 ? 0.f : 2.f; c2 = t2 ? 0.f : 1.f; c3 = t2 ? 0.f : 1.f / 6.f;
+  }
+  N = ((c3 * r + c2) * r + c1) * r + c0;
+  dN = inv_h * ((3.f * c3 * r + 2.f * c2) * r + c1);
+  if (S == 0) {
+    N = r < 2.f ? N : 0.f;
+    dN = r < 2.f ? dN : 0.f;
+  }
 }
 
-// MODE 0: P2G of mass, momentum and force (force accumulates in mom_nt).
-// MODE 1: MUSL first map (x, v of particle j written to slot j of `out`) and
-//         re-map of momentum into mom_nt.
-// Thread t of a workgroup owns cell (t % CELLS) of the tile and walks the
-// particles k = t / CELLS, t / CELLS + ROUNDS, ... of that cell, so every wave
-// holds 64 distinct cells (conflict-free LDS atomics at a fixed window offset)
-// and a cell's particles spread over ROUNDS waves.
-// The finished LDS block is written with plain stores to this tile's slice of
-// `partials` (in-grid nodes; flags[tile] says whether the slice is live) and
-// the grid kernel sums the <= 2^D slices covering each node. LDS positions
-// outside the grid hold the reference's linear-index wrap-around deposits
-// (weighted with the wrapped node's types); they are flushed with one global
-// atomic per tile and field into the node arrays.
+// Per-particle inputs of a P2G deposit.
+struct P2GRec {
+  float x[3], v[3], m;
+  Sym A;  // -V sigma (MODE 0)
+};
+template <int D, int MODE>
+__device__ inline void p2g_load(const pmpm_particles &in, const pmpm_particles &out, int n, int slot, int j,
+                                P2GRec &r) {
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    r.x[a] = in.x[a * n + slot];
+    r.v[a] = MODE == 0 ? in.v[a * n + slot] : out.v[a * n + j];
+  }
+  r.m = in.mass[slot];
+  if (MODE == 0) {
+    const Sym s = load_sym(in.stress, n, slot);
+    const float mvol = -in.volume[slot];
+    r.A = Sym{mvol * s.xx, mvol * s.yy, mvol * s.zz, mvol * s.xy, mvol * s.xz, mvol * s.yz};
+  } else {
+    r.A = sym_zero();
+  }
+}
+
+// P2G of a particle whose recomputed bin is not the cell it is listed in
+// (only possible when the partition was built from other positions): this
+// lane's y slice of its window goes straight to the node arrays.
 template <int D, int SF, int MODE>
-__global__ __launch_bounds__(kTileThreads) void p2g_tile_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles in,
-                                                                pmpm_particles out, pmpm_spatial part,
-                                                                pmpm_nodes nodes, int tiles_x, int tiles_y,
-                                                                float *__restrict__ partials,
-                                                                int32_t *__restrict__ flags) {
-  using SP = Spline<SF>;
-  using T = Tile<D>;
-  constexpr int CELLS = T::X * T::Y * T::Z;
-  constexpr int ROUNDS = kTileThreads / CELLS;
-  constexpr int NX = T::X + SP::SPAN - 1;
-  constexpr int NY = D > 1 ? T::Y + SP::SPAN - 1 : 1;
-  constexpr int NZ = D > 2 ? T::Z + SP::SPAN - 1 : 1;
-  constexpr int NXP = lds_pitch(NX, D > 1 ? 64 / T::X : 1);
-  constexpr int NT = NXP * NY * NZ;  // LDS nodes incl. padding
-  constexpr int NV = MODE == 0 ? 1 + 2 * D : D;
-  constexpr int E = SP::ENTRIES;
-  constexpr int EY = D > 1 ? E : 1;
+__device__ void p2g_stray(const pmpm_params &prm, const pmpm_grid &g, const pmpm_nodes &nodes, const float *x, int n,
+                          int slot, int sy, float m, const float *v, const Sym &A) {
+  constexpr int E = Spline<SF>::ENTRIES;
   constexpr int EZ = D > 2 ? E : 1;
+  if (sy >= (D > 1 ? E : 1)) return;
+  PW<D, SF> w;
+  float xp[3];
+  particle_weights<D, SF>(prm, g, x, n, slot, w, xp);
+  float mv[3] = {0.f, 0.f, 0.f}, A3[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (a < D) mv[a] = m * v[a];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) A3[a][b] = (a < D && b < D) ? sym_at(A, a, b) : 0.f;
+  }
+#pragma unroll 1
+  for (int ex = 0; ex < E; ++ex)
+#pragma unroll 1
+    for (int ez = 0; ez < EZ; ++ez) {
+      const int e3[3] = {ex, sy, ez};
+      scatter_entry_global<D, SF, MODE>(prm, g, nodes, g.num_cells_total, w, e3, m, mv, A3);
+    }
+}
+
+// Sums r (P2G field order) of node vector nb, which lies outside the grid,
+// into the node its linear index reaches, if that index is in range.
+template <int D, int MODE>
+__device__ inline void deposit_wrapped(const pmpm_grid &g, const pmpm_nodes &nodes, const int *nb, const float *r) {
+  const int nn = g.num_cells_total;
+  const int64_t lin = lin_index(g, nb, D);
+  if (lin < 0 || lin >= nn) return;
+  const int node = (int)lin;
+  if (MODE == 0) {
+    if (r[0] != 0.f) unsafeAtomicAdd(&nodes.mass[node], r[0]);
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      if (r[1 + a] != 0.f) unsafeAtomicAdd(&nodes.mom[a * nn + node], r[1 + a]);
+      if (r[1 + D + a] != 0.f) unsafeAtomicAdd(&nodes.mom_nt[a * nn + node], r[1 + D + a]);
+    }
+  } else {
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      if (r[a] != 0.f) unsafeAtomicAdd(&nodes.mom_nt[a * nn + node], r[a]);
+  }
+}
+
+// P2G of one tile of cells.
+//   MODE 0: mass, momentum and force (force accumulates in mom_nt).
+//   MODE 1: MUSL re-map of m v' into mom_nt; v' is read from out.v in cell
+//           order (written by the first map), weights from the positions in.x.
+// Lane (cell c, sy) walks the particles of c in sorted order and sums in
+// registers their contributions to its SPAN x SZ nodes c + NLO + (sx, sy, sz).
+// The sums then enter an LDS node tile in SPAN x SZ phases: within one phase
+// the lanes of the workgroup address distinct nodes, so a plain read-add-write
+// is race free (LDS float atomics are over an order of magnitude slower on
+// gfx950). The tile goes with plain stores to this workgroup's slice of
+// `partials` (flags[tile] marks the slice live) and the grid kernel sums the
+// slices covering each node.
+// Node offsets outside the grid: the reference bound-checks only the linear
+// node index (usl_inline.h:122-126, shapefunction.cpp:96-98), so such a
+// deposit lands on the wrapped node, weighted with that node's cubic types.
+// Those sums go to the node arrays with one global atomic per cell, node
+// offset and field (wrap_quirk), or are dropped.
+// (3 waves per SIMD: the 3D cubic accumulators fit in 168 registers)
+template <int D, int SF, int MODE>
+__global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_waves_per_eu(3))) void p2g_kernel(pmpm_params prm, pmpm_grid g,
+                                                                      pmpm_particles in, pmpm_particles out,
+                                                                      pmpm_spatial part, pmpm_nodes nodes,
+                                                                      int tiles_x, int tiles_y,
+                                                                      float *__restrict__ partials,
+                                                                      int32_t *__restrict__ flags) {
+  using G = P2GGeom<D, SF>;
+  constexpr int SPAN = G::SPAN, NLO = G::NLO, SZ = G::SZ, SZH = G::SZH, NE = SPAN * SZH;
+  constexpr int NV = MODE == 0 ? 1 + 2 * D : D;
+  constexpr int NXP = G::NXP, NY = G::NY, NT = G::NT;
   __shared__ float tile[NV * NT];
 
   const int n = in.n;
   const int nn = g.num_cells_total;
+  const float inv_h = g.inv_cell_size;
   const int t = threadIdx.x;
-  const int cl = t % CELLS;
-  const int round = t / CELLS;
+  const int lane = t & 63, wave = t >> 6;
+  const int sy = D > 1 ? lane / G::CX : 0;
+  const int lx = (D == 3 ? 0 : wave * G::CX) + lane % G::CX;  // tile-local cell
+  const int lz = D == 3 ? wave / G::ZH : 0;
+  const int zh = D == 3 ? wave % G::ZH : 0;  // which half of the z node offsets
   const int bx = blockIdx.x % tiles_x;
   const int by = (blockIdx.x / tiles_x) % tiles_y;
   const int bz = blockIdx.x / (tiles_x * tiles_y);
-  const int c[3] = {bx * T::X + cl % T::X, by * T::Y + (cl / T::X) % T::Y, bz * T::Z + cl / (T::X * T::Y)};
-  const int n0[3] = {bx * T::X + SP::NLO, D > 1 ? by * T::Y + SP::NLO : 0, D > 2 ? bz * T::Z + SP::NLO : 0};
+  const int c[3] = {bx * G::TX + lx, D > 1 ? by : 0, D > 2 ? bz * G::TZ + lz : 0};
 
-  for (int i = t; i < NV * NT; i += kTileThreads) tile[i] = 0.f;
+  for (int i = t; i < NV * NT; i += G::THREADS) tile[i] = 0.f;
 
   bool in_grid = c[0] < g.num_cells[0];
   if (D > 1) in_grid = in_grid && c[1] < g.num_cells[1];
@@ -251,149 +398,292 @@ __global__ __launch_bounds__(kTileThreads) void p2g_tile_kernel(pmpm_params prm,
     ce = part.cell_end[cell];
     cs = part.cell_start[cell];
   }
-  if (!__syncthreads_or(ce - cs > round)) {
+  // cubic weights without node types are zero (cubic.cuh: type 0): nothing
+  // to deposit
+  const bool none = SF == PMPM_SF_CUBIC && !prm.node_types_built;
+  if (!__syncthreads_or(ce > cs) || none) {
     if (t == 0) flags[blockIdx.x] = 0;
     return;
   }
 
-  for (int j = cs + round; j < ce; j += ROUNDS) {
-    const int slot = part.sorted[j];
-    PW<D, SF> w;
-    float xp[3];
-    particle_weights<D, SF>(prm, g, in.x, n, slot, w, xp);
-    const float m = in.mass[slot];
-    float mv[3] = {0.f, 0.f, 0.f};
-    float A[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-    if (MODE == 0) {
+  // which of this lane's nodes lie inside the grid
+  bool x_in[SPAN], z_in[SPAN];
+  const int yn = c[1] + NLO + sy;
+  const bool y_in = D < 2 || (yn >= 0 && yn < g.num_cells[1]);
+  bool all_in = y_in;
 #pragma unroll
-      for (int a = 0; a < D; ++a) mv[a] = m * in.v[a * n + slot];
-      const Sym s = load_sym(in.stress, n, slot);
-      const float mvol = -in.volume[slot];
+  for (int s = 0; s < SPAN; ++s) {
+    const int xn = c[0] + NLO + s, zn = c[2] + NLO + s;
+    x_in[s] = xn >= 0 && xn < g.num_cells[0];
+    z_in[s] = D < 3 || (zn >= 0 && zn < g.num_cells[2]);
+    all_in = all_in && x_in[s] && (s >= SZ || z_in[s]);
+  }
+
+  float acc[NE][NV];
 #pragma unroll
-      for (int a = 0; a < D; ++a)
+  for (int e = 0; e < NE; ++e)
 #pragma unroll
-        for (int b = 0; b < D; ++b) A[a][b] = mvol * sym_at(s, a, b);
+    for (int k = 0; k < NV; ++k) acc[e][k] = 0.f;
+
+  // cubic node types of this lane's span nodes (own particles share them)
+  int tx[SPAN], tz[SZH], tyv[SPAN];
+#pragma unroll
+  for (int s = 0; s < SPAN; ++s) {
+    tx[s] = cubic_node_type(c[0] + NLO + s, g.num_cells[0], prm.node_types_built);
+    tyv[s] = D > 1 ? cubic_node_type(c[1] + NLO + s, g.num_cells[1], prm.node_types_built) : 0;
+  }
+#pragma unroll
+  for (int s = 0; s < SZH; ++s)
+    tz[s] = D > 2 ? cubic_node_type(c[2] + NLO + zh * SZH + s, g.num_cells[2], prm.node_types_built) : 0;
+
+  // the next particle's inputs are loaded while the current one is summed
+  int j = cs;
+  int slot = j < ce ? part.sorted[j] : 0;
+  int slot_next = j + 1 < ce ? part.sorted[j + 1] : 0;
+  P2GRec cur;
+  if (j < ce) p2g_load<D, MODE>(in, out, n, slot, j, cur);
+  for (; j < ce; ++j) {
+    const int jn = j + 1;
+    P2GRec nxt = cur;
+    if (jn < ce) p2g_load<D, MODE>(in, out, n, slot_next, jn, nxt);
+    const int slot_cur = slot;
+    slot = slot_next;
+    slot_next = jn + 1 < ce ? part.sorted[jn + 1] : 0;
+
+    float frac[3] = {0.f, 0.f, 0.f};
+    bool own = true;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      const float ta = cell_coord(cur.x[a], g.origin[a], inv_h);
+      const int ba = (int)ta;
+      own = own && ba == c[a];
+      frac[a] = __fsub_rn(ta, (float)ba);
+    }
+    if (!own) {
+      if (zh == 0) p2g_stray<D, SF, MODE>(prm, g, nodes, in.x, n, slot_cur, sy, cur.m, cur.v, cur.A);
+      cur = nxt;
+      continue;
+    }
+    float Nx[SPAN], dNx[SPAN], Nz[SZH], dNz[SZH], Ny = 1.f, dNy = 0.f;
+    span_eval<SF, 0>(frac[0], tx[0], inv_h, Nx[0], dNx[0]);
+    span_eval<SF, 1>(frac[0], tx[1], inv_h, Nx[1], dNx[1]);
+    if constexpr (SPAN > 2) {
+      span_eval<SF, 2>(frac[0], tx[2], inv_h, Nx[2], dNx[2]);
+      span_eval<SF, 3>(frac[0], tx[3], inv_h, Nx[3], dNx[3]);
+    }
+    if constexpr (D > 2) {
+      // this wave's half of the z offsets (zh is uniform per wave)
+      if (zh == 0) {
+        span_eval<SF, 0>(frac[2], tz[0], inv_h, Nz[0], dNz[0]);
+        if constexpr (SZH > 1) span_eval<SF, 1>(frac[2], tz[1], inv_h, Nz[1], dNz[1]);
+      } else {
+        span_eval<SF, SZH>(frac[2], tz[0], inv_h, Nz[0], dNz[0]);
+        if constexpr (SZH > 1) span_eval<SF, SZH + 1>(frac[2], tz[1], inv_h, Nz[1], dNz[1]);
+      }
     } else {
-      float vinc[3] = {0.f, 0.f, 0.f}, dvinc[3] = {0.f, 0.f, 0.f};
-      gather_grid<D, SF, true, false>(g, nodes, nn, w, vinc, dvinc, nullptr);
-      const float am = (float)(1.0 - (double)prm.alpha);
+      Nz[0] = 1.f;
+      dNz[0] = 0.f;
+    }
+    if constexpr (D > 1) {
+      // all span offsets along y, then this lane's one (selects, no branches)
+      float Nys[SPAN], dNys[SPAN];
+      span_eval<SF, 0>(frac[1], tyv[0], inv_h, Nys[0], dNys[0]);
+      span_eval<SF, 1>(frac[1], tyv[1], inv_h, Nys[1], dNys[1]);
+      if constexpr (SPAN > 2) {
+        span_eval<SF, 2>(frac[1], tyv[2], inv_h, Nys[2], dNys[2]);
+        span_eval<SF, 3>(frac[1], tyv[3], inv_h, Nys[3], dNys[3]);
+      }
+      Ny = Nys[0];
+      dNy = dNys[0];
 #pragma unroll
-      for (int a = 0; a < D; ++a) {
-        const float vn = am * vinc[a] + prm.alpha * (in.v[a * n + slot] + dvinc[a]);
-        out.v[a * n + j] = vn;
-        out.x[a * n + j] = xp[a] + prm.dt * vinc[a];
-        mv[a] = m * vn;
+      for (int s = 1; s < SPAN; ++s) {
+        Ny = sy == s ? Nys[s] : Ny;
+        dNy = sy == s ? dNys[s] : dNy;
       }
     }
-    bool fast = true;  // own cell and the whole window inside the grid
+    const float m = cur.m;
 #pragma unroll
-    for (int a = 0; a < D; ++a)
-      fast = fast && w.bin[a] == c[a] && w.bin[a] + w.first[a] >= 0 && w.bin[a] + w.first[a] + E - 1 < g.num_cells[a];
-    int base = w.bin[0] + w.first[0] - n0[0];
-    if (D > 1) base += NXP * (w.bin[1] + w.first[1] - n0[1]);
-    if (D > 2) base += NXP * NY * (w.bin[2] + w.first[2] - n0[2]);
-    const bool own = (w.bin[0] == c[0]) && (D < 2 || w.bin[1] == c[1]) && (D < 3 || w.bin[2] == c[2]);
+    for (int sx = 0; sx < SPAN; ++sx) {
+      const float nxy = Nx[sx] * Ny;
+      const float nxym = nxy * m;
+      float mvxy[3], p[3], q[3];
+      // f_a = sum_b A_ab g_b with g = (dNx Ny Nz, Nx dNy Nz, Nx Ny dNz):
+      // f_a = p_a Nz + q_a dNz
+      const float dxy = dNx[sx] * Ny, dyx = dNy * Nx[sx];
 #pragma unroll
-    for (int ey = 0; ey < EY; ++ey) {
+      for (int a = 0; a < D; ++a) {
+        mvxy[a] = nxym * cur.v[a];
+        p[a] = sym_at(cur.A, a, 0) * dxy;
+        if (D > 1) p[a] += sym_at(cur.A, a, 1) * dyx;
+        q[a] = D > 2 ? sym_at(cur.A, a, 2) * nxy : 0.f;
+      }
 #pragma unroll
-      for (int ex = 0; ex < E; ++ex) {
-        const float nxy = w.N[0][ex] * w.N[1][ey];
-        const float dxy = w.dN[0][ex] * w.N[1][ey];
-        const float dyx = w.dN[1][ey] * w.N[0][ex];
+      for (int sz = 0; sz < SZH; ++sz) {
+        float *r = acc[sx * SZH + sz];
+        if (MODE == 0) {
+          r[0] += nxym * Nz[sz];
 #pragma unroll
-        for (int ez = 0; ez < EZ; ++ez) {
-          const int li = base + ex + NXP * ey + NXP * NY * ez;
-          float wgt = nxy * w.N[2][ez];
-          float g0 = dxy * w.N[2][ez];
-          float g1 = dyx * w.N[2][ez];
-          float g2 = w.dN[2][ez] * nxy;
-          if (!fast) {
-            const int e3[3] = {ex, ey, ez};
-            if (!own) {  // particle outside its listed cell: straight to global memory
-              scatter_entry_global<D, SF, MODE>(prm, g, nodes, nn, w, e3, m, mv, A);
-              continue;
-            }
-            int nb[3] = {0, 0, 0}, off[3] = {0, 0, 0};
-            bool inside = true;
-#pragma unroll
-            for (int a = 0; a < D; ++a) {
-              off[a] = w.first[a] + e3[a];
-              nb[a] = w.bin[a] + off[a];
-              inside = inside && nb[a] >= 0 && nb[a] < g.num_cells[a];
-            }
-            if (!inside) {
-              // the reference bound-checks only the linear node index
-              const int64_t lin = lin_index(g, nb, D);
-              if (!prm.wrap_quirk || lin < 0 || lin >= nn) continue;
-              int ids[3], types[3] = {0, 0, 0};
-              node_ids<D>(g, lin, ids);
-#pragma unroll
-              for (int a = 0; a < D; ++a) types[a] = cubic_node_type(ids[a], g.num_cells[a], prm.node_types_built);
-              float gr[3];
-              entry_weight<D, SF>(w, off, types, g.inv_cell_size, wgt, gr);
-              g0 = gr[0];
-              g1 = gr[1];
-              g2 = gr[2];
-            }
+          for (int a = 0; a < D; ++a) {
+            r[1 + a] += mvxy[a] * Nz[sz];
+            r[1 + D + a] += p[a] * Nz[sz];
+            if (D > 2) r[1 + D + a] += q[a] * dNz[sz];
           }
-          if (MODE == 0) {
-            atomicAdd(&tile[li], wgt * m);
+        } else {
 #pragma unroll
-            for (int a = 0; a < D; ++a) {
-              atomicAdd(&tile[(1 + a) * NT + li], wgt * mv[a]);
-              float f = A[a][0] * g0;
-              if (D > 1) f += A[a][1] * g1;
-              if (D > 2) f += A[a][2] * g2;
-              atomicAdd(&tile[(1 + D + a) * NT + li], f);
-            }
-          } else {
-#pragma unroll
-            for (int a = 0; a < D; ++a) atomicAdd(&tile[a * NT + li], wgt * mv[a]);
-          }
+          for (int a = 0; a < D; ++a) r[a] += mvxy[a] * Nz[sz];
         }
       }
     }
+    cur = nxt;
   }
+
+  // in-grid sums into the LDS tile, one node offset (sx, sz) per phase; the
+  // waves holding that z offset write
+#pragma unroll
+  for (int sx = 0; sx < SPAN; ++sx)
+#pragma unroll
+    for (int sz = 0; sz < SZ; ++sz) {
+      __syncthreads();
+      if (sz / SZH == zh && x_in[sx] && y_in && z_in[sz]) {
+        const int li = (lx + sx) + NXP * (sy + NY * (lz + sz));
+#pragma unroll
+        for (int k = 0; k < NV; ++k) tile[k * NT + li] += acc[sx * SZH + sz % SZH][k];
+      }
+    }
+
+  // wrap-around deposits of node offsets outside the grid; with typed cubic
+  // splines the weights differ (wrapped node's types) and p2g_wrap_kernel
+  // deposits them instead
+  if (!all_in && prm.wrap_quirk && !(SF == PMPM_SF_CUBIC && prm.node_types_built)) {
+#pragma unroll
+    for (int sx = 0; sx < SPAN; ++sx)
+#pragma unroll
+      for (int s = 0; s < SZH; ++s) {
+        const int sz = zh * SZH + s;
+        if (x_in[sx] && y_in && z_in[sz]) continue;
+        const int nb[3] = {c[0] + NLO + sx, c[1] + NLO + sy, c[2] + NLO + sz};
+        deposit_wrapped<D, MODE>(g, nodes, nb, acc[sx * SZH + s]);
+      }
+  }
+
   __syncthreads();
-  constexpr int NTN = NX * NY * NZ;
+  constexpr int NX = G::NX, NTN = G::NTN;
   float *slice = partials + (int64_t)blockIdx.x * NV * NTN;
   if (t == 0) flags[blockIdx.x] = 1;
-  for (int i = t; i < NTN; i += kTileThreads) {
+  for (int i = t; i < NTN; i += G::THREADS) {
     const int ix = i % NX;
     const int iy = (i / NX) % NY;
     const int iz = i / (NX * NY);
-    const int nb[3] = {n0[0] + ix, n0[1] + iy, n0[2] + iz};
     const int li = ix + NXP * (iy + NY * iz);
-    bool inside = nb[0] >= 0 && nb[0] < g.num_cells[0];
-    if (D > 1) inside = inside && nb[1] >= 0 && nb[1] < g.num_cells[1];
-    if (D > 2) inside = inside && nb[2] >= 0 && nb[2] < g.num_cells[2];
-    if (inside) {
 #pragma unroll
-      for (int v = 0; v < NV; ++v) slice[v * NTN + i] = tile[v * NT + li];
-      continue;
+    for (int k = 0; k < NV; ++k) slice[k * NTN + i] = tile[k * NT + li];
+  }
+}
+
+// Cells whose cubic window can wrap into the grid through the linear index:
+// x in {0, nx-2, nx-1} (any y, z) and, in 3D, y in {0, ny-2, ny-1} (other
+// x). Offsets outside the grid along the slowest axis always leave the index
+// range, so no other cell wraps. Returns false for w past the set.
+template <int D> __device__ inline bool wrap_cell(const pmpm_grid &g, int w, int *c) {
+  const int nx = g.num_cells[0], ny = D > 1 ? g.num_cells[1] : 1, nz = D > 2 ? g.num_cells[2] : 1;
+  if (D < 2) return false;
+  auto pick = [](int i, int len, int &v) {  // i-th of {0, len-2, len-1}, each cell once
+    v = i == 0 ? 0 : len - 3 + i;
+    return i == 0 || v > 0;
+  };
+  const int nxs = 3 * ny * (D > 2 ? nz : 1);
+  if (w < nxs) {
+    c[1] = (w / 3) % ny;
+    c[2] = D > 2 ? w / (3 * ny) : 0;
+    return pick(w % 3, nx, c[0]);
+  }
+  if (D < 3) return false;
+  w -= nxs;
+  const int mid = nx - 3;  // x in [1, nx-3]
+  if (mid <= 0 || w >= 3 * mid * nz) return false;
+  c[0] = 1 + (w / 3) % mid;
+  c[2] = w / (3 * mid);
+  return pick(w % 3, ny, c[1]);
+}
+template <int D> __host__ inline int wrap_cell_count(const pmpm_grid &g) {
+  const int nx = g.num_cells[0], ny = D > 1 ? g.num_cells[1] : 1, nz = D > 2 ? g.num_cells[2] : 1;
+  if (D < 2) return 0;
+  return 3 * ny * (D > 2 ? nz : 1) + (D > 2 && nx > 3 ? 3 * (nx - 3) * nz : 0);
+}
+
+// Wrap-around deposits of typed cubic splines: one wave per cell that can
+// wrap, lane = node offset (sx, sy, sz) of the cell's span. A lane whose node
+// lies outside the grid but whose linear index is in range re-sums the cell's
+// particles with the types of the node that index reaches
+// (shapefunction.cpp:96-112) and adds the sums there with global atomics.
+template <int D, int MODE>
+__global__ __launch_bounds__(64) void p2g_wrap_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles in,
+                                                      pmpm_particles out, pmpm_spatial part, pmpm_nodes nodes) {
+  constexpr int SPAN = 4, NLO = -1;
+  constexpr int NV = MODE == 0 ? 1 + 2 * D : D;
+  int c[3] = {0, 0, 0};
+  if (!wrap_cell<D>(g, blockIdx.x, c)) return;
+  const int cell = (int)lin_index(g, c, D);
+  const int ce = part.cell_end[cell];
+  const int cs = part.cell_start[cell];
+  if (ce <= cs) return;
+  const int lane = threadIdx.x;
+  const int s[3] = {lane % SPAN, D > 1 ? (lane / SPAN) % SPAN : 0, D > 2 ? lane / (SPAN * SPAN) : 0};
+  const int nlanes = D == 3 ? 64 : (D == 2 ? 16 : 4);
+  if (lane >= nlanes) return;
+  int nb[3] = {0, 0, 0};
+  bool inside = true;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    nb[a] = c[a] + NLO + s[a];
+    inside = inside && nb[a] >= 0 && nb[a] < g.num_cells[a];
+  }
+  const int nn = g.num_cells_total;
+  const int64_t lin = lin_index(g, nb, D);
+  if (inside || lin < 0 || lin >= nn) return;
+  int ids[3], type[3] = {0, 0, 0};
+  node_ids<D>(g, lin, ids);
+#pragma unroll
+  for (int a = 0; a < D; ++a) type[a] = cubic_node_type(ids[a], g.num_cells[a], prm.node_types_built);
+
+  const int n = in.n;
+  const float inv_h = g.inv_cell_size;
+  float acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.f;
+  for (int j = cs; j < ce; ++j) {
+    const int slot = part.sorted[j];
+    float frac[3] = {0.f, 0.f, 0.f};
+    bool own = true;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      const float ta = cell_coord(in.x[a * n + slot], g.origin[a], inv_h);
+      const int ba = (int)ta;
+      own = own && ba == c[a];
+      frac[a] = __fsub_rn(ta, (float)ba);
     }
-    // wrap-around deposits, reduced over the tile: one atomic per field
-    const int64_t lin = lin_index(g, nb, D);
-    if (lin < 0 || lin >= nn) continue;
-    const int node = (int)lin;
+    if (!own) continue;  // deposited whole by p2g_stray
+    P2GRec r;
+    p2g_load<D, MODE>(in, out, n, slot, j, r);
+    float N[3] = {1.f, 1.f, 1.f}, dN[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < D; ++a) cubic_eval(__fsub_rn(frac[a], (float)(NLO + s[a])), type[a], inv_h, N[a], dN[a]);
+    const float wm = ((N[0] * N[1]) * N[2]) * r.m;
+    const float gr[3] = {(dN[0] * N[1]) * N[2], (dN[1] * N[0]) * N[2], (dN[2] * N[0]) * N[1]};
     if (MODE == 0) {
-      const float mm = tile[li];
-      if (mm != 0.f) unsafeAtomicAdd(&nodes.mass[node], mm);
+      acc[0] += wm;
 #pragma unroll
       for (int a = 0; a < D; ++a) {
-        const float pm = tile[(1 + a) * NT + li];
-        if (pm != 0.f) unsafeAtomicAdd(&nodes.mom[a * nn + node], pm);
-        const float f = tile[(1 + D + a) * NT + li];
-        if (f != 0.f) unsafeAtomicAdd(&nodes.mom_nt[a * nn + node], f);
+        acc[1 + a] += wm * r.v[a];
+#pragma unroll
+        for (int b = 0; b < D; ++b) acc[1 + D + a] += sym_at(r.A, a, b) * gr[b];
       }
     } else {
 #pragma unroll
-      for (int a = 0; a < D; ++a) {
-        const float pm = tile[a * NT + li];
-        if (pm != 0.f) unsafeAtomicAdd(&nodes.mom_nt[a * nn + node], pm);
-      }
+      for (int a = 0; a < D; ++a) acc[a] += wm * r.v[a];
     }
   }
+  deposit_wrapped<D, MODE>(g, nodes, nb, acc);
 }
 
 // Geometry of the P2G tiles for the grid kernel's gather.
@@ -597,6 +887,8 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
 }
 
 // MODE 0: USL G2P (usl_inline.h:245-332) + return map.
+// MODE 1: MUSL first map: x, v of particle j updated from the grid and
+//         written to slot j of `out` (cell order); nothing else.
 // MODE 2: MUSL second map: L, F, V from the re-mapped velocities + return map
 //         (x, v of slot j of `out` were written by the first map).
 template <int D, int SF, int MODE>
@@ -607,21 +899,23 @@ __global__ __launch_bounds__(256) void g2p_kernel(pmpm_params prm, pmpm_grid g, 
   if (j >= n) return;
   const int nn = g.num_cells_total;
   const int src = part.sorted[j];
-  const int dst = reorder ? j : src;
+  const int dst = (reorder || MODE == 1) ? j : src;
   PW<D, SF> w;
   float xp[3];
   particle_weights<D, SF>(prm, g, in.x, n, src, w, xp);
   float vinc[3] = {0.f, 0.f, 0.f}, dvinc[3] = {0.f, 0.f, 0.f};
   float L[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   if (MODE == 0) gather_grid<D, SF, true, true>(g, nodes, nn, w, vinc, dvinc, L);
+  else if (MODE == 1) gather_grid<D, SF, true, false>(g, nodes, nn, w, vinc, dvinc, L);
   else gather_grid<D, SF, false, true>(g, nodes, nn, w, vinc, dvinc, L);
-  if (MODE == 0) {
+  if (MODE == 0 || MODE == 1) {
     const float am = (float)(1.0 - (double)prm.alpha);
 #pragma unroll
     for (int a = 0; a < D; ++a) {
       out.v[a * n + dst] = am * vinc[a] + prm.alpha * (in.v[a * n + src] + dvinc[a]);
       out.x[a * n + dst] = xp[a] + prm.dt * vinc[a];
     }
+    if (MODE == 1) return;
   }
   // F = (I + L dt) F, V = det(F) V0
   float Fo[9], Fn[9];
@@ -760,39 +1054,54 @@ __global__ void shape_function_kernel(pmpm_grid g, int types_built, const float 
     default: return PMPM_ERR_INVALID;                                             \
   }
 
-TileGeom tile_geom(int dim, int sf, const pmpm_grid &g, int gather) {
+template <int D, int SF> TileGeom tile_geom_of(const pmpm_grid &g, int gather) {
+  using G = P2GGeom<D, SF>;
   TileGeom tg{};
-  const int T3[3] = {16, 4, 2}, T2[3] = {16, 8, 1}, T1[3] = {128, 1, 1};
-  const int *T = dim == 3 ? T3 : (dim == 2 ? T2 : T1);
-  const int span = (sf == PMPM_SF_LINEAR) ? 2 : 4;
-  tg.nlo = (sf == PMPM_SF_LINEAR) ? 0 : -1;
+  const int T[3] = {G::TX, G::TY, G::TZ}, N[3] = {G::NX, G::NY, G::NZ};
+  tg.nlo = G::NLO;
   for (int a = 0; a < 3; ++a) {
-    tg.T[a] = a < dim ? T[a] : 1;
-    tg.N[a] = a < dim ? T[a] + span - 1 : 1;
-    tg.tiles[a] = a < dim ? (int)ceil_div(g.num_cells[a], T[a]) : 1;
+    tg.T[a] = a < D ? T[a] : 1;
+    tg.N[a] = a < D ? N[a] : 1;
+    tg.tiles[a] = a < D ? (int)ceil_div(g.num_cells[a], T[a]) : 1;
   }
   tg.gather = gather;
-  tg.nv = gather == 2 ? dim : 1 + 2 * dim;
+  tg.nv = gather == 2 ? D : 1 + 2 * D;
   return tg;
+}
+TileGeom tile_geom(int dim, int sf, const pmpm_grid &g, int gather) {
+  switch (dim * 3 + sf) {
+    case 3: return tile_geom_of<1, PMPM_SF_LINEAR>(g, gather);
+    case 4: return tile_geom_of<1, PMPM_SF_QUADRATIC>(g, gather);
+    case 5: return tile_geom_of<1, PMPM_SF_CUBIC>(g, gather);
+    case 6: return tile_geom_of<2, PMPM_SF_LINEAR>(g, gather);
+    case 7: return tile_geom_of<2, PMPM_SF_QUADRATIC>(g, gather);
+    case 8: return tile_geom_of<2, PMPM_SF_CUBIC>(g, gather);
+    case 9: return tile_geom_of<3, PMPM_SF_LINEAR>(g, gather);
+    case 10: return tile_geom_of<3, PMPM_SF_QUADRATIC>(g, gather);
+    default: return tile_geom_of<3, PMPM_SF_CUBIC>(g, gather);
+  }
 }
 static size_t tile_count(const TileGeom &tg) { return (size_t)tg.tiles[0] * tg.tiles[1] * tg.tiles[2]; }
 
-template <int D, int SF, int MODE>
-void launch_p2g_tile(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
-                     const pmpm_spatial &part, const pmpm_nodes &nodes, float *partials, int32_t *flags,
-                     hipStream_t s) {
-  const int tx = (int)ceil_div(g.num_cells[0], Tile<D>::X);
-  const int ty = D > 1 ? (int)ceil_div(g.num_cells[1], Tile<D>::Y) : 1;
-  const int tz = D > 2 ? (int)ceil_div(g.num_cells[2], Tile<D>::Z) : 1;
-  hipLaunchKernelGGL((p2g_tile_kernel<D, SF, MODE>), dim3(tx * ty * tz), dim3(kTileThreads), 0, s, prm, g, in, out,
-                     part, nodes, tx, ty, partials, flags);
-}
 template <int D, int SF>
 void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
                 const pmpm_spatial &part, const pmpm_nodes &nodes, int mode, float *partials, int32_t *flags,
                 hipStream_t s) {
-  if (mode == 0) launch_p2g_tile<D, SF, 0>(prm, g, in, out, part, nodes, partials, flags, s);
-  else launch_p2g_tile<D, SF, 1>(prm, g, in, out, part, nodes, partials, flags, s);
+  const TileGeom tg = tile_geom_of<D, SF>(g, 1);
+  const dim3 grid((unsigned)tile_count(tg)), block(P2GGeom<D, SF>::THREADS);
+  if (mode == 0)
+    hipLaunchKernelGGL((p2g_kernel<D, SF, 0>), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
+                       tg.tiles[1], partials, flags);
+  else
+    hipLaunchKernelGGL((p2g_kernel<D, SF, 1>), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
+                       tg.tiles[1], partials, flags);
+  const int wcells = wrap_cell_count<D>(g);
+  if (SF == PMPM_SF_CUBIC && prm.node_types_built && prm.wrap_quirk && wcells > 0) {
+    if (mode == 0)
+      hipLaunchKernelGGL((p2g_wrap_kernel<D, 0>), dim3(wcells), dim3(64), 0, s, prm, g, in, out, part, nodes);
+    else
+      hipLaunchKernelGGL((p2g_wrap_kernel<D, 1>), dim3(wcells), dim3(64), 0, s, prm, g, in, out, part, nodes);
+  }
 }
 template <int D, int SF>
 void launch_g2p(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
@@ -800,6 +1109,8 @@ void launch_g2p(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles
   const unsigned blocks = (unsigned)ceil_div(in.n, 256);
   if (mode == 0)
     hipLaunchKernelGGL((g2p_kernel<D, SF, 0>), dim3(blocks), dim3(256), 0, s, prm, g, in, out, part, nodes, reorder);
+  else if (mode == 1)
+    hipLaunchKernelGGL((g2p_kernel<D, SF, 1>), dim3(blocks), dim3(256), 0, s, prm, g, in, out, part, nodes, reorder);
   else
     hipLaunchKernelGGL((g2p_kernel<D, SF, 2>), dim3(blocks), dim3(256), 0, s, prm, g, in, out, part, nodes, reorder);
 }
@@ -876,12 +1187,16 @@ int do_g2p(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in,
   return PMPM_OK;
 }
 
-// First map + re-map into the tile partials; a grid_kernel pass with
-// gather = 2 completes mom_nt.
+// First map (x, v into `out` in cell order) + re-map of m v into the tile
+// partials; a grid_kernel pass with gather = 2 completes mom_nt.
 int do_musl_map(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in,
                 const pmpm_particles &out, const pmpm_spatial &part, const pmpm_nodes &nodes, hipStream_t s) {
   int rc = reserve_tiles(ctx, prm, g);
   if (rc != PMPM_OK) return rc;
+  if (in.n > 0) {
+    PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_g2p, prm, g, in, out, part, nodes, 1, 1, s);
+    PMPM_LAUNCH_CHECK();
+  }
   if ((rc = zero_fill(nodes.mom_nt, (int64_t)g.num_cells_total * prm.dim, s)) != PMPM_OK) return rc;
   PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, in, out, part, nodes, 1, ctx->partials,
                     ctx->tile_flags, s);
