@@ -32,7 +32,7 @@ METRIC = "particle-updates/s at 1M DP particles; adjoint/forward time ratio"
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic HBM bytes per particle of each entry point (FP32, 3D, cubic,
-# Drucker-Prager, MUSL); derivation in DESIGN.md.
+# Drucker-Prager, MUSL): the particle fields each one must read and write.
 BYTES_PER_PARTICLE = {
     "partition": 76,     # hash: x 12 + key 4; 3 radix passes x (key 4 + key/val 8 in + 8 out); ranges 4
     "p2g": 64,           # sorted 4, x 12, v 12, m 4, V 4, stress 24, + node flush
@@ -51,14 +51,16 @@ def parse():
     ap.add_argument("--cell-size", type=float, default=0.01)
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward time ratio")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="before the workload, run 3 plain device copies of MIB MiB (calibrates HBM byte counters)")
     return ap.parse_args()
 
 
 class Workload:
-    """Config-2 column collapse held in torch tensors on the GPU, driven through
-    the C ABI (ctypes)."""
+    """The 3D Drucker-Prager column collapse held in torch tensors on the GPU,
+    driven through the C ABI (ctypes)."""
 
     def __init__(self, per_axis=3, h=0.01, seed=0):
         import torch
@@ -194,11 +196,47 @@ def cpu_baseline(spec, steps):
                       f"{dt:.1f} s"}
 
 
+def adjoint_ratio(steps, every=20):
+    """The differentiable 3D Drucker-Prager column: h = 0.02 (51 x 26 x 26
+    nodes), 8 particles per cell (40,000 particles), USL, cubic, dt = 1e-4,
+    checkpoints every 20 steps. Loss = mean squared distance of the final
+    positions to a target run (phi 32 deg, c 50 Pa, E 1.2e6); gradient with
+    respect to (phi, c, E) from (28 deg, 0, 1e6). Returns the backward time per
+    step (segment recomputation included) over the forward time per step."""
+    import torch
+    from pyroclastmpm_amd.diff import DiffMPM, simulate
+    from tests import scenarios as S
+
+    spec = S.column(h=0.02, per_axis=2, dt=1e-4, jitter=0.1, seed=0, musl=False)
+    model = DiffMPM(3, spec.origin, spec.end, 0.02, spec.positions, 8, dt=1e-4, checkpoint_every=every)
+    with torch.no_grad():
+        target = simulate(model, torch.tensor([32.0, 50.0, 1.2e6], dtype=torch.float64), steps)
+    theta = torch.tensor([28.0, 0.0, 1e6], dtype=torch.float64, requires_grad=True)
+    x = simulate(model, theta, min(steps, 2 * every))  # warm-up (kernels, workspaces)
+    ((x - target) ** 2).mean().backward()
+    theta.grad = None
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    torch.cuda.synchronize()
+    ev[0].record()
+    x = simulate(model, theta, steps)
+    ev[1].record()
+    loss = ((x - target) ** 2).mean()
+    loss.backward()
+    ev[2].record()
+    torch.cuda.synchronize()
+    fwd, bwd = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+    return {"metric": "adjoint/forward time ratio", "value": round(bwd / fwd, 3), "unit": "ratio",
+            "higher_is_better": False, "forward_ms_per_step": round(fwd / steps, 4),
+            "backward_ms_per_step": round(bwd / steps, 4), "steps": steps, "checkpoint_every": every,
+            "particles": model.n, "loss": float(loss), "grad": [float(v) for v in theta.grad],
+            "config": "3D Drucker-Prager column, h 0.02, 8 per cell, USL, cubic, dt 1e-4"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+              "(the multi-GPU slab decomposition is not built)", file=sys.stderr)
         sys.exit(2)
     import torch
 
@@ -265,6 +303,12 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if not args.no_adjoint:
+        try:
+            line["adjoint"] = adjoint_ratio(args.adjoint_steps)
+            print("bench: adjoint " + json.dumps(line["adjoint"]), file=sys.stderr, flush=True)
+        except Exception as e:  # reported, the throughput line stands on its own
+            line["adjoint"] = {"value": None, "error": str(e)}
     print(json.dumps(line), flush=True)
 
 

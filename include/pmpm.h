@@ -245,6 +245,33 @@ int pmpm_step(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid, 
               const pmpm_particles *a, pmpm_particles *b, pmpm_spatial *part,
               pmpm_nodes *nodes, pmpm_stream_t stream);
 
+/* ---- adjoint of one USL step (differentiable form; the reference has no
+ * counterpart: it is the reverse mode of the step pmpm_step(musl = 0) runs,
+ * usl.cpp:69-115, for the hand-written adjoint kernels of P2G, the grid
+ * update, G2P and the return maps).
+ * Adjoints of the particle state, SoA as pmpm_particles. */
+typedef struct pmpm_adjoint {
+  float *x;       /* [dim][n] */
+  float *v;       /* [dim][n] */
+  float *stress;  /* [6][n] */
+  float *F;       /* [dim*dim][n] row-major */
+  float *eps_e;   /* [6][n] */
+} pmpm_adjoint;
+
+/* Given the state `state` at the start of a USL step and the adjoint of the
+ * step's output `out_bar` (in the output's order: slot j is the j-th particle
+ * of the cell order of `state`, as pmpm_step writes it), writes the adjoint of
+ * `state` into `in_bar` (state's order) and adds the gradient with respect to
+ * each material's p[0..3] to param_bar[4 * material + k]. The step's
+ * partition, P2G and grid update are recomputed into `part` and `nodes`. The
+ * active sets (return-map branch, slip clamps, mass cutoff) are those of the
+ * forward step; no smoothing. Supported: linear elastic and Drucker-Prager
+ * materials, gravity and node-domain conditions; anything else returns
+ * PMPM_ERR_UNSUPPORTED. */
+int pmpm_adjoint_step(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid,
+                      const pmpm_particles *state, const pmpm_adjoint *out_bar, pmpm_adjoint *in_bar,
+                      float *param_bar, pmpm_spatial *part, pmpm_nodes *nodes, pmpm_stream_t stream);
+
 /* Build identification string (static storage). */
 const char *pmpm_version(void);
 

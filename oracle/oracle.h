@@ -31,7 +31,9 @@
 // oracle) and on the dimension D (1, 2, 3).
 //
 // Pinning (what checks each function against something independent of this
-// file) is listed per function in DESIGN.md and enforced by tests/test_oracle_*.
+// file): tests/test_oracle_goldens.py (the reference's unit-test values) and
+// tests/test_oracle_closed_forms.py (closed forms and invariants); each test
+// names the function it pins.
 #pragma once
 
 #include <algorithm>

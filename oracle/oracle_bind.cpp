@@ -242,6 +242,8 @@ template <class R, int D> void bind_sim(py::module &m, const std::string &name) 
              if (k == "shear_modulus") return py::float_((double)mt.shear_modulus);
              if (k == "lame_modulus") return py::float_((double)mt.lame_modulus);
              if (k == "eta_overline") return py::float_((double)mt.eta_overline);
+             if (k == "M") return py::float_((double)mt.M);
+             if (k == "Pt") return py::float_((double)mt.Pt);
              throw std::invalid_argument("unknown material field " + k);
            })
       .def("set_material",
@@ -249,6 +251,12 @@ template <class R, int D> void bind_sim(py::module &m, const std::string &name) 
              auto &mt = s.materials.at(mat_id);
              if (k == "eps_e") mt.eps_e = to_mats<R, 3>(a);
              else if (k == "stress_ref") mt.stress_ref = to_mats<R, 3>(a);
+             // scalar parameters (finite-difference checks of parameter gradients)
+             else if (k == "bulk_modulus") mt.bulk_modulus = (R)a.cast<double>();
+             else if (k == "shear_modulus") mt.shear_modulus = (R)a.cast<double>();
+             else if (k == "lame_modulus") mt.lame_modulus = (R)a.cast<double>();
+             else if (k == "M") mt.M = (R)a.cast<double>();
+             else if (k == "Pt") mt.Pt = (R)a.cast<double>();
              else throw std::invalid_argument("unknown material field " + k);
            })
       .def("grid_info",

@@ -47,6 +47,10 @@ class Spatial(C.Structure):
                 ("bins", vp)]
 
 
+class Adjoint(C.Structure):
+    _fields_ = [("x", vp), ("v", vp), ("stress", vp), ("F", vp), ("eps_e", vp)]
+
+
 class Material(C.Structure):
     _fields_ = [("kind", C.c_int32), ("flags", C.c_int32), ("p", C.c_float * 16)]
 
@@ -83,6 +87,8 @@ PROTOTYPES = {
     "pmpm_stress_update": (C.c_int, [P(Params), C.c_int32, P(Particles), vp]),
     "pmpm_step": (C.c_int, [vp, P(Params), P(Grid), C.c_int32, P(Particles), P(Particles), P(Spatial), P(Nodes),
                             vp]),
+    "pmpm_adjoint_step": (C.c_int, [vp, P(Params), P(Grid), P(Particles), P(Adjoint), P(Adjoint), vp, P(Spatial),
+                                    P(Nodes), vp]),
     "pmpm_version": (C.c_char_p, []),
 }
 

@@ -22,12 +22,20 @@ struct pmpm_context {
   size_t partials_cap = 0;       // floats
   int32_t *tile_flags = nullptr;  // 1 = tile had particles this launch
   size_t tile_flags_cap = 0;
+  // adjoint step: per-particle scatter records (15 floats, cell order) and
+  // node adjoints (1 + 2 dim floats per node)
+  float *adj_rec = nullptr;
+  size_t adj_rec_cap = 0;
+  float *adj_nodes = nullptr;
+  size_t adj_nodes_cap = 0;
 };
 
 namespace pmpm {
 // Grows the tile-partial workspace (hipMalloc on first use of a larger grid;
 // run one step before capturing a graph).
 int ensure_partials(pmpm_context *ctx, size_t floats, size_t tiles);
+// Grows a context-owned float buffer to at least `floats`.
+int ensure_buffer(float *&buf, size_t &cap, size_t floats);
 }  // namespace pmpm
 
 namespace pmpm {

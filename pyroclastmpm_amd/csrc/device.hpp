@@ -70,6 +70,26 @@ template <int SF> __device__ inline void spline_eval(float r, int type, float in
   else cubic_eval(r, type, inv_h, N, dN);
 }
 
+// Second derivative d2N/dx2 of the same pieces (for position adjoints): the
+// linear spline's is 0, the quadratic's -2 and 1 (times inv_h^2), the cubic
+// piece's 2 c2 + 6 c3 r.
+template <int SF> __device__ inline float spline_d2(float r, int type, float inv_h) {
+  const float h2 = inv_h * inv_h;
+  if (SF == PMPM_SF_LINEAR) return 0.f;
+  if (SF == PMPM_SF_QUADRATIC) {
+    const float a = fabsf(r);
+    return a < 0.5f ? -2.f * h2 : (a < 1.5f ? h2 : 0.f);
+  }
+  int k;
+  if (r >= -2.f && r < -1.f) k = 0;
+  else if (r >= -1.f && r < 0.f) k = 1;
+  else if (r >= 0.f && r < 1.f) k = 2;
+  else if (r >= 1.f && r < 2.f) k = 3;
+  else return 0.f;
+  const float *c = kCubic[type][k];
+  return h2 * (2.f * c[2] + 6.f * c[3] * r);
+}
+
 // t = (x - origin) * inv_h rounded after each operation, exactly as the
 // reference evaluates it (no fused multiply-add: the cell index and the
 // relative coordinates must come out bit-identical).

@@ -211,8 +211,21 @@ int pmpm::ensure_partials(pmpm_context *ctx, size_t floats, size_t tiles) {
   return PMPM_OK;
 }
 
+int pmpm::ensure_buffer(float *&buf, size_t &cap, size_t floats) {
+  if (floats > cap) {
+    (void)hipFree(buf);
+    buf = nullptr;
+    cap = 0;
+    PMPM_HIP_CHECK(hipMalloc(&buf, floats * sizeof(float)));
+    cap = floats;
+  }
+  return PMPM_OK;
+}
+
 extern "C" int pmpm_context_destroy(pmpm_context *ctx) {
   if (!ctx) return PMPM_OK;
+  (void)hipFree(ctx->adj_rec);
+  (void)hipFree(ctx->adj_nodes);
   (void)hipFree(ctx->partials);
   (void)hipFree(ctx->tile_flags);
   (void)hipFree(ctx->keys_tmp);

@@ -18,16 +18,18 @@
 // particle x node is stored.
 #include <hip/hip_runtime.h>
 
+#include "adjoint_math.hpp"
 #include "context.hpp"
 #include "device.hpp"
 
 namespace pmpm {
 
 // P2G workgroup. Lane = (cell, y node offset sy of the cell's node span); the
-// CX = 64 / SY cells of a wave lie along x. 3D: 8 waves, a pair per z layer
-// of 4 consecutive layers (tile CX x 1 x 4 cells); the two waves of a pair
-// take the lower and upper half of the z node offsets. 1D/2D: 4 waves, 4
-// consecutive runs along x (tile 4 CX x 1 cells).
+// CX = 64 / SY cells of a wave lie along x. 3D: 4 waves on 4 consecutive z
+// layers (tile CX x 1 x 4 cells); ZH waves per layer would each take a share
+// of the z node offsets (fewer registers, more redundant weight work; 1 is
+// faster on gfx950). 1D/2D: 4 waves, 4 consecutive runs along x (tile
+// 4 CX x 1 cells).
 
 // LDS row pitch of the P2G node tile: at least nx, and an odd multiple of cx
 // so the SY rows of cx lanes that one flush instruction covers fall into
@@ -41,9 +43,9 @@ template <int D, int SF> struct P2GGeom {
   static constexpr int NLO = Spline<SF>::NLO;    // the first of them, relative to the cell
   static constexpr int SY = D > 1 ? SPAN : 1;    // lanes per cell
   static constexpr int SZ = D > 2 ? SPAN : 1;    // z node offsets per cell
-  static constexpr int ZH = D > 2 ? 2 : 1;       // waves sharing a z layer
+  static constexpr int ZH = 1;                   // waves sharing a z layer
   static constexpr int SZH = SZ / ZH;            // z node offsets per lane
-  static constexpr int WAVES = D > 2 ? 8 : 4;
+  static constexpr int WAVES = 4;
   static constexpr int THREADS = 64 * WAVES;
   static constexpr int CX = 64 / SY;             // cells per wave
   static constexpr int TX = D == 3 ? CX : CX * WAVES;
@@ -94,28 +96,37 @@ __device__ inline void particle_weights(const pmpm_params &prm, const pmpm_grid 
   }
 }
 
+// Where a gather reads node velocities from: the node arrays (lo = 0,
+// strides of the grid, component stride nn) or a box of them staged in LDS.
+struct NodeSrc {
+  const float *vel, *dvel;  // vel_nt and dvel, component-major
+  int cstride;              // distance between components
+  int lo[3], stride[3];     // node (i, j, k) sits at sum (id_a - lo_a) stride_a
+};
+
 // Grid gather in the reference's window order (y outer, x, z inner). The
 // reference skips entries outside the grid on any axis (usl_inline.h:281-293);
 // here such entries get zero weights and a clamped, valid node index, which
 // leaves every sum unchanged and keeps the loop free of branches. Nodes below
 // the mass cutoff carry zero velocities, which is equivalent to skipping them.
 template <int D, int SF, bool WANT_V, bool WANT_L>
-__device__ inline void gather_grid(const pmpm_grid &g, const pmpm_nodes &nodes, int nn, const PW<D, SF> &w,
-                                   float *vinc, float *dvinc, float *L) {
+__device__ inline void gather_grid(const pmpm_grid &g, const NodeSrc &src, const PW<D, SF> &w, float *vinc,
+                                   float *dvinc, float *L) {
   constexpr int E = Spline<SF>::ENTRIES;
   constexpr int EY = D > 1 ? E : 1;
   constexpr int EZ = D > 2 ? E : 1;
+  const int nn = src.cstride;
   float N[3][4], dN[3][4];
   int off[3][4];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const int stride = a == 0 ? 1 : (a == 1 ? g.num_cells[0] : g.num_cells[0] * g.num_cells[1]);
+    const int stride = src.stride[a];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       if (a < D && e < E) {
         const int i = w.bin[a] + w.first[a] + e;
         const bool ok = i >= 0 && i < g.num_cells[a];
-        off[a][e] = min(max(i, 0), g.num_cells[a] - 1) * stride;
+        off[a][e] = (min(max(i, 0), g.num_cells[a] - 1) - src.lo[a]) * stride;
         N[a][e] = ok ? w.N[a][e] : 0.f;
         dN[a][e] = ok ? w.dN[a][e] : 0.f;
       } else {
@@ -142,11 +153,11 @@ __device__ inline void gather_grid(const pmpm_grid &g, const pmpm_nodes &nodes, 
         gr[2] = (dN[2][ez] * N[0][ex]) * N[1][ey];
         float vnt[3];
 #pragma unroll
-        for (int a = 0; a < D; ++a) vnt[a] = nodes.vel_nt[a * nn + node];
+        for (int a = 0; a < D; ++a) vnt[a] = src.vel[a * nn + node];
         if (WANT_V) {
 #pragma unroll
           for (int a = 0; a < D; ++a) {
-            const float dv = nodes.dvel[a * nn + node];
+            const float dv = src.dvel[a * nn + node];
             dvinc[a] += wgt * dv;
             vinc[a] += wgt * vnt[a];
           }
@@ -269,18 +280,43 @@ __device__ inline void span_eval(float frac, int type, float inv_h, float &N, fl
   }
 }
 
+// span_eval for the NS consecutive offsets S0 .. S0 + NS - 1.
+template <int SF, int S0, int NS>
+__device__ inline void span_evals(float frac, const int *type, float inv_h, float *N, float *dN) {
+  if constexpr (NS > 0) {
+    span_eval<SF, S0>(frac, type[0], inv_h, N[0], dN[0]);
+    span_evals<SF, S0 + 1, NS - 1>(frac, type + 1, inv_h, N + 1, dN + 1);
+  }
+}
+
 // Per-particle inputs of a P2G deposit.
+//   MODE 0/1: mass, velocity, -V sigma.
+//   MODE 2 (adjoint scatter): v = adjoint of the PIC velocity, dv = adjoint of
+//   the FLIP increment, Lb = adjoint of the velocity gradient (row-major 3x3),
+//   read from the record array `rec` in cell order.
 struct P2GRec {
   float x[3], v[3], m;
   Sym A;  // -V sigma (MODE 0)
+  float dv[3], Lb[9];
 };
 template <int D, int MODE>
-__device__ inline void p2g_load(const pmpm_particles &in, const pmpm_particles &out, int n, int slot, int j,
-                                P2GRec &r) {
+__device__ inline void p2g_load(const pmpm_particles &in, const pmpm_particles &out, const float *rec, int n,
+                                int slot, int j, P2GRec &r) {
 #pragma unroll
   for (int a = 0; a < D; ++a) {
     r.x[a] = in.x[a * n + slot];
-    r.v[a] = MODE == 0 ? in.v[a * n + slot] : out.v[a * n + j];
+    r.v[a] = MODE == 0 ? in.v[a * n + slot] : (MODE == 1 ? out.v[a * n + j] : rec[a * n + j]);
+  }
+  if (MODE == 2) {
+    r.m = 1.f;
+    r.A = sym_zero();
+#pragma unroll
+    for (int a = 0; a < D; ++a) r.dv[a] = rec[(3 + a) * n + j];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int b = 0; b < D; ++b) r.Lb[3 * a + b] = rec[(6 + 3 * a + b) * n + j];
+    return;
   }
   r.m = in.mass[slot];
   if (MODE == 0) {
@@ -359,17 +395,19 @@ __device__ inline void deposit_wrapped(const pmpm_grid &g, const pmpm_nodes &nod
 // deposit lands on the wrapped node, weighted with that node's cubic types.
 // Those sums go to the node arrays with one global atomic per cell, node
 // offset and field (wrap_quirk), or are dropped.
-// (3 waves per SIMD: the 3D cubic accumulators fit in 168 registers)
+// (at least 2 waves per SIMD: the 3D cubic accumulators need up to 256
+// registers)
 template <int D, int SF, int MODE>
-__global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_waves_per_eu(3))) void p2g_kernel(pmpm_params prm, pmpm_grid g,
+__global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_waves_per_eu(2))) void p2g_kernel(pmpm_params prm, pmpm_grid g,
                                                                       pmpm_particles in, pmpm_particles out,
                                                                       pmpm_spatial part, pmpm_nodes nodes,
                                                                       int tiles_x, int tiles_y,
                                                                       float *__restrict__ partials,
-                                                                      int32_t *__restrict__ flags) {
+                                                                      int32_t *__restrict__ flags,
+                                                                      const float *__restrict__ rec) {
   using G = P2GGeom<D, SF>;
   constexpr int SPAN = G::SPAN, NLO = G::NLO, SZ = G::SZ, SZH = G::SZH, NE = SPAN * SZH;
-  constexpr int NV = MODE == 0 ? 1 + 2 * D : D;
+  constexpr int NV = MODE == 0 ? 1 + 2 * D : (MODE == 1 ? D : 2 * D);
   constexpr int NXP = G::NXP, NY = G::NY, NT = G::NT;
   __shared__ float tile[NV * NT];
 
@@ -426,26 +464,32 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
     for (int k = 0; k < NV; ++k) acc[e][k] = 0.f;
 
   // cubic node types of this lane's span nodes (own particles share them)
-  int tx[SPAN], tz[SZH], tyv[SPAN];
+  int tx[SPAN], tz[SZH];
 #pragma unroll
-  for (int s = 0; s < SPAN; ++s) {
-    tx[s] = cubic_node_type(c[0] + NLO + s, g.num_cells[0], prm.node_types_built);
-    tyv[s] = D > 1 ? cubic_node_type(c[1] + NLO + s, g.num_cells[1], prm.node_types_built) : 0;
-  }
+  for (int s = 0; s < SPAN; ++s) tx[s] = cubic_node_type(c[0] + NLO + s, g.num_cells[0], prm.node_types_built);
 #pragma unroll
   for (int s = 0; s < SZH; ++s)
     tz[s] = D > 2 ? cubic_node_type(c[2] + NLO + zh * SZH + s, g.num_cells[2], prm.node_types_built) : 0;
+  // this lane's one y offset: its cubic piece (3 - sy) and the node's type
+  // fix the coefficients for every particle of the cell
+  const float ry0 = (float)(NLO + sy);
+  float yc[4] = {0.f, 0.f, 0.f, 0.f};
+  if (SF == PMPM_SF_CUBIC && D > 1) {
+    const int ty = cubic_node_type(c[1] + NLO + sy, g.num_cells[1], prm.node_types_built);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) yc[k] = kCubic[ty][3 - sy][k];
+  }
 
   // the next particle's inputs are loaded while the current one is summed
   int j = cs;
   int slot = j < ce ? part.sorted[j] : 0;
   int slot_next = j + 1 < ce ? part.sorted[j + 1] : 0;
   P2GRec cur;
-  if (j < ce) p2g_load<D, MODE>(in, out, n, slot, j, cur);
+  if (j < ce) p2g_load<D, MODE>(in, out, rec, n, slot, j, cur);
   for (; j < ce; ++j) {
     const int jn = j + 1;
     P2GRec nxt = cur;
-    if (jn < ce) p2g_load<D, MODE>(in, out, n, slot_next, jn, nxt);
+    if (jn < ce) p2g_load<D, MODE>(in, out, rec, n, slot_next, jn, nxt);
     const int slot_cur = slot;
     slot = slot_next;
     slot_next = jn + 1 < ce ? part.sorted[jn + 1] : 0;
@@ -460,45 +504,32 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
       frac[a] = __fsub_rn(ta, (float)ba);
     }
     if (!own) {
-      if (zh == 0) p2g_stray<D, SF, MODE>(prm, g, nodes, in.x, n, slot_cur, sy, cur.m, cur.v, cur.A);
+      // (MODE 2 is only ever called on a partition of the same positions)
+      if (MODE != 2 && zh == 0) p2g_stray<D, SF, MODE>(prm, g, nodes, in.x, n, slot_cur, sy, cur.m, cur.v, cur.A);
       cur = nxt;
       continue;
     }
     float Nx[SPAN], dNx[SPAN], Nz[SZH], dNz[SZH], Ny = 1.f, dNy = 0.f;
-    span_eval<SF, 0>(frac[0], tx[0], inv_h, Nx[0], dNx[0]);
-    span_eval<SF, 1>(frac[0], tx[1], inv_h, Nx[1], dNx[1]);
-    if constexpr (SPAN > 2) {
-      span_eval<SF, 2>(frac[0], tx[2], inv_h, Nx[2], dNx[2]);
-      span_eval<SF, 3>(frac[0], tx[3], inv_h, Nx[3], dNx[3]);
-    }
+    span_evals<SF, 0, SPAN>(frac[0], tx, inv_h, Nx, dNx);
     if constexpr (D > 2) {
-      // this wave's half of the z offsets (zh is uniform per wave)
-      if (zh == 0) {
-        span_eval<SF, 0>(frac[2], tz[0], inv_h, Nz[0], dNz[0]);
-        if constexpr (SZH > 1) span_eval<SF, 1>(frac[2], tz[1], inv_h, Nz[1], dNz[1]);
-      } else {
-        span_eval<SF, SZH>(frac[2], tz[0], inv_h, Nz[0], dNz[0]);
-        if constexpr (SZH > 1) span_eval<SF, SZH + 1>(frac[2], tz[1], inv_h, Nz[1], dNz[1]);
+      if constexpr (G::ZH == 1) {
+        span_evals<SF, 0, SZH>(frac[2], tz, inv_h, Nz, dNz);
+      } else {  // this wave's share of the z offsets (zh is uniform per wave)
+        if (zh == 0) span_evals<SF, 0, SZH>(frac[2], tz, inv_h, Nz, dNz);
+        else span_evals<SF, SZH, SZH>(frac[2], tz, inv_h, Nz, dNz);
       }
     } else {
       Nz[0] = 1.f;
       dNz[0] = 0.f;
     }
     if constexpr (D > 1) {
-      // all span offsets along y, then this lane's one (selects, no branches)
-      float Nys[SPAN], dNys[SPAN];
-      span_eval<SF, 0>(frac[1], tyv[0], inv_h, Nys[0], dNys[0]);
-      span_eval<SF, 1>(frac[1], tyv[1], inv_h, Nys[1], dNys[1]);
-      if constexpr (SPAN > 2) {
-        span_eval<SF, 2>(frac[1], tyv[2], inv_h, Nys[2], dNys[2]);
-        span_eval<SF, 3>(frac[1], tyv[3], inv_h, Nys[3], dNys[3]);
-      }
-      Ny = Nys[0];
-      dNy = dNys[0];
-#pragma unroll
-      for (int s = 1; s < SPAN; ++s) {
-        Ny = sy == s ? Nys[s] : Ny;
-        dNy = sy == s ? dNys[s] : dNy;
+      const float r = __fsub_rn(frac[1], ry0);
+      if (SF == PMPM_SF_CUBIC) {  // the Horner form of cubic_eval on this lane's piece
+        Ny = ((yc[3] * r + yc[2]) * r + yc[1]) * r + yc[0];
+        dNy = inv_h * ((3.f * yc[3] * r + 2.f * yc[2]) * r + yc[1]);
+        if (sy == 0 && !(r < 2.f)) Ny = dNy = 0.f;  // frac = 1 - 2^-24 (see span_eval)
+      } else {
+        spline_eval<SF>(r, 0, inv_h, Ny, dNy);
       }
     }
     const float m = cur.m;
@@ -510,12 +541,21 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
       // f_a = sum_b A_ab g_b with g = (dNx Ny Nz, Nx dNy Nz, Nx Ny dNz):
       // f_a = p_a Nz + q_a dNz
       const float dxy = dNx[sx] * Ny, dyx = dNy * Nx[sx];
+      float wd[3];
 #pragma unroll
       for (int a = 0; a < D; ++a) {
         mvxy[a] = nxym * cur.v[a];
-        p[a] = sym_at(cur.A, a, 0) * dxy;
-        if (D > 1) p[a] += sym_at(cur.A, a, 1) * dyx;
-        q[a] = D > 2 ? sym_at(cur.A, a, 2) * nxy : 0.f;
+        if (MODE == 2) {
+          // adjoint: sum_c Lb_ca g_c = p_a Nz + q_a dNz
+          wd[a] = nxy * cur.dv[a];
+          p[a] = cur.Lb[a] * dxy;
+          if (D > 1) p[a] += cur.Lb[3 + a] * dyx;
+          q[a] = D > 2 ? cur.Lb[6 + a] * nxy : 0.f;
+        } else {
+          p[a] = sym_at(cur.A, a, 0) * dxy;
+          if (D > 1) p[a] += sym_at(cur.A, a, 1) * dyx;
+          q[a] = D > 2 ? sym_at(cur.A, a, 2) * nxy : 0.f;
+        }
       }
 #pragma unroll
       for (int sz = 0; sz < SZH; ++sz) {
@@ -527,6 +567,13 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
             r[1 + a] += mvxy[a] * Nz[sz];
             r[1 + D + a] += p[a] * Nz[sz];
             if (D > 2) r[1 + D + a] += q[a] * dNz[sz];
+          }
+        } else if (MODE == 2) {
+#pragma unroll
+          for (int a = 0; a < D; ++a) {
+            r[a] += (mvxy[a] + p[a]) * Nz[sz];
+            if (D > 2) r[a] += q[a] * dNz[sz];
+            r[D + a] += wd[a] * Nz[sz];
           }
         } else {
 #pragma unroll
@@ -554,7 +601,8 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
   // wrap-around deposits of node offsets outside the grid; with typed cubic
   // splines the weights differ (wrapped node's types) and p2g_wrap_kernel
   // deposits them instead
-  if (!all_in && prm.wrap_quirk && !(SF == PMPM_SF_CUBIC && prm.node_types_built)) {
+  // (MODE 2 transposes G2P, which never reads wrapped nodes)
+  if (MODE != 2 && !all_in && prm.wrap_quirk && !(SF == PMPM_SF_CUBIC && prm.node_types_built)) {
 #pragma unroll
     for (int sx = 0; sx < SPAN; ++sx)
 #pragma unroll
@@ -664,7 +712,7 @@ __global__ __launch_bounds__(64) void p2g_wrap_kernel(pmpm_params prm, pmpm_grid
     }
     if (!own) continue;  // deposited whole by p2g_stray
     P2GRec r;
-    p2g_load<D, MODE>(in, out, n, slot, j, r);
+    p2g_load<D, MODE>(in, out, nullptr, n, slot, j, r);
     float N[3] = {1.f, 1.f, 1.f}, dN[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int a = 0; a < D; ++a) cubic_eval(__fsub_rn(frac[a], (float)(NLO + s[a])), type[a], inv_h, N[a], dN[a]);
@@ -891,23 +939,75 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
 //         written to slot j of `out` (cell order); nothing else.
 // MODE 2: MUSL second map: L, F, V from the re-mapped velocities + return map
 //         (x, v of slot j of `out` were written by the first map).
+// The 256 particles of a G2P workgroup are consecutive in cell order, so
+// their windows cover a small box of nodes: the box's node velocities are
+// staged in LDS once (when it holds at most kG2PStage nodes) and every window
+// read hits LDS instead of L2.
+constexpr int kG2PThreads = 256;
+constexpr int kG2PStage = 1536;
+
 template <int D, int SF, int MODE>
-__global__ __launch_bounds__(256) void g2p_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles in, pmpm_particles out,
-                                                  pmpm_spatial part, pmpm_nodes nodes, int reorder) {
+__global__ __launch_bounds__(kG2PThreads) void g2p_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles in,
+                                                          pmpm_particles out, pmpm_spatial part, pmpm_nodes nodes,
+                                                          int reorder) {
+  constexpr int E = Spline<SF>::ENTRIES;
+  constexpr int NF = (MODE == 2 ? 1 : 2) * D;  // staged floats per node: vel_nt (+ dvel)
+  __shared__ float stage[NF * kG2PStage];
+  __shared__ int box[6];
   const int n = in.n;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+  const int t = threadIdx.x;
+  const int j = blockIdx.x * kG2PThreads + t;
+  const bool valid = j < n;
   const int nn = g.num_cells_total;
-  const int src = part.sorted[j];
+  if (t < 3) {
+    box[t] = 0x7fffffff;
+    box[3 + t] = -0x7fffffff;
+  }
+  const int src = valid ? part.sorted[j] : 0;
   const int dst = (reorder || MODE == 1) ? j : src;
   PW<D, SF> w;
   float xp[3];
   particle_weights<D, SF>(prm, g, in.x, n, src, w, xp);
+  __syncthreads();
+  if (valid) {
+    // the clamped window (gather_grid reads clamped node indices)
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      const int top = g.num_cells[a] - 1;
+      atomicMin(&box[a], min(max(w.bin[a] + w.first[a], 0), top));
+      atomicMax(&box[3 + a], min(max(w.bin[a] + w.first[a] + E - 1, 0), top));
+    }
+  }
+  __syncthreads();
+  int lo[3] = {0, 0, 0}, ext[3] = {1, 1, 1};
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    lo[a] = box[a];
+    ext[a] = max(box[3 + a] - box[a] + 1, 1);
+  }
+  const int64_t count = (int64_t)ext[0] * ext[1] * ext[2];
+  NodeSrc ns;
+  if (count <= kG2PStage) {  // uniform over the workgroup
+    for (int i = t; i < (int)count; i += kG2PThreads) {
+      const int ix = i % ext[0], iy = (i / ext[0]) % ext[1], iz = i / (ext[0] * ext[1]);
+      const int node = (lo[0] + ix) + (lo[1] + iy) * g.num_cells[0] + (lo[2] + iz) * g.num_cells[0] * g.num_cells[1];
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        stage[k * kG2PStage + i] = nodes.vel_nt[k * nn + node];
+        if (MODE != 2) stage[(D + k) * kG2PStage + i] = nodes.dvel[k * nn + node];
+      }
+    }
+    ns = NodeSrc{stage, stage + D * kG2PStage, kG2PStage, {lo[0], lo[1], lo[2]}, {1, ext[0], ext[0] * ext[1]}};
+  } else {
+    ns = NodeSrc{nodes.vel_nt, nodes.dvel, nn, {0, 0, 0}, {1, g.num_cells[0], g.num_cells[0] * g.num_cells[1]}};
+  }
+  __syncthreads();
+  if (!valid) return;
   float vinc[3] = {0.f, 0.f, 0.f}, dvinc[3] = {0.f, 0.f, 0.f};
   float L[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  if (MODE == 0) gather_grid<D, SF, true, true>(g, nodes, nn, w, vinc, dvinc, L);
-  else if (MODE == 1) gather_grid<D, SF, true, false>(g, nodes, nn, w, vinc, dvinc, L);
-  else gather_grid<D, SF, false, true>(g, nodes, nn, w, vinc, dvinc, L);
+  if (MODE == 0) gather_grid<D, SF, true, true>(g, ns, w, vinc, dvinc, L);
+  else if (MODE == 1) gather_grid<D, SF, true, false>(g, ns, w, vinc, dvinc, L);
+  else gather_grid<D, SF, false, true>(g, ns, w, vinc, dvinc, L);
   if (MODE == 0 || MODE == 1) {
     const float am = (float)(1.0 - (double)prm.alpha);
 #pragma unroll
@@ -959,7 +1059,10 @@ __global__ __launch_bounds__(256) void g2p_kernel(pmpm_params prm, pmpm_grid g, 
     out.color[dst] = (uint8_t)color;
     out.flags[dst] = pf;
     out.id[dst] = in.id[src];
-    if (in.stress_ref && out.stress_ref) store_sym(out.stress_ref, n, dst, load_sym(in.stress_ref, n, src));
+    // reference stresses only matter (and only travel) when a material reads them
+    bool any_ref = false;
+    for (int k = 0; k < prm.num_materials; ++k) any_ref = any_ref || (prm.materials[k].flags & PMPM_MATF_HAS_STRESS_REF);
+    if (any_ref && in.stress_ref && out.stress_ref) store_sym(out.stress_ref, n, dst, load_sym(in.stress_ref, n, src));
   }
 }
 
@@ -1065,7 +1168,7 @@ template <int D, int SF> TileGeom tile_geom_of(const pmpm_grid &g, int gather) {
     tg.tiles[a] = a < D ? (int)ceil_div(g.num_cells[a], T[a]) : 1;
   }
   tg.gather = gather;
-  tg.nv = gather == 2 ? D : 1 + 2 * D;
+  tg.nv = gather == 2 ? D : (gather == 3 ? 2 * D : 1 + 2 * D);  // 3: adjoint scatter
   return tg;
 }
 TileGeom tile_geom(int dim, int sf, const pmpm_grid &g, int gather) {
@@ -1083,20 +1186,24 @@ TileGeom tile_geom(int dim, int sf, const pmpm_grid &g, int gather) {
 }
 static size_t tile_count(const TileGeom &tg) { return (size_t)tg.tiles[0] * tg.tiles[1] * tg.tiles[2]; }
 
+// mode 0: P2G, 1: MUSL re-map, 2: adjoint scatter of the records `rec`
 template <int D, int SF>
 void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
                 const pmpm_spatial &part, const pmpm_nodes &nodes, int mode, float *partials, int32_t *flags,
-                hipStream_t s) {
+                hipStream_t s, const float *rec = nullptr) {
   const TileGeom tg = tile_geom_of<D, SF>(g, 1);
   const dim3 grid((unsigned)tile_count(tg)), block(P2GGeom<D, SF>::THREADS);
   if (mode == 0)
     hipLaunchKernelGGL((p2g_kernel<D, SF, 0>), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
-                       tg.tiles[1], partials, flags);
-  else
+                       tg.tiles[1], partials, flags, rec);
+  else if (mode == 1)
     hipLaunchKernelGGL((p2g_kernel<D, SF, 1>), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
-                       tg.tiles[1], partials, flags);
+                       tg.tiles[1], partials, flags, rec);
+  else
+    hipLaunchKernelGGL((p2g_kernel<D, SF, 2>), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
+                       tg.tiles[1], partials, flags, rec);
   const int wcells = wrap_cell_count<D>(g);
-  if (SF == PMPM_SF_CUBIC && prm.node_types_built && prm.wrap_quirk && wcells > 0) {
+  if (mode != 2 && SF == PMPM_SF_CUBIC && prm.node_types_built && prm.wrap_quirk && wcells > 0) {
     if (mode == 0)
       hipLaunchKernelGGL((p2g_wrap_kernel<D, 0>), dim3(wcells), dim3(64), 0, s, prm, g, in, out, part, nodes);
     else
@@ -1274,6 +1381,529 @@ extern "C" int pmpm_stress_update(const pmpm_params *prm, int32_t mat_id, pmpm_p
     case 2: hipLaunchKernelGGL(stress_update_kernel<2>, dim3(blocks), dim3(256), 0, s, *prm, mat_id, *p); break;
     default: hipLaunchKernelGGL(stress_update_kernel<3>, dim3(blocks), dim3(256), 0, s, *prm, mat_id, *p); break;
   }
+  PMPM_LAUNCH_CHECK();
+  return PMPM_OK;
+}
+
+// ===========================================================================
+// Adjoint of one USL step.
+//
+// Forward (per step):  P2G   m_i = sum w m,  P_i = sum w m v,  f_i = -sum V sigma grad w
+//                      grid  Q_i = P_i + dt f_i, BCs, vnt_i = Q_i / m_i, dv_i = vnt_i - P_i / m_i
+//                      G2P   vpic = sum w vnt, dvp = sum w dv, L = sum grad w (x) vnt,
+//                            v' = (1 - a) vpic + a (v + dvp), x' = x + dt vpic,
+//                            F' = (I + dt L) F, (sigma', eps') = return map
+// The adjoint runs these backwards: g2p_adjoint_kernel (per particle: return
+// map VJP, F, v, x, and the position adjoint through the weights), the
+// adjoint scatter (p2g_kernel MODE 2) of the PIC/FLIP/L adjoints to the
+// nodes, grid_adjoint_kernel (per node: velocities, BCs, forward moment), and
+// p2g_adjoint_kernel (per particle: gather of the node adjoints into v, sigma,
+// V = det(F) V0 and x). The weights' position dependence enters through
+// grad w and its Hessian.
+// ===========================================================================
+namespace pmpm {
+
+// Second derivatives of the window weights (same entries and node types as
+// particle_weights).
+template <int D, int SF>
+__device__ inline void window_d2(const pmpm_params &prm, const pmpm_grid &g, const PW<D, SF> &w, float d2N[3][4]) {
+  constexpr int E = Spline<SF>::ENTRIES;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      d2N[a][e] = 0.f;
+      if (a < D && e < E) {
+        const float frac = __fsub_rn(w.t[a], (float)w.bin[a]);
+        const int o = w.first[a] + e;
+        const int type = cubic_node_type(w.bin[a] + o, g.num_cells[a], prm.node_types_built);
+        d2N[a][e] = spline_d2<SF>(__fsub_rn(frac, (float)o), type, g.inv_cell_size);
+      }
+    }
+}
+
+// Weight, gradient and Hessian (xx, yy, zz, xy, xz, yz) of one window entry
+// from per-axis values.
+__device__ inline void entry_derivs(const float *N, const float *dN, const float *d2N, float &wgt, float *gr,
+                                    float *H) {
+  wgt = (N[0] * N[1]) * N[2];
+  gr[0] = (dN[0] * N[1]) * N[2];
+  gr[1] = (dN[1] * N[0]) * N[2];
+  gr[2] = (dN[2] * N[0]) * N[1];
+  H[0] = d2N[0] * N[1] * N[2];
+  H[1] = N[0] * d2N[1] * N[2];
+  H[2] = N[0] * N[1] * d2N[2];
+  H[3] = dN[0] * dN[1] * N[2];
+  H[4] = dN[0] * N[1] * dN[2];
+  H[5] = N[0] * dN[1] * dN[2];
+}
+
+// Position adjoint through the G2P weights (same window and per-axis checks
+// as gather_grid): xbar_c += sum_i grad_c w (vnt . vpb + dv . dvb)
+//                                 + sum_ab d2w/dx_c dx_a Lb_ab vnt_b.
+template <int D, int SF>
+__device__ inline void g2p_weights_adjoint(const pmpm_params &prm, const pmpm_grid &g, const pmpm_nodes &nodes,
+                                           const PW<D, SF> &w, const float *vpb, const float *dvb, const float *Lb,
+                                           float *xbar) {
+  constexpr int E = Spline<SF>::ENTRIES;
+  constexpr int EY = D > 1 ? E : 1;
+  constexpr int EZ = D > 2 ? E : 1;
+  const int nn = g.num_cells_total;
+  float d2w[3][4];
+  window_d2<D, SF>(prm, g, w, d2w);
+  float N[3][4], dN[3][4], d2N[3][4];
+  int off[3][4];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int stride = a == 0 ? 1 : (a == 1 ? g.num_cells[0] : g.num_cells[0] * g.num_cells[1]);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (a < D && e < E) {
+        const int i = w.bin[a] + w.first[a] + e;
+        const bool ok = i >= 0 && i < g.num_cells[a];
+        off[a][e] = min(max(i, 0), g.num_cells[a] - 1) * stride;
+        N[a][e] = ok ? w.N[a][e] : 0.f;
+        dN[a][e] = ok ? w.dN[a][e] : 0.f;
+        d2N[a][e] = ok ? d2w[a][e] : 0.f;
+      } else {
+        off[a][e] = 0;
+        N[a][e] = 1.f;
+        dN[a][e] = 0.f;
+        d2N[a][e] = 0.f;
+      }
+    }
+  }
+#pragma unroll
+  for (int ey = 0; ey < EY; ++ey)
+#pragma unroll
+    for (int ex = 0; ex < E; ++ex)
+#pragma unroll
+      for (int ez = 0; ez < EZ; ++ez) {
+        const int node = off[0][ex] + off[1][ey] + off[2][ez];
+        const float n3[3] = {N[0][ex], N[1][ey], N[2][ez]}, d3[3] = {dN[0][ex], dN[1][ey], dN[2][ez]},
+                    h3[3] = {d2N[0][ex], d2N[1][ey], d2N[2][ez]};
+        float wgt, gr[3], H[6];
+        entry_derivs(n3, d3, h3, wgt, gr, H);
+        float vnt[3] = {0.f, 0.f, 0.f}, s = 0.f, u[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int b = 0; b < D; ++b) {
+          vnt[b] = nodes.vel_nt[b * nn + node];
+          s += vnt[b] * vpb[b] + nodes.dvel[b * nn + node] * dvb[b];
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+          for (int b = 0; b < D; ++b) u[a] += Lb[3 * a + b] * vnt[b];
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          float hu = 0.f;
+#pragma unroll
+          for (int a = 0; a < D; ++a) hu += H[sym_idx(c, a)] * u[a];
+          xbar[c] += gr[c] * s + hu;
+        }
+      }
+}
+
+// Wave sum of v (all 64 lanes take part).
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Per particle (cell order j, state slot src): return-map VJP, F, v, x, and
+// the records (vpic adjoint, FLIP adjoint, L adjoint) for the node scatter.
+template <int D, int SF>
+__global__ __launch_bounds__(256) void g2p_adjoint_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles st,
+                                                          pmpm_adjoint ob, pmpm_adjoint ib, pmpm_spatial part,
+                                                          pmpm_nodes nodes, float *__restrict__ rec,
+                                                          float *__restrict__ pbar) {
+  const int n = st.n;
+  const int nn = g.num_cells_total;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = j < n;
+  float pb[4] = {0.f, 0.f, 0.f, 0.f};
+  int color = -1;
+  if (valid) {
+    const int src = part.sorted[j];
+    PW<D, SF> w;
+    float xp[3];
+    particle_weights<D, SF>(prm, g, st.x, n, src, w, xp);
+    const NodeSrc ns{nodes.vel_nt, nodes.dvel, nn, {0, 0, 0}, {1, g.num_cells[0], g.num_cells[0] * g.num_cells[1]}};
+    float vpic[3] = {0.f, 0.f, 0.f}, dvp[3] = {0.f, 0.f, 0.f};
+    float L[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    gather_grid<D, SF, true, true>(g, ns, w, vpic, dvp, L);
+
+    float xo[3] = {0.f, 0.f, 0.f}, vo[3] = {0.f, 0.f, 0.f}, Fo[9], F[9], so6[6], eo6[6], s6[6], e6[6], sr6[6];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      xo[a] = ob.x[a * n + j];
+      vo[a] = ob.v[a * n + j];
+    }
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) {
+      Fo[k] = ob.F[k * n + j];
+      F[k] = st.F[k * n + src];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      so6[k] = ob.stress[k * n + j];
+      eo6[k] = ob.eps_e ? ob.eps_e[k * n + j] : 0.f;
+      s6[k] = st.stress[k * n + src];
+      e6[k] = st.eps_e ? st.eps_e[k * n + src] : 0.f;
+      sr6[k] = 0.f;
+    }
+    color = st.color[src];
+    const uint8_t pf = st.flags[src];
+
+    // return map (stress and elastic strain pass through unless updated)
+    float Lb[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, sb6[6], eb6[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { sb6[k] = 0.f; eb6[k] = 0.f; }
+    bool s_through = true, e_through = true;
+    if (color < prm.num_materials) {
+      const pmpm_material &m = prm.materials[color];
+      if (m.kind == PMPM_MAT_LINEAR_ELASTIC && (pf & PMPM_PF_ACTIVE)) {
+        le_vjp<D>(m.p, prm.dt, L, s6, so6, sb6, Lb, pb);
+        s_through = false;
+      } else if (m.kind == PMPM_MAT_DRUCKER_PRAGER) {
+        const bool history = (m.flags & PMPM_MATF_UPDATE_HISTORY) != 0;
+        if (st.stress_ref && (m.flags & PMPM_MATF_HAS_STRESS_REF))
+#pragma unroll
+          for (int k = 0; k < 6; ++k) sr6[k] = st.stress_ref[k * n + src];
+        dp_vjp<D>(m.p, m.p[4], (m.flags & PMPM_MATF_VELGRAD_IS_INCREMENT) != 0, history,
+                  (m.flags & PMPM_MATF_APEX_GUARD) != 0, prm.dt, L, sr6, e6, so6, eo6, eb6, Lb, pb);
+        s_through = false;  // the stress is recomputed from the strain
+        e_through = !history;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      if (s_through) sb6[k] += so6[k];
+      if (e_through) eb6[k] += eo6[k];
+    }
+
+    // F' = (I + dt L) F
+    float Fb[9];
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+#pragma unroll
+      for (int b = 0; b < D; ++b) {
+        float acc = 0.f;
+#pragma unroll
+        for (int a = 0; a < D; ++a) acc += ((a == k ? 1.f : 0.f) + prm.dt * L[3 * a + k]) * Fo[a * D + b];
+        Fb[k * D + b] = acc;
+      }
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        float acc = 0.f;
+#pragma unroll
+        for (int b = 0; b < D; ++b) acc += Fo[a * D + b] * F[k * D + b];
+        Lb[3 * a + k] += prm.dt * acc;
+      }
+
+    // v' = (1 - a) vpic + a (v + dvp), x' = x + dt vpic
+    const float al = prm.alpha, am = (float)(1.0 - (double)prm.alpha);
+    float vpb[3] = {0.f, 0.f, 0.f}, dvb[3] = {0.f, 0.f, 0.f}, xb[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      vpb[a] = am * vo[a] + prm.dt * xo[a];
+      dvb[a] = al * vo[a];
+      xb[a] = xo[a];
+    }
+    g2p_weights_adjoint<D, SF>(prm, g, nodes, w, vpb, dvb, Lb, xb);
+
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      ib.x[a * n + src] = xb[a];
+      ib.v[a * n + src] = al * vo[a];
+      rec[a * n + j] = vpb[a];
+      rec[(3 + a) * n + j] = dvb[a];
+    }
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) ib.F[k * n + src] = Fb[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      ib.stress[k * n + src] = sb6[k];
+      if (ib.eps_e) ib.eps_e[k * n + src] = eb6[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) rec[(6 + k) * n + j] = Lb[k];
+  }
+  // material parameter adjoints: one atomic per wave and material
+  for (int k = 0; k < prm.num_materials; ++k)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float s = wave_sum(color == k ? pb[q] : 0.f);
+      if ((threadIdx.x & 63) == 0 && s != 0.f) atomicAdd(&pbar[4 * k + q], s);
+    }
+}
+
+// Per node: adjoints of the node velocities (scattered in the tile partials)
+// -> adjoints of mass, moment and force. Boundary conditions in reverse list
+// order with the forward's active sets (read from the post-condition
+// values: a slip clamp passed a component iff it is nonzero on the kept side).
+template <int D>
+__global__ void grid_adjoint_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, TileGeom tg,
+                                    const float *__restrict__ partials, const int32_t *__restrict__ flags,
+                                    float *__restrict__ nbar) {
+  const int nn = g.num_cells_total;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  int ids[3];
+  node_ids<D>(g, i, ids);
+  float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  gather_partials<D>(tg, partials, flags, ids, acc);
+  const float m = nodes.mass[i];
+  float mb = 0.f, Pb[3] = {0.f, 0.f, 0.f}, Qb[3] = {0.f, 0.f, 0.f}, fb[3] = {0.f, 0.f, 0.f};
+  if (m > prm.small_mass_cutoff) {
+    float P[3] = {0.f, 0.f, 0.f}, Q[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      P[a] = nodes.mom[a * nn + i];
+      Q[a] = nodes.mom_nt[a * nn + i];
+      const float vb = acc[a], db = acc[D + a];
+      Qb[a] = (vb + db) / m;
+      Pb[a] = -db / m;
+      mb -= (vb * nodes.vel_nt[a * nn + i] + db * nodes.dvel[a * nn + i]) / m;
+    }
+    for (int k = prm.num_bcs - 1; k >= 0; --k) {
+      const pmpm_bc &bc = prm.bcs[k];
+      if (bc.kind == PMPM_BC_NODE_DOMAIN) {
+        if (m <= 0.00001f) continue;
+        for (int a = D - 1; a >= 0; --a) {
+          int mode = -1;
+          bool low = false;
+          if (ids[a] < 1) { mode = bc.i[a]; low = true; }
+          else if (ids[a] >= g.num_cells[a] - 1) mode = bc.i[3 + a];
+          if (mode == 0) {
+#pragma unroll
+            for (int b = 0; b < D; ++b) { Pb[b] = 0.f; Qb[b] = 0.f; }
+          } else if (mode == 1) {
+            if (!(low ? P[a] > 0.f : P[a] < 0.f)) Pb[a] = 0.f;
+            if (!(low ? Q[a] > 0.f : Q[a] < 0.f)) Qb[a] = 0.f;
+          }
+        }
+      } else if (bc.kind == PMPM_BC_GRAVITY) {
+        if (m <= 0.000000001f) continue;
+#pragma unroll
+        for (int a = 0; a < D; ++a) mb += bc.f[a] * prm.dt * Qb[a];
+      }
+    }
+    // Q = P + dt f
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      Pb[a] += Qb[a];
+      fb[a] = prm.dt * Qb[a];
+    }
+  }
+  nbar[i] = mb;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    nbar[(1 + a) * nn + i] = Pb[a];
+    nbar[(1 + D + a) * nn + i] = fb[a];
+  }
+}
+
+// d det(F) / dF (row-major D x D).
+template <int D> __device__ inline void cofactor(const float *F, float *C) {
+  if (D == 1) { C[0] = 1.f; return; }
+  if (D == 2) { C[0] = F[3]; C[1] = -F[2]; C[2] = -F[1]; C[3] = F[0]; return; }
+  C[0] = F[4] * F[8] - F[5] * F[7];
+  C[1] = F[5] * F[6] - F[3] * F[8];
+  C[2] = F[3] * F[7] - F[4] * F[6];
+  C[3] = F[2] * F[7] - F[1] * F[8];
+  C[4] = F[0] * F[8] - F[2] * F[6];
+  C[5] = F[1] * F[6] - F[0] * F[7];
+  C[6] = F[1] * F[5] - F[2] * F[4];
+  C[7] = F[2] * F[3] - F[0] * F[5];
+  C[8] = F[0] * F[4] - F[1] * F[3];
+}
+
+// Per particle: gather of the node adjoints through the P2G window (the same
+// entries as the forward deposit, linear-index wrap-around included) into the
+// adjoints of v, sigma, F (through V = det(F) V0) and x; added to ib.
+template <int D, int SF>
+__global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles st,
+                                                          pmpm_adjoint ib, pmpm_spatial part,
+                                                          const float *__restrict__ nbar) {
+  constexpr int E = Spline<SF>::ENTRIES;
+  constexpr int EY = D > 1 ? E : 1;
+  constexpr int EZ = D > 2 ? E : 1;
+  const int n = st.n;
+  const int nn = g.num_cells_total;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int src = part.sorted[j];
+  PW<D, SF> w;
+  float xp[3];
+  particle_weights<D, SF>(prm, g, st.x, n, src, w, xp);
+  float d2w[3][4];
+  window_d2<D, SF>(prm, g, w, d2w);
+  const float m = st.mass[src], V = st.volume[src];
+  float v[3] = {0.f, 0.f, 0.f}, S[3][3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) v[a] = st.v[a * n + src];
+  const Sym s = load_sym(st.stress, n, src);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) S[a][b] = sym_at(s, a, b);
+  float vb[3] = {0.f, 0.f, 0.f}, xb[3] = {0.f, 0.f, 0.f}, Mb[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f},
+                                                                         {0.f, 0.f, 0.f}};
+  float Vb = 0.f;
+#pragma unroll
+  for (int ey = 0; ey < EY; ++ey)
+#pragma unroll
+    for (int ex = 0; ex < E; ++ex)
+#pragma unroll
+      for (int ez = 0; ez < EZ; ++ez) {
+        const int e3[3] = {ex, ey, ez};
+        int nb[3] = {0, 0, 0}, off[3] = {0, 0, 0};
+        bool inside = true;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          off[a] = w.first[a] + e3[a];
+          nb[a] = w.bin[a] + off[a];
+          inside = inside && nb[a] >= 0 && nb[a] < g.num_cells[a];
+        }
+        const int64_t lin = lin_index(g, nb, D);
+        if (lin < 0 || lin >= nn || (!inside && !prm.wrap_quirk)) continue;
+        float N[3] = {1.f, 1.f, 1.f}, dN[3] = {0.f, 0.f, 0.f}, d2N[3] = {0.f, 0.f, 0.f};
+        if (inside) {
+#pragma unroll
+          for (int a = 0; a < D; ++a) {
+            N[a] = w.N[a][e3[a]];
+            dN[a] = w.dN[a][e3[a]];
+            d2N[a] = d2w[a][e3[a]];
+          }
+        } else {  // the wrapped node's types, as the forward deposit
+          int ids[3];
+          node_ids<D>(g, lin, ids);
+#pragma unroll
+          for (int a = 0; a < D; ++a) {
+            const int type = cubic_node_type(ids[a], g.num_cells[a], prm.node_types_built);
+            const float r = __fsub_rn(__fsub_rn(w.t[a], (float)w.bin[a]), (float)off[a]);
+            spline_eval<SF>(r, type, g.inv_cell_size, N[a], dN[a]);
+            d2N[a] = spline_d2<SF>(r, type, g.inv_cell_size);
+          }
+        }
+        const int node = (int)lin;
+        float wgt, gr[3], H[6];
+        entry_derivs(N, dN, d2N, wgt, gr, H);
+        const float mb = nbar[node];
+        float Pb[3] = {0.f, 0.f, 0.f}, fb[3] = {0.f, 0.f, 0.f}, vP = 0.f;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          Pb[a] = nbar[(1 + a) * nn + node];
+          fb[a] = nbar[(1 + D + a) * nn + node];
+          vP += v[a] * Pb[a];
+        }
+        float sf[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int b = 0; b < D; ++b)
+#pragma unroll
+          for (int a = 0; a < D; ++a) sf[b] += S[a][b] * fb[a];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          vb[a] += m * wgt * Pb[a];
+          Vb -= sf[a] * gr[a];
+#pragma unroll
+          for (int b = 0; b < D; ++b) Mb[a][b] -= V * fb[a] * gr[b];
+        }
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          float hs = 0.f;
+#pragma unroll
+          for (int b = 0; b < D; ++b) hs += H[sym_idx(c, b)] * sf[b];
+          xb[c] += m * gr[c] * (mb + vP) - V * hs;
+        }
+      }
+  float F[9], C[9];
+#pragma unroll
+  for (int k = 0; k < D * D; ++k) F[k] = st.F[k * n + src];
+  cofactor<D>(F, C);
+  const float V0 = st.volume0[src];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    ib.x[a * n + src] += xb[a];
+    ib.v[a * n + src] += vb[a];
+  }
+#pragma unroll
+  for (int k = 0; k < D * D; ++k) ib.F[k * n + src] += Vb * V0 * C[k];
+  // sigma is stored symmetric: an off-diagonal component feeds both entries
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int b = a; b < D; ++b) ib.stress[sym_idx(a, b) * n + src] += a == b ? Mb[a][a] : Mb[a][b] + Mb[b][a];
+}
+
+template <int D, int SF>
+void launch_adjoint_particles(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &st,
+                              const pmpm_adjoint &ob, const pmpm_adjoint &ib, const pmpm_spatial &part,
+                              const pmpm_nodes &nodes, float *rec, float *pbar, int which, const float *nbar,
+                              hipStream_t s) {
+  const unsigned blocks = (unsigned)ceil_div(st.n, 256);
+  if (which == 0)
+    hipLaunchKernelGGL((g2p_adjoint_kernel<D, SF>), dim3(blocks), dim3(256), 0, s, prm, g, st, ob, ib, part, nodes,
+                       rec, pbar);
+  else
+    hipLaunchKernelGGL((p2g_adjoint_kernel<D, SF>), dim3(blocks), dim3(256), 0, s, prm, g, st, ib, part, nbar);
+}
+
+}  // namespace pmpm
+
+extern "C" int pmpm_adjoint_step(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid,
+                                 const pmpm_particles *state, const pmpm_adjoint *out_bar, pmpm_adjoint *in_bar,
+                                 float *param_bar, pmpm_spatial *part, pmpm_nodes *nodes, pmpm_stream_t stream) {
+  if (!ctx || !params_ok(prm) || !grid || !particles_ok(state) || !out_bar || !in_bar || !param_bar || !part ||
+      !nodes)
+    return PMPM_ERR_INVALID;
+  const int n = state->n;
+  if (n == 0) return PMPM_OK;
+  if (!out_bar->x || !out_bar->v || !out_bar->F || !out_bar->stress || !in_bar->x || !in_bar->v || !in_bar->F ||
+      !in_bar->stress)
+    return PMPM_ERR_INVALID;
+  for (int k = 0; k < prm->num_materials; ++k) {
+    const int kind = prm->materials[k].kind;
+    if (kind != PMPM_MAT_LINEAR_ELASTIC && kind != PMPM_MAT_DRUCKER_PRAGER) return PMPM_ERR_UNSUPPORTED;
+  }
+  for (int k = 0; k < prm->num_bcs; ++k) {
+    const int kind = prm->bcs[k].kind;
+    if (kind != PMPM_BC_GRAVITY && kind != PMPM_BC_NODE_DOMAIN) return PMPM_ERR_UNSUPPORTED;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const pmpm_grid &g = *grid;
+  const int D = prm->dim;
+  // the forward step's partition, P2G and grid update
+  int rc = pmpm_partition(ctx, D, grid, state->x, n, part, stream);
+  if (rc != PMPM_OK) return rc;
+  if ((rc = do_p2g(ctx, *prm, g, *state, *part, *nodes, s)) != PMPM_OK) return rc;
+  if ((rc = launch_grid(ctx, *prm, g, *nodes, 1, 0, prm->num_bcs, 0, 1, s)) != PMPM_OK) return rc;
+  if ((rc = ensure_buffer(ctx->adj_rec, ctx->adj_rec_cap, (size_t)15 * n)) != PMPM_OK) return rc;
+  if ((rc = ensure_buffer(ctx->adj_nodes, ctx->adj_nodes_cap, (size_t)(1 + 2 * D) * g.num_cells_total)) != PMPM_OK)
+    return rc;
+  // G2P adjoint, then the scatter of its records to the nodes
+  PMPM_DISPATCH_DSF(D, prm->shape_fn, launch_adjoint_particles, *prm, g, *state, *out_bar, *in_bar, *part, *nodes,
+                    ctx->adj_rec, param_bar, 0, nullptr, s);
+  PMPM_LAUNCH_CHECK();
+  PMPM_DISPATCH_DSF(D, prm->shape_fn, launch_p2g, *prm, g, *state, *state, *part, *nodes, 2, ctx->partials,
+                    ctx->tile_flags, s, ctx->adj_rec);
+  PMPM_LAUNCH_CHECK();
+  // grid adjoint
+  const TileGeom tg = tile_geom(D, prm->shape_fn, g, 3);
+  const unsigned nblocks = (unsigned)ceil_div(g.num_cells_total, 256);
+  switch (D) {
+    case 1: hipLaunchKernelGGL(grid_adjoint_kernel<1>, dim3(nblocks), dim3(256), 0, s, *prm, g, *nodes, tg, ctx->partials, ctx->tile_flags, ctx->adj_nodes); break;
+    case 2: hipLaunchKernelGGL(grid_adjoint_kernel<2>, dim3(nblocks), dim3(256), 0, s, *prm, g, *nodes, tg, ctx->partials, ctx->tile_flags, ctx->adj_nodes); break;
+    default: hipLaunchKernelGGL(grid_adjoint_kernel<3>, dim3(nblocks), dim3(256), 0, s, *prm, g, *nodes, tg, ctx->partials, ctx->tile_flags, ctx->adj_nodes); break;
+  }
+  PMPM_LAUNCH_CHECK();
+  // P2G adjoint
+  PMPM_DISPATCH_DSF(D, prm->shape_fn, launch_adjoint_particles, *prm, g, *state, *out_bar, *in_bar, *part, *nodes,
+                    ctx->adj_rec, param_bar, 1, ctx->adj_nodes, s);
   PMPM_LAUNCH_CHECK();
   return PMPM_OK;
 }

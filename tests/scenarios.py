@@ -71,7 +71,7 @@ def dp_params(phi_deg, cohesion, psi_deg=0.0):
 
 # ------------------------------------------------------------------ scenarios
 def two_disc(alpha=0.99, musl=False, swirl=0.0):
-    """Config 1: 2D two-disc elastic impact, linear, USL (416 particles).
+    """2D two-disc elastic impact, linear, USL (416 particles).
     swirl > 0 adds a radial + shear velocity field inside each disc so the
     discs deform from the first step (used by parity checks: with rigid
     translation alone the stress is rounding noise until the impact)."""
@@ -92,7 +92,7 @@ def two_disc(alpha=0.99, musl=False, swirl=0.0):
 
 def column(h=0.01, per_axis=3, phi=30.0, cohesion=0.0, E=1e6, dt=1e-5, jitter=0.1, seed=0,
            domain=(1.0, 0.5, 0.5), col=(0.2, 0.5, 0.4), shape_function="cubic", musl=True):
-    """Config 2: 3D granular column collapse, Drucker-Prager, MUSL, cubic."""
+    """3D granular column collapse, Drucker-Prager, MUSL, cubic."""
     spacing = h / per_axis
     pos = box_lattice([0, 0, 0], col, spacing, jitter=jitter, seed=seed)
     M, Pt = dp_params(phi, cohesion)
@@ -106,7 +106,7 @@ def column(h=0.01, per_axis=3, phi=30.0, cohesion=0.0, E=1e6, dt=1e-5, jitter=0.
 
 
 def scoop(h=0.0125, per_axis=2, domain=(2.0, 1.0, 1.0), bed=0.4, dt=5e-6, seed=0):
-    """Config 3: 3D rigid box scoop into a mu(I) bed, quadratic B-spline, USL."""
+    """3D rigid box scoop into a mu(I) bed, quadratic B-spline, USL."""
     spacing = h / per_axis
     pos = box_lattice([0, 0, 0], [domain[0], domain[1], bed], spacing, jitter=0.0, seed=seed)
     return Spec(dim=3, origin=[0.0, 0.0, 0.0], end=list(domain), cell_size=h, shape_function="quadratic", dt=dt,

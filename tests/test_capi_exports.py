@@ -63,7 +63,7 @@ def test_invalid_arguments_rejected_on_host():
 
 def test_grid_matches_reference_construction():
     from pyroclastmpm_amd import capi
-    # config 3 grid: 2 x 1 x 1 m at h = 0.0125 (computed in FP32 as the reference)
+    # the 2 x 1 x 1 m scoop grid at h = 0.0125 (computed in FP32 as the reference)
     g = capi.make_grid([0, 0, 0], [2.0, 1.0, 1.0], 0.0125, 3)
     assert list(g.num_cells) == [161, 81, 81]
     g = capi.make_grid([0, 0, 0], [1.0, 0.5, 0.5], 0.01, 3)

@@ -1,0 +1,234 @@
+"""GPU checks of the adjoint of the USL step (pmpm_adjoint_step) and of the
+differentiable solver built on it (pyroclastmpm_amd.diff).
+
+  * one step, all state and parameter directions: <adjoint(c), delta> against
+    the central finite difference of <c, step(S + eps delta)> computed by the
+    FP64 oracle (relative error <= 2e-3; the adjoint runs in FP32);
+  * several steps with checkpoint segments: d<c, x_final>/d(phi, c, E) against
+    central finite differences of the GPU forward (relative step 1e-3; FP32,
+    relative error <= 2e-2) and of the FP64 oracle (relative step 1e-6,
+    relative error <= 1e-2).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from tests import scenarios as S
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _banner(msg):
+    print(f"\n[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+
+
+def _column_2d(h=0.05, per_axis=2, seed=3):
+    pos = S.box_lattice([0.0, 0.0], [0.2, 0.3], h / per_axis, jitter=0.1, seed=seed)
+    return pos, dict(dim=2, origin=[0.0, 0.0], end=[0.6, 0.4], cell_size=h, ppc=per_axis ** 2)
+
+
+def _sym6_to_mat(s6, D):
+    idx = {(0, 0): 0, (1, 1): 1, (2, 2): 2, (0, 1): 3, (1, 0): 3, (0, 2): 4, (2, 0): 4, (1, 2): 5, (2, 1): 5}
+    n = s6.shape[1]
+    m = np.zeros((n, 3, 3))
+    for (i, j), k in idx.items():
+        m[:, i, j] = s6[k]
+    return m
+
+
+def _mat_to_sym6(m):
+    return np.stack([m[:, 0, 0], m[:, 1, 1], m[:, 2, 2], m[:, 0, 1], m[:, 0, 2], m[:, 1, 2]])
+
+
+def _oracle_for(oracle_mod, model, geo, pos, theta, state):
+    """FP64 oracle holding `state` (numpy dict, the state's particle order)."""
+    from pyroclastmpm_amd.diff import dp_material
+
+    D = geo["dim"]
+    K, G, M, Pt, _ = dp_material(*theta, model.pois)
+    sim = getattr(oracle_mod, f"Sim{D}D_f64")(geo["origin"], geo["end"], geo["cell_size"], "cubic", model.dt,
+                                             geo["ppc"])
+    sim.set_particles(state["x"].T, np.zeros_like(state["x"].T), None, None)
+    sim.add_material("DP_rheo", {"density": 1500.0, "E": float(theta[2]), "pois": model.pois, "M": M, "Pt": Pt,
+                                 "dilatancy_angle": 0.0})
+    sim.add_bc("Gravity", {"gravity": model.gravity})
+    sim.add_bc("NodeDomain", {"face0_mode": [1] * D, "face1_mode": [1] * D})
+    sim.musl = False
+    sim.alpha = model.alpha
+    sim.initialize()
+    sim.set_material(0, "bulk_modulus", K)
+    sim.set_material(0, "shear_modulus", G)
+    return sim
+
+
+def _load(sim, st, D):
+    n = st["x"].shape[1]
+    sim.set("positions", st["x"].T)
+    sim.set("velocities", st["v"].T)
+    sim.set("stresses", _sym6_to_mat(st["stress"], D))
+    sim.set("F", st["F"].T.reshape(n, D, D))
+    sim.set("volumes", st["volume"][0])
+    sim.set("volumes_original", st["volume0"][0])
+    sim.set("masses", st["mass"][0])
+    sim.set_material(0, "eps_e", _sym6_to_mat(st["eps_e"], D))
+    sim.set_material(0, "stress_ref", np.zeros((n, 3, 3)))
+
+
+def _outputs(sim, D):
+    n = len(sim.get("positions"))
+    eps = np.asarray(sim.get_material(0, "eps_e"), dtype=np.float64).reshape(n, 3, 3)
+    return {"x": np.asarray(sim.get("positions")).T, "v": np.asarray(sim.get("velocities")).T,
+            "F": np.asarray(sim.get("F")).reshape(n, D * D).T,
+            "stress": _mat_to_sym6(np.asarray(sim.get("stresses")).reshape(n, 3, 3)),
+            "eps_e": _mat_to_sym6(eps)}
+
+
+def test_one_step_adjoint_matches_fp64_oracle(oracle_mod):
+    from pyroclastmpm_amd import capi
+    from pyroclastmpm_amd.diff import AdjointSet, DiffMPM
+    import ctypes as C
+
+    pos, geo = _column_2d()
+    D = geo["dim"]
+    model = DiffMPM(D, geo["origin"], geo["end"], geo["cell_size"], pos, geo["ppc"], dt=1e-4)
+    theta = (30.0, 0.0, 1e6)
+    _banner(f"one-step adjoint: 2D DP column, n={model.n}, 30 warm-up steps")
+    state, _ = model.forward(theta, 30)
+    torch.cuda.synchronize()
+    st = {k: v.double().cpu().numpy() for k, v in state.t.items() if k not in ("color", "flags", "id")}
+    n = model.n
+    rng = np.random.default_rng(0)
+    c = {"x": rng.standard_normal((D, n)), "v": rng.standard_normal((D, n)) * 1e-2,
+         "F": rng.standard_normal((D * D, n)), "stress": rng.standard_normal((6, n)) * 1e-4,
+         "eps_e": rng.standard_normal((6, n)) * 10.0}
+    if D == 2:  # zz, xz, yz are not state in 2D
+        for k in ("stress", "eps_e"):
+            c[k][[2, 4, 5]] = 0.0
+    # output order of the step = cell order of `state`
+    prm = model.params(theta)
+    L = capi.lib()
+    capi.check(L.pmpm_partition(model.ctx.ptr, D, C.byref(model.grid), state.t["x"].data_ptr(), n,
+                                C.byref(model.spatial), torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    order = model.sp["sorted"].long().cpu().numpy()
+    ob = AdjointSet(n, D)
+    for k in c:
+        ob.t[k].copy_(torch.from_numpy(c[k][:, order]).float())
+    ib = AdjointSet(n, D)
+    pbar = torch.zeros(4, dtype=torch.float32, device="cuda")
+    model.adjoint_step(prm, state, ob, ib, pbar)
+    torch.cuda.synchronize()
+    adj = {k: v.double().cpu().numpy() for k, v in ib.t.items()}
+    pb = pbar.double().cpu().numpy()
+
+    sim = _oracle_for(oracle_mod, model, geo, pos, theta, st)
+    V0 = st["volume0"][0]
+
+    def ell(stp, params=None):
+        _load(sim, stp, D)
+        if params is not None:
+            for name, val in params.items():
+                sim.set_material(0, name, val)
+        sim.step()
+        o = _outputs(sim, D)
+        return sum(float(np.sum(c[k] * o[k])) for k in c)
+
+    delta = {"x": rng.standard_normal((D, n)) * 1e-3, "v": rng.standard_normal((D, n)) * 1e-3,
+             "F": rng.standard_normal((D * D, n)) * 1e-3, "stress": rng.standard_normal((6, n)) * 10.0,
+             "eps_e": rng.standard_normal((6, n)) * 1e-6}
+    if D == 2:
+        for k in ("stress", "eps_e"):
+            delta[k][[2, 4, 5]] = 0.0
+    results = {}
+    for field in ["x", "v", "F", "stress", "eps_e"]:
+        eps = 1e-4
+        sp, sm = dict(st), dict(st)
+        sp[field] = st[field] + eps * delta[field]
+        sm[field] = st[field] - eps * delta[field]
+        if field == "F":  # V = det(F) V0
+            for s_ in (sp, sm):
+                Fm = s_["F"].T.reshape(n, D, D)
+                s_["volume"] = (np.linalg.det(Fm) * V0)[None, :]
+        fd = (ell(sp) - ell(sm)) / (2 * eps)
+        ad = float(np.sum(adj[field] * delta[field]))
+        results[field] = (fd, ad)
+    names = ["bulk_modulus", "shear_modulus", "M", "Pt"]
+    base = [sim_p for sim_p in [prm.materials[0].p[i] for i in range(4)]]
+    for i, name in enumerate(names):
+        h = 1e-6 * max(abs(base[i]), 1.0)
+        fd = (ell(st, {name: base[i] + h}) - ell(st, {name: base[i] - h})) / (2 * h)
+        results[name] = (fd, float(pb[i]))
+    print(f"[gpu-test] one-step adjoint (fd, adjoint): {results}", flush=True)
+    for k, (fd, ad) in results.items():
+        assert abs(fd - ad) <= 2e-3 * max(abs(fd), 1e-12) + 1e-9, (k, fd, ad)
+
+
+def test_multi_step_parameter_gradient_matches_finite_differences(oracle_mod):
+    from pyroclastmpm_amd.diff import DiffMPM, simulate
+
+    pos, geo = _column_2d(seed=5)
+    D = geo["dim"]
+    steps = 25
+    model = DiffMPM(D, geo["origin"], geo["end"], geo["cell_size"], pos, geo["ppc"], dt=1e-4, checkpoint_every=10)
+    theta0 = np.array([28.0, 20.0, 1e6])
+    _banner(f"multi-step gradient: 2D DP column, n={model.n}, {steps} steps, checkpoints every 10")
+    rng = np.random.default_rng(1)
+    cw = torch.from_numpy(rng.standard_normal((model.n, D))).float().cuda()
+    th = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    x = simulate(model, th, steps)
+    loss = (x * cw).sum()
+    loss.backward()
+    g_ad = th.grad.numpy()
+
+    def f_gpu(t):
+        with torch.no_grad():
+            return float((simulate(model, torch.tensor(t, dtype=torch.float64), steps) * cw).sum())
+
+    g_fd = np.zeros(3)
+    for i in range(3):
+        h = 1e-3 * abs(theta0[i]) if theta0[i] != 0 else 1e-3
+        tp, tm = theta0.copy(), theta0.copy()
+        tp[i] += h
+        tm[i] -= h
+        g_fd[i] = (f_gpu(tp) - f_gpu(tm)) / (2 * h)
+    # FP64 oracle: the same steps from the same initial state
+    from pyroclastmpm_amd.diff import dp_material
+
+    cw_np = cw.double().cpu().numpy()
+
+    def f_oracle(t):
+        _, _, M, Pt, _ = dp_material(*t, model.pois)
+        sim = getattr(oracle_mod, f"Sim{D}D_f64")(geo["origin"], geo["end"], geo["cell_size"], "cubic", model.dt,
+                                                 geo["ppc"])
+        sim.set_particles(pos)
+        sim.add_material("DP_rheo", {"density": 1500.0, "E": float(t[2]), "pois": model.pois, "M": M, "Pt": Pt,
+                                     "dilatancy_angle": 0.0})
+        sim.add_bc("Gravity", {"gravity": model.gravity})
+        sim.add_bc("NodeDomain", {"face0_mode": [1] * D, "face1_mode": [1] * D})
+        sim.musl = False
+        sim.alpha = model.alpha
+        sim.initialize()
+        for _ in range(steps):
+            sim.step()
+        return float(np.sum(np.asarray(sim.get("positions")) * cw_np))
+
+    g_64 = np.zeros(3)
+    for i in range(3):
+        h = 1e-6 * abs(theta0[i]) if theta0[i] != 0 else 1e-6
+        tp, tm = theta0.copy(), theta0.copy()
+        tp[i] += h
+        tm[i] -= h
+        g_64[i] = (f_oracle(tp) - f_oracle(tm)) / (2 * h)
+    print(f"[gpu-test] d<c,x>/d(phi,c,E) adjoint {g_ad} FD(GPU, FP32) {g_fd} FD(oracle, FP64) {g_64}", flush=True)
+    for i in range(3):
+        assert abs(g_ad[i] - g_fd[i]) <= 2e-2 * abs(g_fd[i]) + 1e-12, (i, g_ad, g_fd)
+        assert abs(g_ad[i] - g_64[i]) <= 1e-2 * abs(g_64[i]) + 1e-12, (i, g_ad, g_64)
