@@ -1,13 +1,15 @@
 """GPU checks of the adjoint of the USL step (pmpm_adjoint_step) and of the
 differentiable solver built on it (pyroclastmpm_amd.diff).
 
-  * one step, all state and parameter directions: <adjoint(c), delta> against
-    the central finite difference of <c, step(S + eps delta)> computed by the
-    FP64 oracle (relative error <= 2e-3; the adjoint runs in FP32);
+  * one step from a yielding state, all state and parameter directions:
+    <adjoint(c), delta> against the central finite difference of
+    <c, step(S + eps delta)> computed by the FP64 oracle (relative error
+    <= 2e-3; the adjoint runs in FP32);
   * several steps with checkpoint segments: d<c, x_final>/d(phi, c, E) against
-    central finite differences of the GPU forward (relative step 1e-3; FP32,
-    relative error <= 2e-2) and of the FP64 oracle (relative step 1e-6,
-    relative error <= 1e-2).
+    central finite differences of the FP64 oracle (relative step 1e-6,
+    relative error <= 1e-3) and, as a coarse check, of the GPU forward itself
+    (relative step 1e-3; the FP32 forward's roundoff makes that difference
+    quotient good to a few percent only, so the bar is 10%).
 """
 import sys
 
@@ -99,8 +101,9 @@ def test_one_step_adjoint_matches_fp64_oracle(oracle_mod):
 
     pos, geo = _column_2d()
     D = geo["dim"]
-    model = DiffMPM(D, geo["origin"], geo["end"], geo["cell_size"], pos, geo["ppc"], dt=1e-4)
-    theta = (30.0, 0.0, 1e6)
+    # the soft column of the multi-step test after 30 steps: yielding
+    model = DiffMPM(D, geo["origin"], geo["end"], geo["cell_size"], pos, geo["ppc"], dt=1e-3)
+    theta = (28.0, 200.0, 1e5)
     _banner(f"one-step adjoint: 2D DP column, n={model.n}, 30 warm-up steps")
     state, _ = model.forward(theta, 30)
     torch.cuda.synchronize()
@@ -169,7 +172,8 @@ def test_one_step_adjoint_matches_fp64_oracle(oracle_mod):
         results[name] = (fd, float(pb[i]))
     print(f"[gpu-test] one-step adjoint (fd, adjoint): {results}", flush=True)
     for k, (fd, ad) in results.items():
-        assert abs(fd - ad) <= 2e-3 * max(abs(fd), 1e-12) + 1e-9, (k, fd, ad)
+        assert abs(fd - ad) <= 2e-3 * max(abs(fd), 1e-12) + 1e-12, (k, fd, ad)
+    assert results["M"][0] != 0.0 and results["Pt"][0] != 0.0, "the state must yield"
 
 
 def test_multi_step_parameter_gradient_matches_finite_differences(oracle_mod):
@@ -177,10 +181,13 @@ def test_multi_step_parameter_gradient_matches_finite_differences(oracle_mod):
 
     pos, geo = _column_2d(seed=5)
     D = geo["dim"]
-    steps = 25
-    model = DiffMPM(D, geo["origin"], geo["end"], geo["cell_size"], pos, geo["ppc"], dt=1e-4, checkpoint_every=10)
-    theta0 = np.array([28.0, 20.0, 1e6])
-    _banner(f"multi-step gradient: 2D DP column, n={model.n}, {steps} steps, checkpoints every 10")
+    # a soft column (E = 1e5, elastic strain rho g H / E ~ 4%) over 60 ms, so
+    # it yields and the friction and cohesion gradients are not zero
+    # (elastic wave speed 8 m/s: dt 1e-3 < h / c = 6e-3)
+    steps = 60
+    model = DiffMPM(D, geo["origin"], geo["end"], geo["cell_size"], pos, geo["ppc"], dt=1e-3, checkpoint_every=25)
+    theta0 = np.array([28.0, 200.0, 1e5])
+    _banner(f"multi-step gradient: 2D DP column, n={model.n}, {steps} steps, checkpoints every 25")
     rng = np.random.default_rng(1)
     cw = torch.from_numpy(rng.standard_normal((model.n, D))).float().cuda()
     th = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
@@ -230,5 +237,6 @@ def test_multi_step_parameter_gradient_matches_finite_differences(oracle_mod):
         g_64[i] = (f_oracle(tp) - f_oracle(tm)) / (2 * h)
     print(f"[gpu-test] d<c,x>/d(phi,c,E) adjoint {g_ad} FD(GPU, FP32) {g_fd} FD(oracle, FP64) {g_64}", flush=True)
     for i in range(3):
-        assert abs(g_ad[i] - g_fd[i]) <= 2e-2 * abs(g_fd[i]) + 1e-12, (i, g_ad, g_fd)
-        assert abs(g_ad[i] - g_64[i]) <= 1e-2 * abs(g_64[i]) + 1e-12, (i, g_ad, g_64)
+        assert abs(g_ad[i] - g_64[i]) <= 1e-3 * abs(g_64[i]) + 1e-15, (i, g_ad, g_64)
+        assert abs(g_ad[i] - g_fd[i]) <= 1e-1 * abs(g_fd[i]) + 1e-15, (i, g_ad, g_fd)
+    assert all(abs(v) > 0 for v in g_64), g_64  # the column yields: every parameter matters
