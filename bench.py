@@ -36,10 +36,19 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_PARTICLE = {
     "partition": 76,     # hash: x 12 + key 4; 3 radix passes x (key 4 + key/val 8 in + 8 out); ranges 4
     "p2g": 64,           # sorted 4, x 12, v 12, m 4, V 4, stress 24, + node flush
-    "musl_map": 64,      # sorted 4, x 12, v 12, m 4 in; x 12, v 12 out (+ node flush)
-    "musl_g2p": 248,     # in: sorted 4 x 12 F 36 V0 4 stress 24 eps 24 m 4 colour 1 flags 1 id 4
+    "musl_map": 84,      # first map: sorted 4, x 12, v 12 in; x 12, v 12 out;
+                         # re-map: sorted 4, x 12, m 4, v' 12 in
+    "musl_g2p": 252,     # in: sorted 4 x 12 F 36 V0 4 stress 24 eps 24 m 4 colour 1 flags 1 id 4
                          # out: F 36 V 4 stress 24 eps 24 L 36 m 4 V0 4 colour 1 flags 1 id 4
 }
+# The roofline kernel: pmpm_musl_g2p launches exactly one kernel (the second
+# map with the return map), the one that moves the most bytes, so its HIP
+# event time is that kernel's duration.
+ROOFLINE_ENTRY = "musl_g2p"
+ROOFLINE_KERNEL = "g2p_kernel<3, 2, 2>"
+# HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE)
+# and scaled by the same counters on a plain device copy of known size.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "g2p_return_map_dp_column_1080k_traffic.json")
 
 
 def parse():
@@ -278,14 +287,17 @@ def main():
     value = n * args.steps / (total_ms / 1e3)
     print("bench: per-entry-point mean ms: " + json.dumps({k: round(v, 4) for k, v in per_call.items()}),
           file=sys.stderr, flush=True)
-    # dominant entry point (each is one main kernel plus, for P2G and the
-    # MUSL map, the node-gather grid kernel)
-    single = {k: v for k, v in per_call.items() if k in BYTES_PER_PARTICLE}
-    dom = max(single, key=single.get)
-    achieved = BYTES_PER_PARTICLE[dom] * n / (single[dom] * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "bytes_per_particle": BYTES_PER_PARTICLE[dom], "kernel_ms": round(single[dom], 4)}
+    dom = ROOFLINE_ENTRY
+    achieved = BYTES_PER_PARTICLE[dom] * n / (per_call[dom] * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f).get(ROOFLINE_KERNEL)
+        if t and t.get("particles") == n:
+            traffic = t["bytes_per_launch"]
+    roof = {"bound": "hbm", "kernel": ROOFLINE_KERNEL, "entry_point": dom, "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "bytes_per_particle": BYTES_PER_PARTICLE[dom], "kernel_ms": round(per_call[dom], 4)}
     cpu = None
     if not args.no_cpu_baseline:
         try:

@@ -939,72 +939,27 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
 //         written to slot j of `out` (cell order); nothing else.
 // MODE 2: MUSL second map: L, F, V from the re-mapped velocities + return map
 //         (x, v of slot j of `out` were written by the first map).
-// The 256 particles of a G2P workgroup are consecutive in cell order, so
-// their windows cover a small box of nodes: the box's node velocities are
-// staged in LDS once (when it holds at most kG2PStage nodes) and every window
-// read hits LDS instead of L2.
+// The window reads go through L2: the node velocities (12-24 B per node)
+// of a 1M-particle grid fit in it, and staging a block's node box in LDS
+// measured slower on gfx950 (fewer resident waves for the same reads).
 constexpr int kG2PThreads = 256;
-constexpr int kG2PStage = 1536;
 
 template <int D, int SF, int MODE>
 __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles in,
                                                           pmpm_particles out, pmpm_spatial part, pmpm_nodes nodes,
                                                           int reorder) {
-  constexpr int E = Spline<SF>::ENTRIES;
-  constexpr int NF = (MODE == 2 ? 1 : 2) * D;  // staged floats per node: vel_nt (+ dvel)
-  __shared__ float stage[NF * kG2PStage];
-  __shared__ int box[6];
   const int n = in.n;
-  const int t = threadIdx.x;
-  const int j = blockIdx.x * kG2PThreads + t;
-  const bool valid = j < n;
+  const int j = blockIdx.x * kG2PThreads + threadIdx.x;
+  if (j >= n) return;
   const int nn = g.num_cells_total;
-  if (t < 3) {
-    box[t] = 0x7fffffff;
-    box[3 + t] = -0x7fffffff;
-  }
-  const int src = valid ? part.sorted[j] : 0;
+  const int src = part.sorted[j];
   const int dst = (reorder || MODE == 1) ? j : src;
   PW<D, SF> w;
   float xp[3];
   particle_weights<D, SF>(prm, g, in.x, n, src, w, xp);
-  __syncthreads();
-  if (valid) {
-    // the clamped window (gather_grid reads clamped node indices)
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      const int top = g.num_cells[a] - 1;
-      atomicMin(&box[a], min(max(w.bin[a] + w.first[a], 0), top));
-      atomicMax(&box[3 + a], min(max(w.bin[a] + w.first[a] + E - 1, 0), top));
-    }
-  }
-  __syncthreads();
-  int lo[3] = {0, 0, 0}, ext[3] = {1, 1, 1};
-#pragma unroll
-  for (int a = 0; a < D; ++a) {
-    lo[a] = box[a];
-    ext[a] = max(box[3 + a] - box[a] + 1, 1);
-  }
-  const int64_t count = (int64_t)ext[0] * ext[1] * ext[2];
-  NodeSrc ns;
-  if (count <= kG2PStage) {  // uniform over the workgroup
-    for (int i = t; i < (int)count; i += kG2PThreads) {
-      const int ix = i % ext[0], iy = (i / ext[0]) % ext[1], iz = i / (ext[0] * ext[1]);
-      const int node = (lo[0] + ix) + (lo[1] + iy) * g.num_cells[0] + (lo[2] + iz) * g.num_cells[0] * g.num_cells[1];
-#pragma unroll
-      for (int k = 0; k < D; ++k) {
-        stage[k * kG2PStage + i] = nodes.vel_nt[k * nn + node];
-        if (MODE != 2) stage[(D + k) * kG2PStage + i] = nodes.dvel[k * nn + node];
-      }
-    }
-    ns = NodeSrc{stage, stage + D * kG2PStage, kG2PStage, {lo[0], lo[1], lo[2]}, {1, ext[0], ext[0] * ext[1]}};
-  } else {
-    ns = NodeSrc{nodes.vel_nt, nodes.dvel, nn, {0, 0, 0}, {1, g.num_cells[0], g.num_cells[0] * g.num_cells[1]}};
-  }
-  __syncthreads();
-  if (!valid) return;
   float vinc[3] = {0.f, 0.f, 0.f}, dvinc[3] = {0.f, 0.f, 0.f};
   float L[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const NodeSrc ns{nodes.vel_nt, nodes.dvel, nn, {0, 0, 0}, {1, g.num_cells[0], g.num_cells[0] * g.num_cells[1]}};
   if (MODE == 0) gather_grid<D, SF, true, true>(g, ns, w, vinc, dvinc, L);
   else if (MODE == 1) gather_grid<D, SF, true, false>(g, ns, w, vinc, dvinc, L);
   else gather_grid<D, SF, false, true>(g, ns, w, vinc, dvinc, L);
