@@ -63,5 +63,15 @@ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // Fill n floats/ints with zero (a kernel, not hipMemsetAsync, so the step can
 // be captured in a HIP graph).
 int zero_fill(void *ptr, int64_t n_words, hipStream_t s);
+constexpr int kZeroLists = 4;
+struct ZeroLists {  // buffers of 32-bit words to clear in one launch
+  uint32_t *p[kZeroLists];
+  int64_t n[kZeroLists];
+  int count;
+  void add(void *ptr, int64_t words) {
+    if (ptr && words > 0 && count < kZeroLists) { p[count] = (uint32_t *)ptr; n[count] = words; ++count; }
+  }
+};
+int zero_fill_lists(const ZeroLists &z, hipStream_t s);
 
 }  // namespace pmpm

@@ -24,6 +24,26 @@ int zero_fill(void *ptr, int64_t n_words, hipStream_t s) {
   return PMPM_OK;
 }
 
+// Up to kZeroLists buffers cleared by one launch (a fill kernel, so the
+// sequence stays valid inside a captured graph).
+__global__ void zero_lists_kernel(ZeroLists z) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  for (int k = 0; k < z.count; ++k)
+    for (int64_t i = t; i < z.n[k]; i += stride) z.p[k][i] = 0u;
+}
+
+int zero_fill_lists(const ZeroLists &z, hipStream_t s) {
+  int64_t most = 0;
+  for (int k = 0; k < z.count; ++k) most = z.n[k] > most ? z.n[k] : most;
+  if (most <= 0) return PMPM_OK;
+  int64_t blocks = ceil_div(most, 256);
+  if (blocks > 16384) blocks = 16384;
+  hipLaunchKernelGGL(zero_lists_kernel, dim3((unsigned)blocks), dim3(256), 0, s, z);
+  PMPM_LAUNCH_CHECK();
+  return PMPM_OK;
+}
+
 // bins = trunc((x - origin) * inv_h) per axis (int cast truncates toward 0, as
 // Eigen's cast<int>), hash = x + y*nx + z*nx*ny. Hashes outside
 // [0, num_cells_total) are counted and clamped to num_cells_total so they sort
@@ -274,9 +294,11 @@ extern "C" int pmpm_partition(pmpm_context *ctx, int32_t dim, const pmpm_grid *g
   const pmpm_grid g = *grid;
   const uint32_t ncells = (uint32_t)g.num_cells_total;
   int rc;
-  if ((rc = zero_fill(ctx->counters, 1, s)) != PMPM_OK) return rc;
-  if ((rc = zero_fill(part->cell_start, ncells, s)) != PMPM_OK) return rc;
-  if ((rc = zero_fill(part->cell_end, ncells, s)) != PMPM_OK) return rc;
+  ZeroLists z{};
+  z.add(ctx->counters, 1);
+  z.add(part->cell_start, ncells);
+  z.add(part->cell_end, ncells);
+  if ((rc = zero_fill_lists(z, s)) != PMPM_OK) return rc;
   if (n == 0) return PMPM_OK;
   const unsigned blocks = (unsigned)ceil_div(n, 256);
   switch (dim) {

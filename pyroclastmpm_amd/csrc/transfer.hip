@@ -38,6 +38,10 @@ __host__ __device__ constexpr int p2g_pitch(int nx, int cx, int sy) {
   return sy <= 1 ? nx : (((nx + cx - 1) / cx) | 1) * cx;
 }
 
+// Tiles whose node block covers one node, at most, per axis: x and z tiles
+// are wider than the span overlap (T >= SPAN - 1), y tiles are one cell.
+constexpr int kCoverX = 2, kCoverY = 4, kCoverZ = 2;
+
 template <int D, int SF> struct P2GGeom {
   static constexpr int SPAN = Spline<SF>::SPAN;  // node offsets a cell's particles reach, per axis
   static constexpr int NLO = Spline<SF>::NLO;    // the first of them, relative to the cell
@@ -58,6 +62,8 @@ template <int D, int SF> struct P2GGeom {
   static constexpr int NT = NXP * NY * NZ;       // LDS nodes incl. padding
   static constexpr int NTN = NX * NY * NZ;       // nodes of one partial slice
   static_assert(CX * SY == 64 && SZH * ZH == SZ, "one wave = CX cells x SY lanes");
+  static_assert((NX - 1) / TX + 1 <= kCoverX && (NY - 1) / TY + 1 <= kCoverY && (NZ - 1) / TZ + 1 <= kCoverZ,
+                "the grid kernel's gather visits at most kCover tiles per axis");
 };
 
 __device__ inline int64_t lin_index(const pmpm_grid &g, const int *b, int D) {
@@ -664,21 +670,26 @@ template <int D> __host__ inline int wrap_cell_count(const pmpm_grid &g) {
 // lies outside the grid but whose linear index is in range re-sums the cell's
 // particles with the types of the node that index reaches
 // (shapefunction.cpp:96-112) and adds the sums there with global atomics.
+// The cell's particles are first loaded into LDS by the whole wave (lane =
+// particle, up to 64 per round), so the per-node loops read LDS instead of
+// walking a chain of dependent global loads.
 template <int D, int MODE>
 __global__ __launch_bounds__(64) void p2g_wrap_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles in,
                                                       pmpm_particles out, pmpm_spatial part, pmpm_nodes nodes) {
   constexpr int SPAN = 4, NLO = -1;
   constexpr int NV = MODE == 0 ? 1 + 2 * D : D;
+  constexpr int NS = MODE == 0 ? 6 : 1;  // staged stress terms (-V sigma)
+  __shared__ float s_frac[3][64], s_v[3][64], s_m[64], s_A[NS][64];
+  __shared__ int s_own[64];
   int c[3] = {0, 0, 0};
-  if (!wrap_cell<D>(g, blockIdx.x, c)) return;
+  if (!wrap_cell<D>(g, blockIdx.x, c)) return;  // uniform over the block
   const int cell = (int)lin_index(g, c, D);
   const int ce = part.cell_end[cell];
   const int cs = part.cell_start[cell];
-  if (ce <= cs) return;
+  if (ce <= cs) return;  // uniform
   const int lane = threadIdx.x;
   const int s[3] = {lane % SPAN, D > 1 ? (lane / SPAN) % SPAN : 0, D > 2 ? lane / (SPAN * SPAN) : 0};
   const int nlanes = D == 3 ? 64 : (D == 2 ? 16 : 4);
-  if (lane >= nlanes) return;
   int nb[3] = {0, 0, 0};
   bool inside = true;
 #pragma unroll
@@ -688,50 +699,88 @@ __global__ __launch_bounds__(64) void p2g_wrap_kernel(pmpm_params prm, pmpm_grid
   }
   const int nn = g.num_cells_total;
   const int64_t lin = lin_index(g, nb, D);
-  if (inside || lin < 0 || lin >= nn) return;
-  int ids[3], type[3] = {0, 0, 0};
-  node_ids<D>(g, lin, ids);
+  const bool active = lane < nlanes && !inside && lin >= 0 && lin < nn;
+  if (!__syncthreads_or(active)) return;  // uniform
+  // r = frac - (NLO + s) lies in piece 3 - s of the wrapped node's spline
+  // (the same piece argument as span_eval): its coefficients are fixed per
+  // lane
+  float cf[3][4];
 #pragma unroll
-  for (int a = 0; a < D; ++a) type[a] = cubic_node_type(ids[a], g.num_cells[a], prm.node_types_built);
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cf[a][k] = 0.f;
+  if (active) {
+    int ids[3];
+    node_ids<D>(g, lin, ids);
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      const int type = cubic_node_type(ids[a], g.num_cells[a], prm.node_types_built);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) cf[a][k] = kCubic[type][3 - s[a]][k];
+    }
+  }
 
   const int n = in.n;
   const float inv_h = g.inv_cell_size;
   float acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.f;
-  for (int j = cs; j < ce; ++j) {
-    const int slot = part.sorted[j];
-    float frac[3] = {0.f, 0.f, 0.f};
-    bool own = true;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      const float ta = cell_coord(in.x[a * n + slot], g.origin[a], inv_h);
-      const int ba = (int)ta;
-      own = own && ba == c[a];
-      frac[a] = __fsub_rn(ta, (float)ba);
-    }
-    if (!own) continue;  // deposited whole by p2g_stray
-    P2GRec r;
-    p2g_load<D, MODE>(in, out, nullptr, n, slot, j, r);
-    float N[3] = {1.f, 1.f, 1.f}, dN[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-    for (int a = 0; a < D; ++a) cubic_eval(__fsub_rn(frac[a], (float)(NLO + s[a])), type[a], inv_h, N[a], dN[a]);
-    const float wm = ((N[0] * N[1]) * N[2]) * r.m;
-    const float gr[3] = {(dN[0] * N[1]) * N[2], (dN[1] * N[0]) * N[2], (dN[2] * N[0]) * N[1]};
-    if (MODE == 0) {
-      acc[0] += wm;
+  for (int base = cs; base < ce; base += 64) {
+    const int cnt = min(64, ce - base);
+    if (lane < cnt) {
+      const int j = base + lane;
+      const int slot = part.sorted[j];
+      bool own = true;
 #pragma unroll
       for (int a = 0; a < D; ++a) {
-        acc[1 + a] += wm * r.v[a];
-#pragma unroll
-        for (int b = 0; b < D; ++b) acc[1 + D + a] += sym_at(r.A, a, b) * gr[b];
+        const float ta = cell_coord(in.x[a * n + slot], g.origin[a], inv_h);
+        const int ba = (int)ta;
+        own = own && ba == c[a];
+        s_frac[a][lane] = __fsub_rn(ta, (float)ba);
       }
-    } else {
+      s_own[lane] = own;
+      P2GRec r;
+      p2g_load<D, MODE>(in, out, nullptr, n, slot, j, r);
+      s_m[lane] = r.m;
 #pragma unroll
-      for (int a = 0; a < D; ++a) acc[a] += wm * r.v[a];
+      for (int a = 0; a < D; ++a) s_v[a][lane] = r.v[a];
+      if constexpr (MODE == 0) {
+        s_A[0][lane] = r.A.xx; s_A[1][lane] = r.A.yy; s_A[2][lane] = r.A.zz;
+        s_A[3][lane] = r.A.xy; s_A[4][lane] = r.A.xz; s_A[5][lane] = r.A.yz;
+      }
     }
+    __syncthreads();
+    if (active) {
+      for (int k = 0; k < cnt; ++k) {
+        if (!s_own[k]) continue;  // deposited whole by p2g_stray
+        float N[3] = {1.f, 1.f, 1.f}, dN[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          const float r = __fsub_rn(s_frac[a][k], (float)(NLO + s[a]));
+          N[a] = ((cf[a][3] * r + cf[a][2]) * r + cf[a][1]) * r + cf[a][0];
+          dN[a] = inv_h * ((3.f * cf[a][3] * r + 2.f * cf[a][2]) * r + cf[a][1]);
+          if (s[a] == 0 && !(r < 2.f)) N[a] = dN[a] = 0.f;  // frac = 1 - 2^-24
+        }
+        const float wm = ((N[0] * N[1]) * N[2]) * s_m[k];
+        const float gr[3] = {(dN[0] * N[1]) * N[2], (dN[1] * N[0]) * N[2], (dN[2] * N[0]) * N[1]};
+        if constexpr (MODE == 0) {
+          const Sym A{s_A[0][k], s_A[1][k], s_A[2][k], s_A[3][k], s_A[4][k], s_A[5][k]};
+          acc[0] += wm;
+#pragma unroll
+          for (int a = 0; a < D; ++a) {
+            acc[1 + a] += wm * s_v[a][k];
+#pragma unroll
+            for (int b = 0; b < D; ++b) acc[1 + D + a] += sym_at(A, a, b) * gr[b];
+          }
+        } else {
+#pragma unroll
+          for (int a = 0; a < D; ++a) acc[a] += wm * s_v[a][k];
+        }
+      }
+    }
+    __syncthreads();
   }
-  deposit_wrapped<D, MODE>(g, nodes, nb, acc);
+  if (active) deposit_wrapped<D, MODE>(g, nodes, nb, acc);
 }
 
 // Geometry of the P2G tiles for the grid kernel's gather.
@@ -744,11 +793,16 @@ struct TileGeom {
   int gather;    // 0: none, 1: P2G (mass, momentum, force), 2: MUSL re-map (momentum)
 };
 
-// Sums the partial slices of the (at most 2 per axis) tiles whose node block
-// covers node `ids`, in a fixed tile order.
-template <int D>
+// Sums the NV fields of the partial slices of the tiles whose node block
+// covers node `ids`, in a fixed tile order (z, y, x ascending). The live
+// flags of all candidate tiles are loaded first; then every candidate's loads
+// are issued without a branch (a dead or out-of-range candidate reads tile
+// 0's first entries and adds zero), so a node waits for two loads in a row,
+// not one per tile.
+template <int D, int NV>
 __device__ inline void gather_partials(const TileGeom &tg, const float *partials, const int32_t *flags,
                                        const int *ids, float *acc) {
+  constexpr int MX = kCoverX, MY = D > 1 ? kCoverY : 1, MZ = D > 2 ? kCoverZ : 1, MC = MX * MY * MZ;
   int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
 #pragma unroll
   for (int a = 0; a < D; ++a) {
@@ -761,17 +815,33 @@ __device__ inline void gather_partials(const TileGeom &tg, const float *partials
     hi[a] = b_hi;
   }
   const int ntn = tg.N[0] * (D > 1 ? tg.N[1] : 1) * (D > 2 ? tg.N[2] : 1);
-  for (int bz = lo[2]; bz <= hi[2]; ++bz)
-    for (int by = lo[1]; by <= hi[1]; ++by)
-      for (int bx = lo[0]; bx <= hi[0]; ++bx) {
-        const int tile = bx + tg.tiles[0] * (by + tg.tiles[1] * bz);
-        if (!flags[tile]) continue;
+  bool live[MC];
+  int64_t off[MC];
+#pragma unroll
+  for (int cz = 0; cz < MZ; ++cz)
+#pragma unroll
+    for (int cy = 0; cy < MY; ++cy)
+#pragma unroll
+      for (int cx = 0; cx < MX; ++cx) {
+        const int c = (cz * MY + cy) * MX + cx;
+        const int bx = lo[0] + cx, by = lo[1] + cy, bz = lo[2] + cz;
+        const bool in = bx <= hi[0] && by <= hi[1] && bz <= hi[2];
+        const int tile = in ? bx + tg.tiles[0] * (by + tg.tiles[1] * bz) : 0;
+        live[c] = in && flags[tile] != 0;
         int li = ids[0] - (bx * tg.T[0] + tg.nlo);
         if (D > 1) li += tg.N[0] * (ids[1] - (by * tg.T[1] + tg.nlo));
         if (D > 2) li += tg.N[0] * tg.N[1] * (ids[2] - (bz * tg.T[2] + tg.nlo));
-        const float *s = partials + (int64_t)tile * tg.nv * ntn + li;
-        for (int v = 0; v < tg.nv; ++v) acc[v] += s[(int64_t)v * ntn];
+        off[c] = (int64_t)tile * NV * ntn + li;
       }
+#pragma unroll
+  for (int c = 0; c < MC; ++c) {
+    const float *s = partials + (live[c] ? off[c] : 0);
+    float v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = s[(int64_t)k * ntn];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] += live[c] ? v[k] : 0.f;
+  }
 }
 
 // Per node: cutoff + forward moment (finalize), boundary conditions
@@ -792,7 +862,8 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
     int ids0[3];
     node_ids<D>(g, i, ids0);
     float acc[1 + 2 * 3] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    gather_partials<D>(tg, partials, flags, ids0, acc);
+    if (tg.gather == 1) gather_partials<D, 1 + 2 * D>(tg, partials, flags, ids0, acc);
+    else gather_partials<D, D>(tg, partials, flags, ids0, acc);
     if (tg.gather == 1) {
       m += acc[0];
 #pragma unroll
@@ -1214,9 +1285,11 @@ static bool particles_ok(const pmpm_particles *p) {
 int do_zero_nodes(const pmpm_params &prm, const pmpm_grid &g, const pmpm_nodes &nodes, hipStream_t s) {
   int rc;
   const int64_t nn = g.num_cells_total;
-  if ((rc = zero_fill(nodes.mass, nn, s)) != PMPM_OK) return rc;
-  if ((rc = zero_fill(nodes.mom, nn * prm.dim, s)) != PMPM_OK) return rc;
-  if ((rc = zero_fill(nodes.mom_nt, nn * prm.dim, s)) != PMPM_OK) return rc;
+  ZeroLists z{};
+  z.add(nodes.mass, nn);
+  z.add(nodes.mom, nn * prm.dim);
+  z.add(nodes.mom_nt, nn * prm.dim);
+  if ((rc = zero_fill_lists(z, s)) != PMPM_OK) return rc;
   return PMPM_OK;
 }
 
@@ -1610,7 +1683,7 @@ __global__ void grid_adjoint_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nod
   int ids[3];
   node_ids<D>(g, i, ids);
   float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  gather_partials<D>(tg, partials, flags, ids, acc);
+  gather_partials<D, 2 * D>(tg, partials, flags, ids, acc);
   const float m = nodes.mass[i];
   float mb = 0.f, Pb[3] = {0.f, 0.f, 0.f}, Qb[3] = {0.f, 0.f, 0.f}, fb[3] = {0.f, 0.f, 0.f};
   if (m > prm.small_mass_cutoff) {
