@@ -13,7 +13,7 @@ struct pmpm_context {
   int32_t cap_cells = 0;
   uint32_t *keys_tmp = nullptr;  // radix sort ping-pong keys
   int32_t *vals_tmp = nullptr;   // radix sort ping-pong values
-  uint32_t *hist = nullptr;      // [RADIX][blocks] digit histograms
+  uint32_t *hist = nullptr;      // [radix][blocks] digit histograms
   size_t hist_len = 0;
   uint32_t *counters = nullptr;  // [4]: out-of-grid particles, ...
   // P2G tile partials: per tile, the LDS node block of each accumulated field
@@ -43,8 +43,8 @@ namespace pmpm {
 constexpr int kSortBlock = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortBlock * kSortItems;
-constexpr int kRadixBits = 8;
-constexpr int kRadix = 1 << kRadixBits;
+constexpr int kMaxRadixBits = 10;  // digits of 8 bits (keys < 2^8) or 10 bits
+constexpr int kMaxRadix = 1 << kMaxRadixBits;
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 

@@ -76,21 +76,24 @@ __global__ void hash_kernel(const float *__restrict__ x, int n, pmpm_grid g, uin
   }
 }
 
-// Per-block digit histogram, digit-major layout hist[digit * nblocks + block].
+// Per-block digit histogram (digits of RB bits), digit-major layout
+// hist[digit * nblocks + block].
+template <int RB>
 __global__ __launch_bounds__(kSortBlock) void radix_hist_kernel(const uint32_t *__restrict__ keys, int n, int shift,
                                                                 uint32_t *__restrict__ hist, int nblocks) {
-  __shared__ uint32_t h[kRadix];
+  constexpr int R = 1 << RB;
+  __shared__ uint32_t h[R];
   const int t = threadIdx.x;
-  h[t] = 0u;
+  for (int d = t; d < R; d += kSortBlock) h[d] = 0u;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * kSortTile;
 #pragma unroll 4
   for (int r = 0; r < kSortItems; ++r) {
     const int64_t i = base + (int64_t)r * kSortBlock + t;
-    if (i < n) atomicAdd(&h[(keys[i] >> shift) & (kRadix - 1)], 1u);
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & (R - 1)], 1u);
   }
   __syncthreads();
-  hist[(int64_t)t * nblocks + blockIdx.x] = h[t];
+  for (int d = t; d < R; d += kSortBlock) hist[(int64_t)d * nblocks + blockIdx.x] = h[d];
 }
 
 // Exclusive scan of each digit's column hist[d][0..nblocks) (one workgroup
@@ -122,43 +125,63 @@ __global__ __launch_bounds__(256) void scan_columns_kernel(uint32_t *__restrict_
   if (t == 255) totals[blockIdx.x] = part[255];
 }
 
-// Stable scatter of one radix pass. Within a round of 256 keys (thread order
-// == input order) a lane's rank among same-digit lanes of its wave comes from
-// 8 ballots; per-wave counts are combined per digit in LDS.
+// Stable scatter of one radix pass (digits of RB bits). Within a round of
+// 256 keys (thread order == input order) a lane's rank among same-digit lanes
+// of its wave comes from RB ballots; per-wave counts are combined per digit
+// in LDS.
+template <int RB>
 __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
     const uint32_t *__restrict__ keys_in, const int32_t *__restrict__ vals_in, uint32_t *__restrict__ keys_out,
     int32_t *__restrict__ vals_out, int n, int shift, const uint32_t *__restrict__ hist, int nblocks,
     const uint32_t *__restrict__ totals) {
-  __shared__ uint32_t base[kRadix];
-  __shared__ uint32_t wcnt[kSortBlock / 64][kRadix];
+  constexpr int R = 1 << RB, DPT = R / kSortBlock, W = kSortBlock / 64;
+  static_assert(R % kSortBlock == 0, "whole digits per thread");
+  __shared__ uint32_t base[R];
+  __shared__ uint32_t wcnt[W][R];
+  __shared__ uint32_t part[kSortBlock];
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int w = t >> 6;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  // digit bases: exclusive scan of the 256 column totals
-  base[t] = totals[t];
+  // digit bases: exclusive scan of the R column totals (thread t holds
+  // digits t*DPT .. t*DPT + DPT - 1)
+  uint32_t loc[DPT], sum = 0u;
+#pragma unroll
+  for (int k = 0; k < DPT; ++k) {
+    loc[k] = totals[t * DPT + k];
+    sum += loc[k];
+  }
+  part[t] = sum;
   __syncthreads();
-  for (int off = 1; off < kRadix; off <<= 1) {
-    const uint32_t v = (t >= off) ? base[t - off] : 0u;
+  for (int off = 1; off < kSortBlock; off <<= 1) {
+    const uint32_t v = (t >= off) ? part[t - off] : 0u;
     __syncthreads();
-    base[t] += v;
+    part[t] += v;
     __syncthreads();
   }
-  const uint32_t digit_base = (t > 0) ? base[t - 1] : 0u;
-  __syncthreads();
-  base[t] = digit_base + hist[(int64_t)t * nblocks + blockIdx.x];
+  {
+    uint32_t run = (t > 0) ? part[t - 1] : 0u;
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+      const int d = t * DPT + k;
+      base[d] = run + hist[(int64_t)d * nblocks + blockIdx.x];
+      run += loc[k];
+    }
+  }
   const int64_t tile = (int64_t)blockIdx.x * kSortTile;
   for (int r = 0; r < kSortItems; ++r) {
 #pragma unroll
-    for (int q = 0; q < kSortBlock / 64; ++q) wcnt[q][t] = 0u;
+    for (int k = 0; k < DPT; ++k)
+#pragma unroll
+      for (int q = 0; q < W; ++q) wcnt[q][k * kSortBlock + t] = 0u;
     __syncthreads();
     const int64_t i = tile + (int64_t)r * kSortBlock + t;
     const bool valid = i < n;
     const uint32_t key = valid ? keys_in[i] : 0u;
-    const uint32_t d = (key >> shift) & (kRadix - 1);
+    const uint32_t d = (key >> shift) & (R - 1);
     uint64_t peers = __ballot(valid);
 #pragma unroll
-    for (int bit = 0; bit < kRadixBits; ++bit) {
+    for (int bit = 0; bit < RB; ++bit) {
       const bool on = (d >> bit) & 1u;
       const uint64_t m = __ballot(on);
       peers &= on ? m : ~m;
@@ -166,15 +189,17 @@ __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
     const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
     if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
     __syncthreads();
-    {
-      uint32_t run = base[t];
 #pragma unroll
-      for (int q = 0; q < kSortBlock / 64; ++q) {
-        const uint32_t c = wcnt[q][t];
-        wcnt[q][t] = run;
+    for (int k = 0; k < DPT; ++k) {
+      const int dd = k * kSortBlock + t;
+      uint32_t run = base[dd];
+#pragma unroll
+      for (int q = 0; q < W; ++q) {
+        const uint32_t c = wcnt[q][dd];
+        wcnt[q][dd] = run;
         run += c;
       }
-      base[t] = run;
+      base[dd] = run;
     }
     __syncthreads();
     if (valid) {
@@ -265,7 +290,7 @@ extern "C" int pmpm_context_reserve(pmpm_context *ctx, int32_t max_particles, in
     ctx->keys_tmp = nullptr; ctx->vals_tmp = nullptr; ctx->hist = nullptr; ctx->cap_particles = 0;
     const size_t np = (size_t)(max_particles > 0 ? max_particles : 1);
     const size_t nblocks = (size_t)ceil_div((int64_t)np, kSortTile);
-    ctx->hist_len = (size_t)kRadix * nblocks + kRadix;  // + digit totals
+    ctx->hist_len = (size_t)kMaxRadix * nblocks + kMaxRadix;  // + digit totals
     PMPM_HIP_CHECK(hipMalloc(&ctx->keys_tmp, np * sizeof(uint32_t)));
     PMPM_HIP_CHECK(hipMalloc(&ctx->vals_tmp, np * sizeof(int32_t)));
     PMPM_HIP_CHECK(hipMalloc(&ctx->hist, ctx->hist_len * sizeof(uint32_t)));
@@ -310,7 +335,10 @@ extern "C" int pmpm_partition(pmpm_context *ctx, int32_t dim, const pmpm_grid *g
   // keys are in [0, ncells] (ncells = out-of-grid bucket)
   int bits = 1;
   while (bits < 32 && ((uint64_t)1 << bits) <= (uint64_t)ncells) ++bits;
-  const int passes = (bits + kRadixBits - 1) / kRadixBits;
+  // 10-bit digits: 2 passes up to 2^20 cells (8-bit digits when one pass
+  // of them covers the keys)
+  const int rb = bits <= 8 ? 8 : 10;
+  const int passes = (bits + rb - 1) / rb;
   const int nblocks = (int)ceil_div(n, kSortTile);
   const uint32_t *kin = part->hash;
   const int32_t *vin = nullptr;  // identity on the first pass
@@ -318,14 +346,25 @@ extern "C" int pmpm_partition(pmpm_context *ctx, int32_t dim, const pmpm_grid *g
     const bool to_dest = ((passes - 1 - p) % 2) == 0;
     uint32_t *kout = to_dest ? part->hash_sorted : ctx->keys_tmp;
     int32_t *vout = to_dest ? part->sorted : ctx->vals_tmp;
-    const int shift = p * kRadixBits;
-    hipLaunchKernelGGL(radix_hist_kernel, dim3(nblocks), dim3(kSortBlock), 0, s, kin, n, shift, ctx->hist, nblocks);
-    PMPM_LAUNCH_CHECK();
-    uint32_t *totals = ctx->hist + (size_t)kRadix * nblocks;
-    hipLaunchKernelGGL(scan_columns_kernel, dim3(kRadix), dim3(256), 0, s, ctx->hist, nblocks, totals);
-    PMPM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(radix_scatter_kernel, dim3(nblocks), dim3(kSortBlock), 0, s, kin, vin, kout, vout, n, shift,
-                       ctx->hist, nblocks, totals);
+    const int shift = p * rb;
+    const int radix = 1 << rb;
+    uint32_t *totals = ctx->hist + (size_t)radix * nblocks;
+    if (rb == 8) {
+      hipLaunchKernelGGL(radix_hist_kernel<8>, dim3(nblocks), dim3(kSortBlock), 0, s, kin, n, shift, ctx->hist, nblocks);
+      PMPM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(scan_columns_kernel, dim3(radix), dim3(256), 0, s, ctx->hist, nblocks, totals);
+      PMPM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(radix_scatter_kernel<8>, dim3(nblocks), dim3(kSortBlock), 0, s, kin, vin, kout, vout, n,
+                         shift, ctx->hist, nblocks, totals);
+    } else {
+      hipLaunchKernelGGL(radix_hist_kernel<10>, dim3(nblocks), dim3(kSortBlock), 0, s, kin, n, shift, ctx->hist,
+                         nblocks);
+      PMPM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(scan_columns_kernel, dim3(radix), dim3(256), 0, s, ctx->hist, nblocks, totals);
+      PMPM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(radix_scatter_kernel<10>, dim3(nblocks), dim3(kSortBlock), 0, s, kin, vin, kout, vout, n,
+                         shift, ctx->hist, nblocks, totals);
+    }
     PMPM_LAUNCH_CHECK();
     kin = kout;
     vin = vout;
