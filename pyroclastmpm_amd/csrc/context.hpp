@@ -40,8 +40,8 @@ int ensure_buffer(float *&buf, size_t &cap, size_t floats);
 
 namespace pmpm {
 
-constexpr int kSortBlock = 256;
-constexpr int kSortItems = 16;
+constexpr int kSortBlock = 1024;  // 16 waves rank 1024 keys per round
+constexpr int kSortItems = 4;
 constexpr int kSortTile = kSortBlock * kSortItems;
 constexpr int kMaxRadixBits = 10;  // digits of 8 bits (keys < 2^8) or 10 bits
 constexpr int kMaxRadix = 1 << kMaxRadixBits;

@@ -126,54 +126,40 @@ __global__ __launch_bounds__(256) void scan_columns_kernel(uint32_t *__restrict_
 }
 
 // Stable scatter of one radix pass (digits of RB bits). Within a round of
-// 256 keys (thread order == input order) a lane's rank among same-digit lanes
-// of its wave comes from RB ballots; per-wave counts are combined per digit
-// in LDS.
+// kSortBlock keys (thread order == input order) a lane's rank among
+// same-digit lanes of its wave comes from RB ballots; per-wave counts are
+// combined per digit in LDS (thread d owns digit d).
 template <int RB>
 __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
     const uint32_t *__restrict__ keys_in, const int32_t *__restrict__ vals_in, uint32_t *__restrict__ keys_out,
     int32_t *__restrict__ vals_out, int n, int shift, const uint32_t *__restrict__ hist, int nblocks,
     const uint32_t *__restrict__ totals) {
-  constexpr int R = 1 << RB, DPT = R / kSortBlock, W = kSortBlock / 64;
-  static_assert(R % kSortBlock == 0, "whole digits per thread");
+  constexpr int R = 1 << RB, W = kSortBlock / 64;
+  static_assert(R <= kSortBlock, "one digit per thread");
   __shared__ uint32_t base[R];
   __shared__ uint32_t wcnt[W][R];
-  __shared__ uint32_t part[kSortBlock];
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int w = t >> 6;
+  const bool owner = t < R;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  // digit bases: exclusive scan of the R column totals (thread t holds
-  // digits t*DPT .. t*DPT + DPT - 1)
-  uint32_t loc[DPT], sum = 0u;
-#pragma unroll
-  for (int k = 0; k < DPT; ++k) {
-    loc[k] = totals[t * DPT + k];
-    sum += loc[k];
-  }
-  part[t] = sum;
+  // digit bases: exclusive scan of the R column totals
+  const uint32_t tot = owner ? totals[t] : 0u;
+  if (owner) base[t] = tot;
   __syncthreads();
-  for (int off = 1; off < kSortBlock; off <<= 1) {
-    const uint32_t v = (t >= off) ? part[t - off] : 0u;
+  for (int off = 1; off < R; off <<= 1) {
+    const uint32_t v = (owner && t >= off) ? base[t - off] : 0u;
     __syncthreads();
-    part[t] += v;
+    if (owner) base[t] += v;
     __syncthreads();
   }
-  {
-    uint32_t run = (t > 0) ? part[t - 1] : 0u;
-#pragma unroll
-    for (int k = 0; k < DPT; ++k) {
-      const int d = t * DPT + k;
-      base[d] = run + hist[(int64_t)d * nblocks + blockIdx.x];
-      run += loc[k];
-    }
-  }
+  if (owner) base[t] = base[t] - tot + hist[(int64_t)t * nblocks + blockIdx.x];
   const int64_t tile = (int64_t)blockIdx.x * kSortTile;
   for (int r = 0; r < kSortItems; ++r) {
+    if (owner) {
 #pragma unroll
-    for (int k = 0; k < DPT; ++k)
-#pragma unroll
-      for (int q = 0; q < W; ++q) wcnt[q][k * kSortBlock + t] = 0u;
+      for (int q = 0; q < W; ++q) wcnt[q][t] = 0u;
+    }
     __syncthreads();
     const int64_t i = tile + (int64_t)r * kSortBlock + t;
     const bool valid = i < n;
@@ -189,17 +175,15 @@ __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
     const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
     if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < DPT; ++k) {
-      const int dd = k * kSortBlock + t;
-      uint32_t run = base[dd];
+    if (owner) {
+      uint32_t run = base[t];
 #pragma unroll
       for (int q = 0; q < W; ++q) {
-        const uint32_t c = wcnt[q][dd];
-        wcnt[q][dd] = run;
+        const uint32_t c = wcnt[q][t];
+        wcnt[q][t] = run;
         run += c;
       }
-      base[dd] = run;
+      base[t] = run;
     }
     __syncthreads();
     if (valid) {

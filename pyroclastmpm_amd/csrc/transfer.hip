@@ -384,6 +384,49 @@ __device__ inline void deposit_wrapped(const pmpm_grid &g, const pmpm_nodes &nod
   }
 }
 
+// Per-particle P2G inputs staged in LDS, one float (or int bits) per field,
+// field-major within a wave's region:
+//   frac[D]  position in the cell (t - bin, as the weights use it)
+//   lin      linear index of the particle's own bin (-1 outside the grid)
+//   m        mass (MODE 0/1)
+//   v[D]     velocity (MODE 0), v' (MODE 1), adjoint of the PIC velocity (MODE 2)
+//   A        -V sigma, the D(D+1)/2 components used in D dimensions (MODE 0)
+//   dv[D], Lb[D*D]  adjoints of the FLIP increment and of L (MODE 2)
+template <int D, int MODE> struct P2GStage {
+  static constexpr int NA = D * (D + 1) / 2;
+  static constexpr int F_LIN = D, F_M = D + 1, F_V = D + 2, F_X = 2 * D + 2;  // F_X: A (MODE 0) or dv (MODE 2)
+  static constexpr int NF = MODE == 0 ? F_X + NA : (MODE == 1 ? F_X : F_X + D + D * D);
+};
+// Particles of one wave's run that fit its LDS region at once: the whole run
+// of 16 cells at the usual 27 particles per cell (longer runs take more
+// passes). MODE 0 leaves room for one workgroup per CU (the accumulators
+// allow 2 waves per SIMD at most), MODE 1 for two.
+template <int D, int SF, int MODE> constexpr int p2g_stage_cap() {
+  constexpr int NV = MODE == 0 ? 1 + 2 * D : (MODE == 1 ? D : 2 * D);
+  constexpr int budget = MODE == 1 ? 18944 : 40448;  // floats per workgroup
+  constexpr int room = (budget - NV * P2GGeom<D, SF>::NT) / (P2GGeom<D, SF>::WAVES * P2GStage<D, MODE>::NF);
+  constexpr int cap = (room < 512 ? room : 512) / 64 * 64;
+  static_assert(cap >= 64, "LDS budget too small for the staging area");
+  return cap;
+}
+
+// LDS writes of a wave visible to all its lanes (and its earlier reads done)
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ inline int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ inline int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
 // P2G of one tile of cells.
 //   MODE 0: mass, momentum and force (force accumulates in mom_nt).
 //   MODE 1: MUSL re-map of m v' into mom_nt; v' is read from out.v in cell
@@ -396,15 +439,35 @@ __device__ inline void deposit_wrapped(const pmpm_grid &g, const pmpm_nodes &nod
 // gfx950). The tile goes with plain stores to this workgroup's slice of
 // `partials` (flags[tile] marks the slice live) and the grid kernel sums the
 // slices covering each node.
+// A wave's cells are consecutive in the linear cell order, so their particles
+// form one run of the sorted order. The wave loads the run into LDS first,
+// lane = particle, with coalesced loads (fraction, own bin, and the products
+// each lane needs), then the register loop reads only LDS: every particle line
+// is fetched from HBM once instead of being re-fetched as the cells' lanes
+// step through it together.
 // Node offsets outside the grid: the reference bound-checks only the linear
 // node index (usl_inline.h:122-126, shapefunction.cpp:96-98), so such a
 // deposit lands on the wrapped node, weighted with that node's cubic types.
 // Those sums go to the node arrays with one global atomic per cell, node
 // offset and field (wrap_quirk), or are dropped.
-// (at least 2 waves per SIMD: the 3D cubic accumulators need up to 256
-// registers)
+// Which P2G passes stage their particles in LDS: the MUSL re-map (8 floats
+// per particle, two workgroups per CU). The full P2G and the adjoint scatter
+// (14 and 20 floats) would fit one workgroup per CU only, and measured slower
+// that way than reading through L2 at two waves per SIMD.
+template <int MODE> constexpr bool p2g_staged() { return MODE == 1; }
+template <int D, int SF, int MODE> constexpr int p2g_lds_floats() {
+  constexpr int NV = MODE == 0 ? 1 + 2 * D : (MODE == 1 ? D : 2 * D);
+  return NV * P2GGeom<D, SF>::NT +
+         (p2g_staged<MODE>() ? P2GGeom<D, SF>::WAVES * P2GStage<D, MODE>::NF * p2g_stage_cap<D, SF, MODE>() : 1);
+}
+// Waves per SIMD the LDS allows (4 waves per workgroup, 160 KB per CU); at 1
+// the register budget is the whole file.
+template <int D, int SF, int MODE> constexpr int p2g_waves_per_simd() {
+  return p2g_lds_floats<D, SF, MODE>() * 4 * 2 <= 163840 ? 2 : 1;
+}
+
 template <int D, int SF, int MODE>
-__global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_waves_per_eu(2))) void p2g_kernel(pmpm_params prm, pmpm_grid g,
+__global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_waves_per_eu(p2g_waves_per_simd<D, SF, MODE>()))) void p2g_kernel(pmpm_params prm, pmpm_grid g,
                                                                       pmpm_particles in, pmpm_particles out,
                                                                       pmpm_spatial part, pmpm_nodes nodes,
                                                                       int tiles_x, int tiles_y,
@@ -415,7 +478,11 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
   constexpr int SPAN = G::SPAN, NLO = G::NLO, SZ = G::SZ, SZH = G::SZH, NE = SPAN * SZH;
   constexpr int NV = MODE == 0 ? 1 + 2 * D : (MODE == 1 ? D : 2 * D);
   constexpr int NXP = G::NXP, NY = G::NY, NT = G::NT;
+  using ST = P2GStage<D, MODE>;
+  constexpr bool kStaged = p2g_staged<MODE>();
+  constexpr int NF = ST::NF, CW = p2g_stage_cap<D, SF, MODE>();
   __shared__ float tile[NV * NT];
+  __shared__ float stage[kStaged ? G::WAVES * NF * CW : 1];
 
   const int n = in.n;
   const int nn = g.num_cells_total;
@@ -436,9 +503,9 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
   bool in_grid = c[0] < g.num_cells[0];
   if (D > 1) in_grid = in_grid && c[1] < g.num_cells[1];
   if (D > 2) in_grid = in_grid && c[2] < g.num_cells[2];
-  int cs = 0, ce = 0;
+  int cs = 0, ce = 0, cell = -2;
   if (in_grid) {
-    const int cell = (int)lin_index(g, c, D);
+    cell = (int)lin_index(g, c, D);
     ce = part.cell_end[cell];
     cs = part.cell_start[cell];
   }
@@ -486,35 +553,8 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
     for (int k = 0; k < 4; ++k) yc[k] = kCubic[ty][3 - sy][k];
   }
 
-  // the next particle's inputs are loaded while the current one is summed
-  int j = cs;
-  int slot = j < ce ? part.sorted[j] : 0;
-  int slot_next = j + 1 < ce ? part.sorted[j + 1] : 0;
-  P2GRec cur;
-  if (j < ce) p2g_load<D, MODE>(in, out, rec, n, slot, j, cur);
-  for (; j < ce; ++j) {
-    const int jn = j + 1;
-    P2GRec nxt = cur;
-    if (jn < ce) p2g_load<D, MODE>(in, out, rec, n, slot_next, jn, nxt);
-    const int slot_cur = slot;
-    slot = slot_next;
-    slot_next = jn + 1 < ce ? part.sorted[jn + 1] : 0;
-
-    float frac[3] = {0.f, 0.f, 0.f};
-    bool own = true;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      const float ta = cell_coord(cur.x[a], g.origin[a], inv_h);
-      const int ba = (int)ta;
-      own = own && ba == c[a];
-      frac[a] = __fsub_rn(ta, (float)ba);
-    }
-    if (!own) {
-      // (MODE 2 is only ever called on a partition of the same positions)
-      if (MODE != 2 && zh == 0) p2g_stray<D, SF, MODE>(prm, g, nodes, in.x, n, slot_cur, sy, cur.m, cur.v, cur.A);
-      cur = nxt;
-      continue;
-    }
+  // one particle's contribution to this lane's register sums
+  auto deposit = [&](const P2GRec &cur, const float *frac) {
     float Nx[SPAN], dNx[SPAN], Nz[SZH], dNz[SZH], Ny = 1.f, dNy = 0.f;
     span_evals<SF, 0, SPAN>(frac[0], tx, inv_h, Nx, dNx);
     if constexpr (D > 2) {
@@ -587,7 +627,142 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
         }
       }
     }
-    cur = nxt;
+  };
+
+  if constexpr (!kStaged) {
+    // the next particle's inputs are loaded while the current one is summed
+    int j = cs;
+    int slot = j < ce ? part.sorted[j] : 0;
+    int slot_next = j + 1 < ce ? part.sorted[j + 1] : 0;
+    P2GRec cur;
+    if (j < ce) p2g_load<D, MODE>(in, out, rec, n, slot, j, cur);
+    for (; j < ce; ++j) {
+      const int jn = j + 1;
+      P2GRec nxt = cur;
+      if (jn < ce) p2g_load<D, MODE>(in, out, rec, n, slot_next, jn, nxt);
+      const int slot_cur = slot;
+      slot = slot_next;
+      slot_next = jn + 1 < ce ? part.sorted[jn + 1] : 0;
+      float frac[3] = {0.f, 0.f, 0.f};
+      bool own = true;
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        const float ta = cell_coord(cur.x[a], g.origin[a], inv_h);
+        const int ba = (int)ta;
+        own = own && ba == c[a];
+        frac[a] = __fsub_rn(ta, (float)ba);
+      }
+      if (own) deposit(cur, frac);
+      else if (MODE != 2 && zh == 0)  // (MODE 2 runs on a partition of the same positions)
+        p2g_stray<D, SF, MODE>(prm, g, nodes, in.x, n, slot_cur, sy, cur.m, cur.v, cur.A);
+      cur = nxt;
+    }
+  } else {
+    // this wave's run [run_lo, run_hi) of the sorted order (MODE 1)
+    const int run_lo = wave_min(ce > cs ? cs : 0x7fffffff);
+    const int run_hi = wave_max(ce > cs ? ce : 0);
+    float *st = stage + wave * (NF * CW);
+    for (int w0 = run_lo; w0 < run_hi; w0 += CW) {  // wave-uniform
+      const int wn = min(CW, run_hi - w0);
+      // straight-line staging: every slot, then every field load, so the loads
+      // of a lane's CW / 64 particles are in flight together (rounds past the
+      // window read particle w0 and fill LDS slots nobody reads)
+      constexpr int KR = CW / 64;
+      int slots[KR], js[KR];
+#pragma unroll
+      for (int r = 0; r < KR; ++r) {
+        const int q = lane + 64 * r;
+        js[r] = q < wn ? w0 + q : w0;
+        slots[r] = part.sorted[js[r]];
+      }
+#pragma unroll
+      for (int r = 0; r < KR; ++r) {
+        const int q = lane + 64 * r;
+        const int jj = js[r], slot = slots[r];
+        int b[3] = {0, 0, 0};
+        bool ing = true;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          const float ta = cell_coord(in.x[a * n + slot], g.origin[a], inv_h);
+          b[a] = (int)ta;
+          ing = ing && b[a] >= 0 && b[a] < g.num_cells[a];
+          st[a * CW + q] = __fsub_rn(ta, (float)b[a]);
+        }
+        st[ST::F_LIN * CW + q] = __int_as_float(ing ? (int)lin_index(g, b, D) : -1);
+        if (MODE != 2) st[ST::F_M * CW + q] = in.mass[slot];
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+          st[(ST::F_V + a) * CW + q] = MODE == 0 ? in.v[a * n + slot] : (MODE == 1 ? out.v[a * n + jj] : rec[a * n + jj]);
+        if constexpr (MODE == 0) {
+          const float mvol = -in.volume[slot];
+          const Sym sg = load_sym(in.stress, n, slot);
+          const float A6[6] = {mvol * sg.xx, mvol * sg.yy, mvol * sg.zz, mvol * sg.xy, mvol * sg.xz, mvol * sg.yz};
+          // D components in sym order: 1D xx; 2D xx yy xy; 3D all six
+          constexpr int pick[3][6] = {{0}, {0, 1, 3}, {0, 1, 2, 3, 4, 5}};
+#pragma unroll
+          for (int k = 0; k < ST::NA; ++k) st[(ST::F_X + k) * CW + q] = A6[pick[D - 1][k]];
+        } else if constexpr (MODE == 2) {
+#pragma unroll
+          for (int a = 0; a < D; ++a) st[(ST::F_X + a) * CW + q] = rec[(3 + a) * n + jj];
+#pragma unroll
+          for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int bb = 0; bb < D; ++bb) st[(ST::F_X + D + a * D + bb) * CW + q] = rec[(6 + 3 * a + bb) * n + jj];
+        }
+      }
+      wave_lds_sync();
+
+      // this lane's particles in the staged window
+      const int j0 = max(cs, w0), j1 = min(ce, w0 + wn);
+      auto read = [&](int q, P2GRec &r, float *fr, int &lin) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) fr[a] = st[a * CW + q];
+        lin = __float_as_int(st[ST::F_LIN * CW + q]);
+        r.m = MODE == 2 ? 1.f : st[ST::F_M * CW + q];
+#pragma unroll
+        for (int a = 0; a < D; ++a) r.v[a] = st[(ST::F_V + a) * CW + q];
+        r.A = sym_zero();
+        if constexpr (MODE == 0) {
+          if (D == 1) r.A.xx = st[ST::F_X * CW + q];
+          if (D == 2) {
+            r.A.xx = st[ST::F_X * CW + q]; r.A.yy = st[(ST::F_X + 1) * CW + q]; r.A.xy = st[(ST::F_X + 2) * CW + q];
+          }
+          if (D == 3) {
+            r.A.xx = st[ST::F_X * CW + q]; r.A.yy = st[(ST::F_X + 1) * CW + q]; r.A.zz = st[(ST::F_X + 2) * CW + q];
+            r.A.xy = st[(ST::F_X + 3) * CW + q]; r.A.xz = st[(ST::F_X + 4) * CW + q]; r.A.yz = st[(ST::F_X + 5) * CW + q];
+          }
+        } else if constexpr (MODE == 2) {
+#pragma unroll
+          for (int a = 0; a < D; ++a) r.dv[a] = st[(ST::F_X + a) * CW + q];
+#pragma unroll
+          for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int bb = 0; bb < D; ++bb) r.Lb[3 * a + bb] = st[(ST::F_X + D + a * D + bb) * CW + q];
+        }
+      };
+      // the next particle's inputs are read while the current one is summed
+      P2GRec cur, nxt;
+      float fr[3] = {0.f, 0.f, 0.f}, fr_n[3] = {0.f, 0.f, 0.f};
+      int lin = -1, lin_n = -1;
+      if (j0 < j1) read(j0 - w0, cur, fr, lin);
+      for (int j = j0; j < j1; ++j) {
+        if (j + 1 < j1) read(j + 1 - w0, nxt, fr_n, lin_n);
+        if (lin != cell) {
+          // a particle listed in this cell whose bin is another (only when the
+          // partition was built from other positions); MODE 2 is only ever
+          // called on a partition of the same positions
+          if (MODE != 2 && zh == 0)
+            p2g_stray<D, SF, MODE>(prm, g, nodes, in.x, n, part.sorted[j], sy, cur.m, cur.v, cur.A);
+        } else {
+          deposit(cur, fr);
+        }
+        cur = nxt;
+        lin = lin_n;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) fr[a] = fr_n[a];
+      }
+      wave_lds_sync();
+    }
   }
 
   // in-grid sums into the LDS tile, one node offset (sx, sz) per phase; the
