@@ -113,13 +113,13 @@ class Workload:
 
         self.sets = [set_(True), set_(False)]
         self.cur = 0
-        self.nodes_t = {k: f32((nn,) if k == "mass" else (3 * nn,)) for k in ["mass", "mom", "mom_nt", "vel_nt",
-                                                                                "dvel"]}
+        self.nodes_t = {k: f32((nn,) if k == "mass" else (3 * nn,)) for k in ["mass", "mom", "mom_nt"]}
+        self.nodes_t["vel"] = f32((capi.node_vel_stride(3) * nn,))
         self.sp_t = {k: torch.zeros(n, dtype=torch.int32, device=dev) for k in ["hash", "hash_sorted", "sorted"]}
         self.sp_t["cell_start"] = torch.zeros(nn, dtype=torch.int32, device=dev)
         self.sp_t["cell_end"] = torch.zeros(nn, dtype=torch.int32, device=dev)
         self.ctx = capi.Context(n, nn)
-        self.nodes = capi.Nodes(*[self.nodes_t[k].data_ptr() for k in ["mass", "mom", "mom_nt", "vel_nt", "dvel"]])
+        self.nodes = capi.Nodes(*[self.nodes_t[k].data_ptr() for k in ["mass", "mom", "mom_nt", "vel"]])
         self.spatial = capi.Spatial(self.sp_t["hash"].data_ptr(), self.sp_t["hash_sorted"].data_ptr(),
                                     self.sp_t["sorted"].data_ptr(), self.sp_t["cell_start"].data_ptr(),
                                     self.sp_t["cell_end"].data_ptr(), None)

@@ -80,13 +80,19 @@ typedef struct pmpm_particles {
   float *stress_ref; /* [6][n] reference stress (Drucker-Prager) or NULL */
 } pmpm_particles;
 
+/* Floats per node of the derived velocity records pmpm_nodes.vel. */
+#define PMPM_NODE_VEL_STRIDE(dim) ((dim) == 3 ? 8 : 2 * (dim))
+
 /* Node state (nodes.h:113-140 NodesContainer fields, as SoA). */
 typedef struct pmpm_nodes {
-  float *mass;    /* [nn] */
-  float *mom;     /* [dim][nn] moments */
-  float *mom_nt;  /* [dim][nn] forward moments (USL), re-mapped moments (MUSL) */
-  float *vel_nt;  /* [dim][nn] derived: mom_nt / mass (0 where mass <= cutoff) */
-  float *dvel;    /* [dim][nn] derived: (mom_nt - mom) / mass */
+  float *mass;   /* [nn] */
+  float *mom;    /* [dim][nn] moments */
+  float *mom_nt; /* [dim][nn] forward moments (USL), re-mapped moments (MUSL) */
+  float *vel;    /* [nn][PMPM_NODE_VEL_STRIDE(dim)] derived per node, written by
+                    the grid update for the gathers: v_nt = mom_nt / mass
+                    (dim floats), then dv = (mom_nt - mom) / mass (dim floats),
+                    zero padded; all zero where mass <= cutoff. No counterpart
+                    in the reference (it divides per gather). */
 } pmpm_nodes;
 
 /* Spatial partition (spatialpartition.h:77-132 SpatialPartition fields). */
@@ -177,7 +183,7 @@ int pmpm_p2g(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid,
  * Gravity, NodeDomain (gravity.cpp:65-88, nodedomain.cpp:42-64) and the
  * analytic box body, applied in list order to prm->bcs[bc_begin, bc_end).
  * remap_pass = 1 applies only kinematic constraints (MUSL second map).
- * Always recomputes vel_nt and dvel afterwards. */
+ * Always recomputes the node velocity records (vel) afterwards. */
 int pmpm_grid_bc(const pmpm_params *prm, const pmpm_grid *grid, pmpm_nodes *nodes,
                  int32_t bc_begin, int32_t bc_end, int32_t remap_pass, pmpm_stream_t stream);
 

@@ -38,8 +38,13 @@ class Particles(C.Structure):
                 ("eps_e", vp), ("stress_ref", vp)]
 
 
+def node_vel_stride(dim):
+    """Floats per node of pmpm_nodes.vel (PMPM_NODE_VEL_STRIDE)."""
+    return 8 if dim == 3 else 2 * dim
+
+
 class Nodes(C.Structure):
-    _fields_ = [("mass", vp), ("mom", vp), ("mom_nt", vp), ("vel_nt", vp), ("dvel", vp)]
+    _fields_ = [("mass", vp), ("mom", vp), ("mom_nt", vp), ("vel", vp)]
 
 
 class Spatial(C.Structure):

@@ -71,16 +71,14 @@ NodesContainer<D>::NodesContainer(const VecD<D> &o, const VecD<D> &e, float h) :
   mass.resize(nn);
   mom.resize(nn * D);
   mom_nt.resize(nn * D);
-  vel_nt.resize(nn * D);
-  dvel.resize(nn * D);
+  vel.resize(nn * PMPM_NODE_VEL_STRIDE(D));
   reset();
 }
 template <int D> void NodesContainer<D>::reset() {
   hip_check(hipMemset(mass.get(), 0, mass.size() * 4), "memset");
   hip_check(hipMemset(mom.get(), 0, mom.size() * 4), "memset");
   hip_check(hipMemset(mom_nt.get(), 0, mom_nt.size() * 4), "memset");
-  hip_check(hipMemset(vel_nt.get(), 0, vel_nt.size() * 4), "memset");
-  hip_check(hipMemset(dvel.get(), 0, dvel.size() * 4), "memset");
+  hip_check(hipMemset(vel.get(), 0, vel.size() * 4), "memset");
 }
 template <int D> std::vector<VecI<D>> NodesContainer<D>::node_ids() const {
   std::vector<VecI<D>> ids(grid.num_cells_total);
@@ -145,7 +143,7 @@ template <int D> void NodesContainer<D>::set_masses(const std::vector<float> &v)
   mass.upload(v);
 }
 template <int D> pmpm_nodes NodesContainer<D>::view() const {
-  return pmpm_nodes{mass.get(), mom.get(), mom_nt.get(), vel_nt.get(), dvel.get()};
+  return pmpm_nodes{mass.get(), mom.get(), mom_nt.get(), vel.get()};
 }
 
 // ============================ ParticlesContainer ===========================

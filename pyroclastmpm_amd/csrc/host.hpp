@@ -120,7 +120,7 @@ template <int D> class NodesContainer {
   bool node_types_built = false;  // cubic spline selected when constructed
   float small_mass_cutoff = 0.000000001f;
   std::vector<std::string> output_formats;
-  DevBuf<float> mass, mom, mom_nt, vel_nt, dvel;
+  DevBuf<float> mass, mom, mom_nt, vel;  // vel: derived records for the gathers
 };
 
 // ---------------------------------------------------------------------------

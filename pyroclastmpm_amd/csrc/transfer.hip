@@ -102,13 +102,51 @@ __device__ inline void particle_weights(const pmpm_params &prm, const pmpm_grid 
   }
 }
 
-// Where a gather reads node velocities from: the node arrays (lo = 0,
-// strides of the grid, component stride nn) or a box of them staged in LDS.
-struct NodeSrc {
-  const float *vel, *dvel;  // vel_nt and dvel, component-major
-  int cstride;              // distance between components
-  int lo[3], stride[3];     // node (i, j, k) sits at sum (id_a - lo_a) stride_a
+// Node velocity records (pmpm_nodes.vel): NP floats per node, v_nt then dv,
+// so a gather entry is one 16-byte load (two for v_nt and dv in 3D) instead
+// of one load per component.
+template <int D> struct NodeVel {
+  static constexpr int NP = D == 3 ? 8 : 2 * D;
 };
+template <int D, bool WANT_DV>
+__device__ inline void load_node_vel(const float *__restrict__ vel, int node, float *v, float *dv) {
+  if constexpr (D == 3) {
+    const float4 a = *reinterpret_cast<const float4 *>(vel + 8 * (int64_t)node);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z;
+    if constexpr (WANT_DV) {
+      const float4 b = *reinterpret_cast<const float4 *>(vel + 8 * (int64_t)node + 4);
+      dv[0] = a.w; dv[1] = b.x; dv[2] = b.y;
+    }
+  } else if constexpr (D == 2) {
+    const float4 a = *reinterpret_cast<const float4 *>(vel + 4 * (int64_t)node);
+    v[0] = a.x; v[1] = a.y;
+    if constexpr (WANT_DV) { dv[0] = a.z; dv[1] = a.w; }
+  } else {
+    const float2 a = *reinterpret_cast<const float2 *>(vel + 2 * (int64_t)node);
+    v[0] = a.x;
+    if constexpr (WANT_DV) dv[0] = a.y;
+  }
+}
+template <int D> __device__ inline void store_node_vel(float *__restrict__ vel, int node, const float *v, const float *dv) {
+  if constexpr (D == 3) {
+    float4 *p = reinterpret_cast<float4 *>(vel + 8 * (int64_t)node);
+    p[0] = make_float4(v[0], v[1], v[2], dv[0]);
+    p[1] = make_float4(dv[1], dv[2], 0.f, 0.f);
+  } else if constexpr (D == 2) {
+    *reinterpret_cast<float4 *>(vel + 4 * (int64_t)node) = make_float4(v[0], v[1], dv[0], dv[1]);
+  } else {
+    *reinterpret_cast<float2 *>(vel + 2 * (int64_t)node) = make_float2(v[0], dv[0]);
+  }
+}
+// Where a gather reads node velocities from: the records and the grid's node
+// strides.
+struct NodeSrc {
+  const float *vel;  // pmpm_nodes.vel
+  int stride[3];     // node (i, j, k) sits at sum id_a stride_a
+};
+__device__ inline NodeSrc node_src(const pmpm_grid &g, const pmpm_nodes &nodes) {
+  return NodeSrc{nodes.vel, {1, g.num_cells[0], g.num_cells[0] * g.num_cells[1]}};
+}
 
 // Grid gather in the reference's window order (y outer, x, z inner). The
 // reference skips entries outside the grid on any axis (usl_inline.h:281-293);
@@ -121,7 +159,6 @@ __device__ inline void gather_grid(const pmpm_grid &g, const NodeSrc &src, const
   constexpr int E = Spline<SF>::ENTRIES;
   constexpr int EY = D > 1 ? E : 1;
   constexpr int EZ = D > 2 ? E : 1;
-  const int nn = src.cstride;
   float N[3][4], dN[3][4];
   int off[3][4];
 #pragma unroll
@@ -132,7 +169,7 @@ __device__ inline void gather_grid(const pmpm_grid &g, const NodeSrc &src, const
       if (a < D && e < E) {
         const int i = w.bin[a] + w.first[a] + e;
         const bool ok = i >= 0 && i < g.num_cells[a];
-        off[a][e] = (min(max(i, 0), g.num_cells[a] - 1) - src.lo[a]) * stride;
+        off[a][e] = min(max(i, 0), g.num_cells[a] - 1) * stride;
         N[a][e] = ok ? w.N[a][e] : 0.f;
         dN[a][e] = ok ? w.dN[a][e] : 0.f;
       } else {
@@ -157,14 +194,12 @@ __device__ inline void gather_grid(const pmpm_grid &g, const NodeSrc &src, const
         gr[0] = dxy * N[2][ez];
         gr[1] = dyx * N[2][ez];
         gr[2] = (dN[2][ez] * N[0][ex]) * N[1][ey];
-        float vnt[3];
-#pragma unroll
-        for (int a = 0; a < D; ++a) vnt[a] = src.vel[a * nn + node];
+        float vnt[3], dv[3];
+        load_node_vel<D, WANT_V>(src.vel, node, vnt, dv);
         if (WANT_V) {
 #pragma unroll
           for (int a = 0; a < D; ++a) {
-            const float dv = src.dvel[a * nn + node];
-            dvinc[a] += wgt * dv;
+            dvinc[a] += wgt * dv[a];
             vinc[a] += wgt * vnt[a];
           }
         }
@@ -392,18 +427,22 @@ __device__ inline void deposit_wrapped(const pmpm_grid &g, const pmpm_nodes &nod
 //   v[D]     velocity (MODE 0), v' (MODE 1), adjoint of the PIC velocity (MODE 2)
 //   A        -V sigma, the D(D+1)/2 components used in D dimensions (MODE 0)
 //   dv[D], Lb[D*D]  adjoints of the FLIP increment and of L (MODE 2)
+// Fields per node of a P2G pass: MODE 0 mass, momentum, force; MODE 1 and 3
+// re-mapped momentum; MODE 2 the adjoints of v_nt and dv.
+template <int D, int MODE> constexpr int p2g_nv() { return MODE == 0 ? 1 + 2 * D : (MODE == 2 ? 2 * D : D); }
+
 template <int D, int MODE> struct P2GStage {
   static constexpr int NA = D * (D + 1) / 2;
   static constexpr int F_LIN = D, F_M = D + 1, F_V = D + 2, F_X = 2 * D + 2;  // F_X: A (MODE 0) or dv (MODE 2)
-  static constexpr int NF = MODE == 0 ? F_X + NA : (MODE == 1 ? F_X : F_X + D + D * D);
+  static constexpr int NF = MODE == 0 ? F_X + NA : (MODE == 2 ? F_X + D + D * D : F_X);
 };
 // Particles of one wave's run that fit its LDS region at once: the whole run
 // of 16 cells at the usual 27 particles per cell (longer runs take more
 // passes). MODE 0 leaves room for one workgroup per CU (the accumulators
 // allow 2 waves per SIMD at most), MODE 1 for two.
 template <int D, int SF, int MODE> constexpr int p2g_stage_cap() {
-  constexpr int NV = MODE == 0 ? 1 + 2 * D : (MODE == 1 ? D : 2 * D);
-  constexpr int budget = MODE == 1 ? 18944 : 40448;  // floats per workgroup
+  constexpr int NV = p2g_nv<D, MODE>();
+  constexpr int budget = (MODE == 1 || MODE == 3) ? 18944 : 40448;  // floats per workgroup
   constexpr int room = (budget - NV * P2GGeom<D, SF>::NT) / (P2GGeom<D, SF>::WAVES * P2GStage<D, MODE>::NF);
   constexpr int cap = (room < 512 ? room : 512) / 64 * 64;
   static_assert(cap >= 64, "LDS budget too small for the staging area");
@@ -427,10 +466,92 @@ __device__ inline int wave_max(int v) {
   return v;
 }
 
+// v_nt and dv gathered over one particle's own window: the rare path of the
+// fused MUSL map, kept as a compact loop (entries outside the grid skipped, as
+// the reference does) so it costs the fused pass no registers.
+template <int D, int SF>
+__device__ inline void gather_one(const pmpm_params &prm, const pmpm_grid &g, const float *x, int n, int slot,
+                                  const pmpm_nodes &nodes, float *vinc, float *dvinc) {
+  constexpr int SPAN = Spline<SF>::SPAN, NLO = Spline<SF>::NLO;
+  constexpr int NENT = D == 1 ? SPAN : (D == 2 ? SPAN * SPAN : SPAN * SPAN * SPAN);
+  const float inv_h = g.inv_cell_size;
+  float fr[3] = {0.f, 0.f, 0.f};
+  int b[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    const float t = cell_coord(x[a * n + slot], g.origin[a], inv_h);
+    b[a] = (int)t;
+    fr[a] = __fsub_rn(t, (float)b[a]);
+  }
+#pragma unroll 1
+  for (int e = 0; e < NENT; ++e) {
+    int id[3] = {0, 0, 0};
+    float w = 1.f;
+    bool in = true;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      const int o = NLO + (a == 0 ? e % SPAN : (a == 1 ? (e / SPAN) % SPAN : e / (SPAN * SPAN)));
+      id[a] = b[a] + o;
+      in = in && id[a] >= 0 && id[a] < g.num_cells[a];
+      float N, dN;
+      spline_eval<SF>(__fsub_rn(fr[a], (float)o), cubic_node_type(id[a], g.num_cells[a], prm.node_types_built),
+                      inv_h, N, dN);
+      w *= N;
+    }
+    if (!in || w == 0.f) continue;
+    float v[3], dv[3];
+    load_node_vel<D, true>(nodes.vel, (int)lin_index(g, id, D), v, dv);
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      vinc[a] += w * v[a];
+      dvinc[a] += w * dv[a];
+    }
+  }
+}
+
+// Workgroups after the tiles of a MODE 3 launch: they run the first map of
+// the particles outside the grid, which sort last and belong to no cell.
+constexpr int kMuslTailBlocks = 4;
+template <int D, int SF>
+__device__ void musl_first_map_tail(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in,
+                                    const pmpm_particles &out, const pmpm_spatial &part, const pmpm_nodes &nodes,
+                                    int tail_block) {
+  const int n = in.n;
+  const uint32_t nc = (uint32_t)g.num_cells_total;
+  if (n == 0 || part.hash_sorted[n - 1] < nc) return;  // none outside
+  int lo = 0, hi = n - 1;  // first j with hash_sorted[j] == nc
+  while (lo < hi) {
+    const int mid = lo + (hi - lo) / 2;
+    if (part.hash_sorted[mid] >= nc) hi = mid;
+    else lo = mid + 1;
+  }
+  const float am = (float)(1.0 - (double)prm.alpha);
+  for (int j = lo + tail_block * (int)blockDim.x + (int)threadIdx.x; j < n; j += kMuslTailBlocks * (int)blockDim.x) {
+    const int src = part.sorted[j];
+    PW<D, SF> w;
+    float xp[3];
+    particle_weights<D, SF>(prm, g, in.x, n, src, w, xp);
+    float vinc[3] = {0.f, 0.f, 0.f}, dvinc[3] = {0.f, 0.f, 0.f}, L[9];
+    gather_grid<D, SF, true, false>(g, node_src(g, nodes), w, vinc, dvinc, L);
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      out.v[a * n + j] = am * vinc[a] + prm.alpha * (in.v[a * n + src] + dvinc[a]);
+      out.x[a * n + j] = xp[a] + prm.dt * vinc[a];
+    }
+  }
+}
+
 // P2G of one tile of cells.
 //   MODE 0: mass, momentum and force (force accumulates in mom_nt).
 //   MODE 1: MUSL re-map of m v' into mom_nt; v' is read from out.v in cell
 //           order (written by the first map), weights from the positions in.x.
+//   MODE 2: adjoint scatter of the G2P record adjoints.
+//   MODE 3: MUSL first map and re-map in one pass: each particle's v', x'
+//           from the node velocity records (written to slot j of `out`), then
+//           m v' deposited with the same weights (the reference's
+//           KERNEL_MUSL_G2P_DOUBLE_MAPPING and KERNEL_MUSL_P2G_DOUBLE_MAPPING,
+//           musl_kernels.cuh:33-60). Particles outside the grid are mapped by
+//           the kMuslTailBlocks workgroups after the tiles.
 // Lane (cell c, sy) walks the particles of c in sorted order and sums in
 // registers their contributions to its SPAN x SZ nodes c + NLO + (sx, sy, sz).
 // The sums then enter an LDS node tile in SPAN x SZ phases: within one phase
@@ -454,9 +575,9 @@ __device__ inline int wave_max(int v) {
 // per particle, two workgroups per CU). The full P2G and the adjoint scatter
 // (14 and 20 floats) would fit one workgroup per CU only, and measured slower
 // that way than reading through L2 at two waves per SIMD.
-template <int MODE> constexpr bool p2g_staged() { return MODE == 1; }
+template <int MODE> constexpr bool p2g_staged() { return MODE == 1 || MODE == 3; }
 template <int D, int SF, int MODE> constexpr int p2g_lds_floats() {
-  constexpr int NV = MODE == 0 ? 1 + 2 * D : (MODE == 1 ? D : 2 * D);
+  constexpr int NV = p2g_nv<D, MODE>();
   return NV * P2GGeom<D, SF>::NT +
          (p2g_staged<MODE>() ? P2GGeom<D, SF>::WAVES * P2GStage<D, MODE>::NF * p2g_stage_cap<D, SF, MODE>() : 1);
 }
@@ -476,7 +597,7 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
                                                                       const float *__restrict__ rec) {
   using G = P2GGeom<D, SF>;
   constexpr int SPAN = G::SPAN, NLO = G::NLO, SZ = G::SZ, SZH = G::SZH, NE = SPAN * SZH;
-  constexpr int NV = MODE == 0 ? 1 + 2 * D : (MODE == 1 ? D : 2 * D);
+  constexpr int NV = p2g_nv<D, MODE>();
   constexpr int NXP = G::NXP, NY = G::NY, NT = G::NT;
   using ST = P2GStage<D, MODE>;
   constexpr bool kStaged = p2g_staged<MODE>();
@@ -498,6 +619,12 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
   const int bz = blockIdx.x / (tiles_x * tiles_y);
   const int c[3] = {bx * G::TX + lx, D > 1 ? by : 0, D > 2 ? bz * G::TZ + lz : 0};
 
+  if constexpr (MODE == 3) {
+    if ((int)blockIdx.x >= (int)gridDim.x - kMuslTailBlocks) {
+      musl_first_map_tail<D, SF>(prm, g, in, out, part, nodes, (int)blockIdx.x - ((int)gridDim.x - kMuslTailBlocks));
+      return;
+    }
+  }
   for (int i = t; i < NV * NT; i += G::THREADS) tile[i] = 0.f;
 
   bool in_grid = c[0] < g.num_cells[0];
@@ -511,8 +638,9 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
   }
   // cubic weights without node types are zero (cubic.cuh: type 0): nothing
   // to deposit
+  // (MODE 3 still moves the particles: its first map gathers zeros)
   const bool none = SF == PMPM_SF_CUBIC && !prm.node_types_built;
-  if (!__syncthreads_or(ce > cs) || none) {
+  if (!__syncthreads_or(ce > cs) || (none && MODE != 3)) {
     if (t == 0) flags[blockIdx.x] = 0;
     return;
   }
@@ -553,9 +681,38 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
     for (int k = 0; k < 4; ++k) yc[k] = kCubic[ty][3 - sy][k];
   }
 
-  // one particle's contribution to this lane's register sums
-  auto deposit = [&](const P2GRec &cur, const float *frac) {
-    float Nx[SPAN], dNx[SPAN], Nz[SZH], dNz[SZH], Ny = 1.f, dNy = 0.f;
+  // MODE 3: the velocity records of this lane's SPAN x SZH nodes, loaded once
+  // for all the cell's particles (zero outside the grid, as the gather skips
+  // such entries)
+  constexpr int NVN = MODE == 3 ? 2 * D : 1;
+  float vnode[SPAN][SZH][NVN];
+  if constexpr (MODE == 3) {
+#pragma unroll
+    for (int sx = 0; sx < SPAN; ++sx)
+#pragma unroll
+      for (int sz = 0; sz < SZH; ++sz) {
+        float v[3] = {0.f, 0.f, 0.f}, dv[3] = {0.f, 0.f, 0.f};
+        if (in_grid && x_in[sx] && y_in && z_in[zh * SZH + sz]) {
+          const int nb[3] = {c[0] + NLO + sx, c[1] + NLO + sy, c[2] + NLO + zh * SZH + sz};
+          load_node_vel<D, true>(nodes.vel, (int)lin_index(g, nb, D), v, dv);
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          vnode[sx][sz][a] = v[a];
+          vnode[sx][sz][D + a] = dv[a];
+        }
+      }
+  }
+  const float am = (float)(1.0 - (double)prm.alpha);
+
+  // a particle's weights for this lane's nodes: Nx, Nz per span offset and
+  // the lane's one Ny (with derivatives)
+  struct Wts {
+    float Nx[SPAN], dNx[SPAN], Nz[SZH], dNz[SZH], Ny, dNy;
+  };
+  auto weights = [&](const float *frac, Wts &W) {
+    float *Nx = W.Nx, *dNx = W.dNx, *Nz = W.Nz, *dNz = W.dNz;
+    float Ny = 1.f, dNy = 0.f;
     span_evals<SF, 0, SPAN>(frac[0], tx, inv_h, Nx, dNx);
     if constexpr (D > 2) {
       if constexpr (G::ZH == 1) {
@@ -578,6 +735,13 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
         spline_eval<SF>(r, 0, inv_h, Ny, dNy);
       }
     }
+    W.Ny = Ny;
+    W.dNy = dNy;
+  };
+  // one particle's contribution to this lane's register sums
+  auto deposit = [&](const P2GRec &cur, const Wts &W) {
+    const float *Nx = W.Nx, *dNx = W.dNx, *Nz = W.Nz, *dNz = W.dNz;
+    const float Ny = W.Ny, dNy = W.dNy;
     const float m = cur.m;
 #pragma unroll
     for (int sx = 0; sx < SPAN; ++sx) {
@@ -652,8 +816,11 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
         own = own && ba == c[a];
         frac[a] = __fsub_rn(ta, (float)ba);
       }
-      if (own) deposit(cur, frac);
-      else if (MODE != 2 && zh == 0)  // (MODE 2 runs on a partition of the same positions)
+      if (own) {
+        Wts W;
+        weights(frac, W);
+        deposit(cur, W);
+      } else if (MODE != 2 && zh == 0)  // (MODE 2 runs on a partition of the same positions)
         p2g_stray<D, SF, MODE>(prm, g, nodes, in.x, n, slot_cur, sy, cur.m, cur.v, cur.A);
       cur = nxt;
     }
@@ -686,13 +853,15 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
           const float ta = cell_coord(in.x[a * n + slot], g.origin[a], inv_h);
           b[a] = (int)ta;
           ing = ing && b[a] >= 0 && b[a] < g.num_cells[a];
-          st[a * CW + q] = __fsub_rn(ta, (float)b[a]);
+          // (MODE 3 keeps the position itself: the first map moves it)
+          st[a * CW + q] = MODE == 3 ? in.x[a * n + slot] : __fsub_rn(ta, (float)b[a]);
         }
         st[ST::F_LIN * CW + q] = __int_as_float(ing ? (int)lin_index(g, b, D) : -1);
         if (MODE != 2) st[ST::F_M * CW + q] = in.mass[slot];
 #pragma unroll
         for (int a = 0; a < D; ++a)
-          st[(ST::F_V + a) * CW + q] = MODE == 0 ? in.v[a * n + slot] : (MODE == 1 ? out.v[a * n + jj] : rec[a * n + jj]);
+          st[(ST::F_V + a) * CW + q] = (MODE == 0 || MODE == 3) ? in.v[a * n + slot]
+                                                                : (MODE == 1 ? out.v[a * n + jj] : rec[a * n + jj]);
         if constexpr (MODE == 0) {
           const float mvol = -in.volume[slot];
           const Sym sg = load_sym(in.stress, n, slot);
@@ -741,25 +910,95 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
         }
       };
       // the next particle's inputs are read while the current one is summed
+      // (MODE 3 reads each in turn: its node records take the registers)
+      constexpr bool kPrefetch = MODE != 3;
       P2GRec cur, nxt;
       float fr[3] = {0.f, 0.f, 0.f}, fr_n[3] = {0.f, 0.f, 0.f};
       int lin = -1, lin_n = -1;
-      if (j0 < j1) read(j0 - w0, cur, fr, lin);
+      if (kPrefetch && j0 < j1) read(j0 - w0, cur, fr, lin);
       for (int j = j0; j < j1; ++j) {
-        if (j + 1 < j1) read(j + 1 - w0, nxt, fr_n, lin_n);
-        if (lin != cell) {
+        if (!kPrefetch) read(j - w0, cur, fr, lin);
+        else if (j + 1 < j1) read(j + 1 - w0, nxt, fr_n, lin_n);
+        if constexpr (MODE == 3) {
+          // first map of particle j (fr holds its position): the cell's lanes
+          // each gather their y slice of the window, the butterfly over the
+          // cell's lanes completes the sums, then the re-map deposits m v'
+          float vinc[3] = {0.f, 0.f, 0.f}, dvinc[3] = {0.f, 0.f, 0.f};
+          const bool own = lin == cell;
+          Wts W;
+          if (own) {
+            float frac[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int a = 0; a < D; ++a) frac[a] = __fsub_rn(cell_coord(fr[a], g.origin[a], inv_h), (float)c[a]);
+            weights(frac, W);
+            if (!none) {
+#pragma unroll
+              for (int sx = 0; sx < SPAN; ++sx)
+#pragma unroll
+                for (int sz = 0; sz < SZH; ++sz) {
+                  const float wxz = W.Nx[sx] * W.Nz[sz];
+#pragma unroll
+                  for (int a = 0; a < D; ++a) {
+                    vinc[a] += wxz * vnode[sx][sz][a];
+                    dvinc[a] += wxz * vnode[sx][sz][D + a];
+                  }
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+              vinc[a] *= W.Ny;
+              dvinc[a] *= W.Ny;
+            }
+#pragma unroll
+            for (int o = G::CX; o < 64; o <<= 1)
+#pragma unroll
+              for (int a = 0; a < D; ++a) {
+                vinc[a] += __shfl_xor(vinc[a], o);
+                dvinc[a] += __shfl_xor(dvinc[a], o);
+              }
+          } else if (sy == 0) {
+            // a particle listed in this cell whose bin is another (only when
+            // the partition was built from other positions): its own window
+            gather_one<D, SF>(prm, g, in.x, n, part.sorted[j], nodes, vinc, dvinc);
+          }
+          float vn[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+          for (int a = 0; a < D; ++a) {
+            if (!own) {  // the cell's sy = 0 lane gathered
+              vinc[a] = __shfl(vinc[a], lane % G::CX);
+              dvinc[a] = __shfl(dvinc[a], lane % G::CX);
+            }
+            vn[a] = am * vinc[a] + prm.alpha * (cur.v[a] + dvinc[a]);
+            if (a % G::SY == sy) {  // the cell's lanes share the stores
+              out.v[a * n + j] = vn[a];
+              out.x[a * n + j] = fr[a] + prm.dt * vinc[a];
+            }
+          }
+          P2GRec rv = cur;
+#pragma unroll
+          for (int a = 0; a < D; ++a) rv.v[a] = vn[a];
+          if (own) {
+            if (!none) deposit(rv, W);
+          } else if (zh == 0) {
+            p2g_stray<D, SF, 1>(prm, g, nodes, in.x, n, part.sorted[j], sy, rv.m, rv.v, rv.A);
+          }
+        } else if (lin != cell) {
           // a particle listed in this cell whose bin is another (only when the
           // partition was built from other positions); MODE 2 is only ever
           // called on a partition of the same positions
           if (MODE != 2 && zh == 0)
             p2g_stray<D, SF, MODE>(prm, g, nodes, in.x, n, part.sorted[j], sy, cur.m, cur.v, cur.A);
         } else {
-          deposit(cur, fr);
+          Wts W;
+          weights(fr, W);
+          deposit(cur, W);
         }
-        cur = nxt;
-        lin = lin_n;
+        if (kPrefetch) {
+          cur = nxt;
+          lin = lin_n;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) fr[a] = fr_n[a];
+          for (int a = 0; a < 3; ++a) fr[a] = fr_n[a];
+        }
       }
       wave_lds_sync();
     }
@@ -1059,11 +1298,9 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
       for (int a = 0; a < D; ++a) nodes.mom[a * nn + i] = 0.f;
     }
 #pragma unroll
-    for (int a = 0; a < D; ++a) {
-      nodes.mom_nt[a * nn + i] = 0.f;
-      nodes.vel_nt[a * nn + i] = 0.f;
-      nodes.dvel[a * nn + i] = 0.f;
-    }
+    for (int a = 0; a < D; ++a) nodes.mom_nt[a * nn + i] = 0.f;
+    const float z[3] = {0.f, 0.f, 0.f};
+    store_node_vel<D>(nodes.vel, i, z, z);
     return;
   }
 #pragma unroll
@@ -1171,13 +1408,14 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
     nodes.mom_nt[a * nn + i] = Q[a];
   }
   const bool live = m > prm.small_mass_cutoff;
+  float vnt[3] = {0.f, 0.f, 0.f}, dv[3] = {0.f, 0.f, 0.f};
 #pragma unroll
   for (int a = 0; a < D; ++a) {
-    const float vnt = live ? Q[a] / m : 0.f;
+    vnt[a] = live ? Q[a] / m : 0.f;
     const float v = live ? P[a] / m : 0.f;
-    nodes.vel_nt[a * nn + i] = vnt;
-    nodes.dvel[a * nn + i] = vnt - v;
+    dv[a] = vnt[a] - v;
   }
+  store_node_vel<D>(nodes.vel, i, vnt, dv);
 }
 
 // MODE 0: USL G2P (usl_inline.h:245-332) + return map.
@@ -1205,7 +1443,7 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(pmpm_params prm, pmpm_
   particle_weights<D, SF>(prm, g, in.x, n, src, w, xp);
   float vinc[3] = {0.f, 0.f, 0.f}, dvinc[3] = {0.f, 0.f, 0.f};
   float L[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  const NodeSrc ns{nodes.vel_nt, nodes.dvel, nn, {0, 0, 0}, {1, g.num_cells[0], g.num_cells[0] * g.num_cells[1]}};
+  const NodeSrc ns = node_src(g, nodes);
   if (MODE == 0) gather_grid<D, SF, true, true>(g, ns, w, vinc, dvinc, L);
   else if (MODE == 1) gather_grid<D, SF, true, false>(g, ns, w, vinc, dvinc, L);
   else gather_grid<D, SF, false, true>(g, ns, w, vinc, dvinc, L);
@@ -1387,7 +1625,8 @@ TileGeom tile_geom(int dim, int sf, const pmpm_grid &g, int gather) {
 }
 static size_t tile_count(const TileGeom &tg) { return (size_t)tg.tiles[0] * tg.tiles[1] * tg.tiles[2]; }
 
-// mode 0: P2G, 1: MUSL re-map, 2: adjoint scatter of the records `rec`
+// mode 0: P2G, 1: MUSL re-map, 2: adjoint scatter of the records `rec`,
+// 3: MUSL first map + re-map
 template <int D, int SF>
 void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
                 const pmpm_spatial &part, const pmpm_nodes &nodes, int mode, float *partials, int32_t *flags,
@@ -1400,6 +1639,9 @@ void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles
   else if (mode == 1)
     hipLaunchKernelGGL((p2g_kernel<D, SF, 1>), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
                        tg.tiles[1], partials, flags, rec);
+  else if (mode == 3)
+    hipLaunchKernelGGL((p2g_kernel<D, SF, 3>), dim3((unsigned)tile_count(tg) + kMuslTailBlocks), block, 0, s, prm, g,
+                       in, out, part, nodes, tg.tiles[0], tg.tiles[1], partials, flags, rec);
   else
     hipLaunchKernelGGL((p2g_kernel<D, SF, 2>), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
                        tg.tiles[1], partials, flags, rec);
@@ -1503,12 +1745,10 @@ int do_musl_map(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, c
                 const pmpm_particles &out, const pmpm_spatial &part, const pmpm_nodes &nodes, hipStream_t s) {
   int rc = reserve_tiles(ctx, prm, g);
   if (rc != PMPM_OK) return rc;
-  if (in.n > 0) {
-    PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_g2p, prm, g, in, out, part, nodes, 1, 1, s);
-    PMPM_LAUNCH_CHECK();
-  }
+  // the fused pass reads the node velocity records, never mom_nt: clear it
+  // for the re-map's wrap-around atomics first
   if ((rc = zero_fill(nodes.mom_nt, (int64_t)g.num_cells_total * prm.dim, s)) != PMPM_OK) return rc;
-  PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, in, out, part, nodes, 1, ctx->partials,
+  PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, in, out, part, nodes, 3, ctx->partials,
                     ctx->tile_flags, s);
   PMPM_LAUNCH_CHECK();
   return PMPM_OK;
@@ -1687,12 +1927,10 @@ __device__ inline void g2p_weights_adjoint(const pmpm_params &prm, const pmpm_gr
                     h3[3] = {d2N[0][ex], d2N[1][ey], d2N[2][ez]};
         float wgt, gr[3], H[6];
         entry_derivs(n3, d3, h3, wgt, gr, H);
-        float vnt[3] = {0.f, 0.f, 0.f}, s = 0.f, u[3] = {0.f, 0.f, 0.f};
+        float vnt[3] = {0.f, 0.f, 0.f}, dvn[3] = {0.f, 0.f, 0.f}, s = 0.f, u[3] = {0.f, 0.f, 0.f};
+        load_node_vel<D, true>(nodes.vel, node, vnt, dvn);
 #pragma unroll
-        for (int b = 0; b < D; ++b) {
-          vnt[b] = nodes.vel_nt[b * nn + node];
-          s += vnt[b] * vpb[b] + nodes.dvel[b * nn + node] * dvb[b];
-        }
+        for (int b = 0; b < D; ++b) s += vnt[b] * vpb[b] + dvn[b] * dvb[b];
 #pragma unroll
         for (int a = 0; a < D; ++a)
 #pragma unroll
@@ -1732,7 +1970,7 @@ __global__ __launch_bounds__(256) void g2p_adjoint_kernel(pmpm_params prm, pmpm_
     PW<D, SF> w;
     float xp[3];
     particle_weights<D, SF>(prm, g, st.x, n, src, w, xp);
-    const NodeSrc ns{nodes.vel_nt, nodes.dvel, nn, {0, 0, 0}, {1, g.num_cells[0], g.num_cells[0] * g.num_cells[1]}};
+    const NodeSrc ns = node_src(g, nodes);
     float vpic[3] = {0.f, 0.f, 0.f}, dvp[3] = {0.f, 0.f, 0.f};
     float L[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     gather_grid<D, SF, true, true>(g, ns, w, vpic, dvp, L);
@@ -1863,6 +2101,8 @@ __global__ void grid_adjoint_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nod
   float mb = 0.f, Pb[3] = {0.f, 0.f, 0.f}, Qb[3] = {0.f, 0.f, 0.f}, fb[3] = {0.f, 0.f, 0.f};
   if (m > prm.small_mass_cutoff) {
     float P[3] = {0.f, 0.f, 0.f}, Q[3] = {0.f, 0.f, 0.f};
+    float vnt[3] = {0.f, 0.f, 0.f}, dvn[3] = {0.f, 0.f, 0.f};
+    load_node_vel<D, true>(nodes.vel, i, vnt, dvn);
 #pragma unroll
     for (int a = 0; a < D; ++a) {
       P[a] = nodes.mom[a * nn + i];
@@ -1870,7 +2110,7 @@ __global__ void grid_adjoint_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nod
       const float vb = acc[a], db = acc[D + a];
       Qb[a] = (vb + db) / m;
       Pb[a] = -db / m;
-      mb -= (vb * nodes.vel_nt[a * nn + i] + db * nodes.dvel[a * nn + i]) / m;
+      mb -= (vb * vnt[a] + db * dvn[a]) / m;
     }
     for (int k = prm.num_bcs - 1; k >= 0; --k) {
       const pmpm_bc &bc = prm.bcs[k];

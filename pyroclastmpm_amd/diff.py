@@ -122,9 +122,9 @@ class DiffMPM:
         self.spatial = capi.Spatial(*[self.sp[k].data_ptr() for k in ["hash", "hash_sorted", "sorted",
                                                                       "cell_start", "cell_end"]], None)
         f = lambda m: torch.zeros(m, dtype=torch.float32, device=device)
-        self.nd = {"mass": f(nn), "mom": f(dim * nn), "mom_nt": f(dim * nn), "vel_nt": f(dim * nn),
-                   "dvel": f(dim * nn)}
-        self.nodes = capi.Nodes(*[self.nd[k].data_ptr() for k in ["mass", "mom", "mom_nt", "vel_nt", "dvel"]])
+        self.nd = {"mass": f(nn), "mom": f(dim * nn), "mom_nt": f(dim * nn),
+                   "vel": f(capi.node_vel_stride(dim) * nn)}
+        self.nodes = capi.Nodes(*[self.nd[k].data_ptr() for k in ["mass", "mom", "mom_nt", "vel"]])
         self.ctx = capi.Context(n, nn)
         self.device = device
 
