@@ -62,6 +62,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward time ratio")
     ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--no-batched", action="store_true", help="skip the multi-scene sweep")
+    ap.add_argument("--batched-scenes", type=int, default=64)
+    ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="before the workload, run 3 plain device copies of MIB MiB (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -237,8 +240,46 @@ def adjoint_ratio(steps, every=20):
     return {"metric": "adjoint/forward time ratio", "value": round(bwd / fwd, 3), "unit": "ratio",
             "higher_is_better": False, "forward_ms_per_step": round(fwd / steps, 4),
             "backward_ms_per_step": round(bwd / steps, 4), "steps": steps, "checkpoint_every": every,
-            "particles": model.n, "loss": float(loss), "grad": [float(v) for v in theta.grad],
+            "particles": model.n, "loss": float(loss.detach()), "grad": [float(v) for v in theta.grad],
             "config": "3D Drucker-Prager column, h 0.02, 8 per cell, USL, cubic, dt 1e-4"}
+
+
+def batched_throughput(count, steps):
+    """The multi-scene sweep: `count` Drucker-Prager columns (MUSL, cubic,
+    dt 1e-5) with particle counts log-uniform in [16k, 1M], h uniform in
+    [0.01, 0.02] m, friction angle in [20, 40] deg, E in [5e5, 2e6] Pa (seed 1),
+    stepped together by pmpm_batch_step after the library's autotuner picked
+    the stream count and eager or graph replay. Also times the same scenes one
+    after another on one stream (lanes 1, eager) for comparison."""
+    import torch
+    from pyroclastmpm_amd.batch import Batch, column_batch
+
+    scenes = column_batch(count, seed=1)
+    b = Batch(scenes, lanes=4)
+    b.step(2)  # sizes every workspace
+    tuning = b.autotune(steps=2)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def timed(batch):
+        torch.cuda.synchronize()
+        ev[0].record()
+        batch.step(steps)
+        ev[1].record()
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) / steps
+
+    ms = timed(b)
+    serial = Batch(scenes, lanes=1)
+    serial.cur = b.cur
+    ms_serial = timed(serial)
+    n = b.particles
+    sizes = sorted(s.n for s in scenes)
+    return {"metric": "particle-updates/s, multi-scene column sweep", "value": round(n / (ms * 1e-3), 1),
+            "unit": "particle-updates/s", "higher_is_better": True, "scenes": count, "particles": n,
+            "smallest": sizes[0], "largest": sizes[-1], "steps": steps, "ms_per_step": round(ms, 4),
+            "tuning": tuning, "one_stream_eager_ms_per_step": round(ms_serial, 4),
+            "speedup_vs_one_stream": round(ms_serial / ms, 3),
+            "config": "3D Drucker-Prager columns, MUSL, cubic, 8 per cell, dt 1e-5, seed 1"}
 
 
 def main():
@@ -321,6 +362,12 @@ def main():
             print("bench: adjoint " + json.dumps(line["adjoint"]), file=sys.stderr, flush=True)
         except Exception as e:  # reported, the throughput line stands on its own
             line["adjoint"] = {"value": None, "error": str(e)}
+    if not args.no_batched:
+        try:
+            line["batched"] = batched_throughput(args.batched_scenes, args.batched_steps)
+            print("bench: batched " + json.dumps(line["batched"]), file=sys.stderr, flush=True)
+        except Exception as e:  # reported, the throughput line stands on its own
+            line["batched"] = {"value": None, "error": str(e)}
     print(json.dumps(line), flush=True)
 
 

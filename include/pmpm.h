@@ -278,6 +278,79 @@ int pmpm_adjoint_step(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid
                       const pmpm_particles *state, const pmpm_adjoint *out_bar, pmpm_adjoint *in_bar,
                       float *param_bar, pmpm_spatial *part, pmpm_nodes *nodes, pmpm_stream_t stream);
 
+/* ---- batched scenes (no reference counterpart: the reference runs one
+ * scene per process through its globals, global_settings.cpp; this is the
+ * multi-scene form of the step pmpm_step runs, for parameter sweeps and
+ * optimisation over many independent columns).
+ *
+ * A scene is everything one pmpm_step call takes, with its own context and
+ * both particle buffer sets: a step reads set[cur] and writes set[cur ^ 1]. */
+typedef struct pmpm_scene {
+  pmpm_context *ctx;
+  pmpm_params prm;
+  pmpm_grid grid;
+  pmpm_particles set[2];
+  pmpm_spatial part;
+  pmpm_nodes nodes;
+  int32_t musl;
+} pmpm_scene;
+
+#define PMPM_MAX_LANES 8
+
+typedef struct pmpm_batch pmpm_batch;
+
+/* How a batch runs: `lanes` streams forked from the caller's stream (scenes
+ * placed on them by pmpm_batch_assign with `weight` per scene), eager
+ * launches or a replayed HIP graph. */
+typedef struct pmpm_batch_tuning {
+  int32_t lanes;     /* 1 .. PMPM_MAX_LANES */
+  int32_t use_graph; /* 0: enqueue every step; 1: capture once per parity, replay */
+  float step_ms;     /* measured time per batched step of the chosen setting */
+} pmpm_batch_tuning;
+
+/* A batch on the current device with `lanes` streams (default priority). */
+int pmpm_batch_create(pmpm_batch **batch, int32_t lanes);
+int pmpm_batch_destroy(pmpm_batch *batch);
+
+/* Longest-processing-time split (host only): scenes in decreasing weight,
+ * each onto the lane with the least total so far (ties: lowest lane). Writes
+ * lane_of[count]; returns PMPM_OK. */
+int pmpm_batch_assign(const float *weight, int32_t count, int32_t lanes, int32_t *lane_of);
+
+/* Per-scene weights used by pmpm_batch_step (NULL: particle counts). */
+int pmpm_batch_set_weights(pmpm_batch *batch, const float *weight, int32_t count);
+
+/* One step of every scene (reads set[cur], writes set[cur ^ 1]); the lanes
+ * are forked from and joined back into `stream` with events, so the call is
+ * ordered like a single launch on `stream`. Each scene's workspace must
+ * already be sized (one eager step does it) before a graph is captured. */
+int pmpm_batch_step(pmpm_batch *batch, const pmpm_scene *scenes, int32_t count, int32_t cur,
+                    pmpm_stream_t stream);
+
+/* Captures the step of parity `cur` into the batch's graph for that parity
+ * (replacing any earlier one). The scenes' buffers must stay where they are
+ * for as long as the graph is replayed. */
+int pmpm_batch_capture(pmpm_batch *batch, const pmpm_scene *scenes, int32_t count, int32_t cur,
+                       pmpm_stream_t stream);
+/* Replays the captured step of parity `cur` on `stream`. */
+int pmpm_batch_replay(pmpm_batch *batch, int32_t cur, pmpm_stream_t stream);
+
+/* Autotuner: times each scene's step alone (these become the LPT weights),
+ * then `steps` batched steps for every lane count up to the batch's streams,
+ * eager and graph-replayed, and keeps the fastest setting in the batch
+ * (writes it to *out if non-NULL). No graph is kept: when the setting
+ * replays graphs, call pmpm_batch_capture for both parities. It advances
+ * every scene by the steps it runs (two alone, then `steps` per candidate,
+ * rounded up to even) so the buffer parity `cur` is the same on return. */
+int pmpm_batch_autotune(pmpm_batch *batch, const pmpm_scene *scenes, int32_t count, int32_t cur, int32_t steps,
+                        pmpm_stream_t stream, pmpm_batch_tuning *out);
+/* The batch's current setting. */
+int pmpm_batch_get_tuning(const pmpm_batch *batch, pmpm_batch_tuning *out);
+
+/* The last HIP failure an entry point met on the calling thread, as
+ * "call: hipGetErrorString (code)", or "" if none (thread-local storage). */
+const char *pmpm_last_error(void);
+
 /* Build identification string (static storage). */
 const char *pmpm_version(void);
 

@@ -13,6 +13,15 @@ import os as _os
 
 _HERE = _os.path.dirname(_os.path.abspath(__file__))
 
+# One HIP runtime per process: torch ships its own libamdhip64 and the
+# extension links the system one under the same soname. Loaded in this order
+# the extension binds to torch's copy; loaded the other way round the process
+# holds two runtimes and the one initialised second finds no device.
+try:
+    import torch as _torch  # noqa: F401
+except ImportError:  # the engine itself does not need torch
+    _torch = None
+
 try:
     from . import _core  # noqa: F401
 except ImportError as e:  # pragma: no cover - build problem, not a runtime branch

@@ -71,6 +71,18 @@ class Params(C.Structure):
                 ("num_bcs", C.c_int32), ("bcs", BC * MAX_BCS)]
 
 
+MAX_LANES = 8
+
+
+class Scene(C.Structure):
+    _fields_ = [("ctx", vp), ("prm", Params), ("grid", Grid), ("set", Particles * 2), ("part", Spatial),
+                ("nodes", Nodes), ("musl", C.c_int32)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("lanes", C.c_int32), ("use_graph", C.c_int32), ("step_ms", C.c_float)]
+
+
 P = C.POINTER
 # name -> (restype, argtypes); every function declared in include/pmpm.h
 PROTOTYPES = {
@@ -94,6 +106,16 @@ PROTOTYPES = {
                             vp]),
     "pmpm_adjoint_step": (C.c_int, [vp, P(Params), P(Grid), P(Particles), P(Adjoint), P(Adjoint), vp, P(Spatial),
                                     P(Nodes), vp]),
+    "pmpm_batch_create": (C.c_int, [P(vp), C.c_int32]),
+    "pmpm_batch_destroy": (C.c_int, [vp]),
+    "pmpm_batch_assign": (C.c_int, [P(C.c_float), C.c_int32, C.c_int32, P(C.c_int32)]),
+    "pmpm_batch_set_weights": (C.c_int, [vp, P(C.c_float), C.c_int32]),
+    "pmpm_batch_step": (C.c_int, [vp, P(Scene), C.c_int32, C.c_int32, vp]),
+    "pmpm_batch_capture": (C.c_int, [vp, P(Scene), C.c_int32, C.c_int32, vp]),
+    "pmpm_batch_replay": (C.c_int, [vp, C.c_int32, vp]),
+    "pmpm_batch_autotune": (C.c_int, [vp, P(Scene), C.c_int32, C.c_int32, C.c_int32, vp, P(BatchTuning)]),
+    "pmpm_batch_get_tuning": (C.c_int, [vp, P(BatchTuning)]),
+    "pmpm_last_error": (C.c_char_p, []),
     "pmpm_version": (C.c_char_p, []),
 }
 
@@ -117,7 +139,10 @@ def lib():
 
 def check(rc, what="pmpm call"):
     if rc != PMPM_OK:
-        raise RuntimeError(f"{what} failed with code {rc}")
+        detail = ""
+        if rc == PMPM_ERR_HIP:
+            detail = " (" + lib().pmpm_last_error().decode(errors="replace") + ")"
+        raise RuntimeError(f"{what} failed with code {rc}{detail}")
 
 
 def make_grid(origin, end, cell_size, dim):

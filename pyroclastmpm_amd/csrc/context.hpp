@@ -48,16 +48,30 @@ constexpr int kMaxRadix = 1 << kMaxRadixBits;
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Records the failing HIP call of the calling thread (pmpm_last_error).
+void note_hip_error(hipError_t e, const char *what);
+inline bool hip_ok(hipError_t e, const char *what) {
+  if (e == hipSuccess) return true;
+  note_hip_error(e, what);
+  return false;
+}
+
 #define PMPM_HIP_CHECK(expr)                              \
   do {                                                    \
     hipError_t e_ = (expr);                               \
-    if (e_ != hipSuccess) return PMPM_ERR_HIP;            \
+    if (e_ != hipSuccess) {                               \
+      ::pmpm::note_hip_error(e_, #expr);                  \
+      return PMPM_ERR_HIP;                                \
+    }                                                     \
   } while (0)
 
 #define PMPM_LAUNCH_CHECK()                               \
   do {                                                    \
     hipError_t e_ = hipGetLastError();                    \
-    if (e_ != hipSuccess) return PMPM_ERR_HIP;            \
+    if (e_ != hipSuccess) {                               \
+      ::pmpm::note_hip_error(e_, "kernel launch");        \
+      return PMPM_ERR_HIP;                                \
+    }                                                     \
   } while (0)
 
 // Fill n floats/ints with zero (a kernel, not hipMemsetAsync, so the step can

@@ -4,11 +4,17 @@
 // (spatialpartition_inline.h:49-115, spatialpartition.cpp:77-136).
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <new>
 
 #include "context.hpp"
 
 namespace pmpm {
+
+thread_local char g_last_error[256] = "";
+void note_hip_error(hipError_t e, const char *what) {
+  snprintf(g_last_error, sizeof(g_last_error), "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+}
 
 __global__ void zero_kernel(uint32_t *p, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -211,13 +217,19 @@ __global__ void cell_range_kernel(const uint32_t *__restrict__ hs, int n, uint32
 
 using namespace pmpm;
 
+extern "C" const char *pmpm_last_error(void) { return g_last_error; }
+
 extern "C" int pmpm_context_create(pmpm_context **ctx) {
   if (!ctx) return PMPM_ERR_INVALID;
   pmpm_context *c = new (std::nothrow) pmpm_context();
   if (!c) return PMPM_ERR_INVALID;
-  if (hipGetDevice(&c->device) != hipSuccess) { delete c; return PMPM_ERR_HIP; }
-  if (hipMalloc(&c->counters, 4 * sizeof(uint32_t)) != hipSuccess) { delete c; return PMPM_ERR_HIP; }
-  if (hipMemset(c->counters, 0, 4 * sizeof(uint32_t)) != hipSuccess) { (void)hipFree(c->counters); delete c; return PMPM_ERR_HIP; }
+  if (!hip_ok(hipGetDevice(&c->device), "hipGetDevice")) { delete c; return PMPM_ERR_HIP; }
+  if (!hip_ok(hipMalloc(&c->counters, 4 * sizeof(uint32_t)), "hipMalloc")) { delete c; return PMPM_ERR_HIP; }
+  if (!hip_ok(hipMemset(c->counters, 0, 4 * sizeof(uint32_t)), "hipMemset")) {
+    (void)hipFree(c->counters);
+    delete c;
+    return PMPM_ERR_HIP;
+  }
   *ctx = c;
   return PMPM_OK;
 }
