@@ -6,14 +6,14 @@
 //                     force for its nodes in registers, then adds them into an
 //                     LDS copy of the tile's nodes in conflict-free phases and
 //                     the tile is stored to a per-tile slice. MODE 1 is the
-//                     MUSL re-map of momentum.
+//                     MUSL first map fused with the re-map of momentum.
 //   grid_kernel       per node: sums the tile slices covering it, mass cutoff,
 //                     mom_nt = mom + f dt, boundary conditions in list order,
 //                     node velocities for G2P.
 //   g2p_kernel        per particle (cell order): gather from the grid,
 //                     FLIP/PIC update, F and volume, then the return map of the
-//                     particle's material, written in cell order. MODE 1 is the
-//                     MUSL first map (x, v only).
+//                     particle's material, written in cell order. MODE 2 is the
+//                     MUSL second map.
 // Weights are recomputed in registers from the positions; nothing per
 // particle x node is stored.
 #include <hip/hip_runtime.h>
@@ -331,7 +331,9 @@ __device__ inline void span_evals(float frac, const int *type, float inv_h, floa
 }
 
 // Per-particle inputs of a P2G deposit.
-//   MODE 0/1: mass, velocity, -V sigma.
+//   MODE 0: mass, velocity, -V sigma.
+//   MODE 1 (the re-map's wrap-around deposits): mass and v', read from out.v
+//   in cell order after the fused first map wrote it.
 //   MODE 2 (adjoint scatter): v = adjoint of the PIC velocity, dv = adjoint of
 //   the FLIP increment, Lb = adjoint of the velocity gradient (row-major 3x3),
 //   read from the record array `rec` in cell order.
@@ -421,13 +423,14 @@ __device__ inline void deposit_wrapped(const pmpm_grid &g, const pmpm_nodes &nod
 
 // Per-particle P2G inputs staged in LDS, one float (or int bits) per field,
 // field-major within a wave's region:
-//   frac[D]  position in the cell (t - bin, as the weights use it)
+//   frac[D]  position in the cell (t - bin, as the weights use it); MODE 1
+//            keeps the position itself (its first map moves it)
 //   lin      linear index of the particle's own bin (-1 outside the grid)
 //   m        mass (MODE 0/1)
-//   v[D]     velocity (MODE 0), v' (MODE 1), adjoint of the PIC velocity (MODE 2)
+//   v[D]     velocity (MODE 0 and 1), adjoint of the PIC velocity (MODE 2)
 //   A        -V sigma, the D(D+1)/2 components used in D dimensions (MODE 0)
 //   dv[D], Lb[D*D]  adjoints of the FLIP increment and of L (MODE 2)
-// Fields per node of a P2G pass: MODE 0 mass, momentum, force; MODE 1 and 3
+// Fields per node of a P2G pass: MODE 0 mass, momentum, force; MODE 1 the
 // re-mapped momentum; MODE 2 the adjoints of v_nt and dv.
 template <int D, int MODE> constexpr int p2g_nv() { return MODE == 0 ? 1 + 2 * D : (MODE == 2 ? 2 * D : D); }
 
@@ -442,7 +445,7 @@ template <int D, int MODE> struct P2GStage {
 // allow 2 waves per SIMD at most), MODE 1 for two.
 template <int D, int SF, int MODE> constexpr int p2g_stage_cap() {
   constexpr int NV = p2g_nv<D, MODE>();
-  constexpr int budget = (MODE == 1 || MODE == 3) ? 18944 : 40448;  // floats per workgroup
+  constexpr int budget = MODE == 1 ? 18944 : 40448;  // floats per workgroup
   constexpr int room = (budget - NV * P2GGeom<D, SF>::NT) / (P2GGeom<D, SF>::WAVES * P2GStage<D, MODE>::NF);
   constexpr int cap = (room < 512 ? room : 512) / 64 * 64;
   static_assert(cap >= 64, "LDS budget too small for the staging area");
@@ -509,7 +512,7 @@ __device__ inline void gather_one(const pmpm_params &prm, const pmpm_grid &g, co
   }
 }
 
-// Workgroups after the tiles of a MODE 3 launch: they run the first map of
+// Workgroups after the tiles of a MODE 1 launch: they run the first map of
 // the particles outside the grid, which sort last and belong to no cell.
 constexpr int kMuslTailBlocks = 4;
 template <int D, int SF>
@@ -543,15 +546,13 @@ __device__ void musl_first_map_tail(const pmpm_params &prm, const pmpm_grid &g, 
 
 // P2G of one tile of cells.
 //   MODE 0: mass, momentum and force (force accumulates in mom_nt).
-//   MODE 1: MUSL re-map of m v' into mom_nt; v' is read from out.v in cell
-//           order (written by the first map), weights from the positions in.x.
-//   MODE 2: adjoint scatter of the G2P record adjoints.
-//   MODE 3: MUSL first map and re-map in one pass: each particle's v', x'
+//   MODE 1: MUSL first map and re-map in one pass: each particle's v', x'
 //           from the node velocity records (written to slot j of `out`), then
 //           m v' deposited with the same weights (the reference's
 //           KERNEL_MUSL_G2P_DOUBLE_MAPPING and KERNEL_MUSL_P2G_DOUBLE_MAPPING,
 //           musl_kernels.cuh:33-60). Particles outside the grid are mapped by
 //           the kMuslTailBlocks workgroups after the tiles.
+//   MODE 2: adjoint scatter of the G2P record adjoints.
 // Lane (cell c, sy) walks the particles of c in sorted order and sums in
 // registers their contributions to its SPAN x SZ nodes c + NLO + (sx, sy, sz).
 // The sums then enter an LDS node tile in SPAN x SZ phases: within one phase
@@ -571,11 +572,11 @@ __device__ void musl_first_map_tail(const pmpm_params &prm, const pmpm_grid &g, 
 // deposit lands on the wrapped node, weighted with that node's cubic types.
 // Those sums go to the node arrays with one global atomic per cell, node
 // offset and field (wrap_quirk), or are dropped.
-// Which P2G passes stage their particles in LDS: the MUSL re-map (8 floats
-// per particle, two workgroups per CU). The full P2G and the adjoint scatter
+// Which P2G passes stage their particles in LDS: the MUSL map (8 floats per
+// particle, two workgroups per CU). The full P2G and the adjoint scatter
 // (14 and 20 floats) would fit one workgroup per CU only, and measured slower
 // that way than reading through L2 at two waves per SIMD.
-template <int MODE> constexpr bool p2g_staged() { return MODE == 1 || MODE == 3; }
+template <int MODE> constexpr bool p2g_staged() { return MODE == 1; }
 template <int D, int SF, int MODE> constexpr int p2g_lds_floats() {
   constexpr int NV = p2g_nv<D, MODE>();
   return NV * P2GGeom<D, SF>::NT +
@@ -619,7 +620,7 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
   const int bz = blockIdx.x / (tiles_x * tiles_y);
   const int c[3] = {bx * G::TX + lx, D > 1 ? by : 0, D > 2 ? bz * G::TZ + lz : 0};
 
-  if constexpr (MODE == 3) {
+  if constexpr (MODE == 1) {
     if ((int)blockIdx.x >= (int)gridDim.x - kMuslTailBlocks) {
       musl_first_map_tail<D, SF>(prm, g, in, out, part, nodes, (int)blockIdx.x - ((int)gridDim.x - kMuslTailBlocks));
       return;
@@ -638,9 +639,9 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
   }
   // cubic weights without node types are zero (cubic.cuh: type 0): nothing
   // to deposit
-  // (MODE 3 still moves the particles: its first map gathers zeros)
+  // (MODE 1 still moves the particles: its first map gathers zeros)
   const bool none = SF == PMPM_SF_CUBIC && !prm.node_types_built;
-  if (!__syncthreads_or(ce > cs) || (none && MODE != 3)) {
+  if (!__syncthreads_or(ce > cs) || (none && MODE != 1)) {
     if (t == 0) flags[blockIdx.x] = 0;
     return;
   }
@@ -681,12 +682,12 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
     for (int k = 0; k < 4; ++k) yc[k] = kCubic[ty][3 - sy][k];
   }
 
-  // MODE 3: the velocity records of this lane's SPAN x SZH nodes, loaded once
+  // MODE 1: the velocity records of this lane's SPAN x SZH nodes, loaded once
   // for all the cell's particles (zero outside the grid, as the gather skips
   // such entries)
-  constexpr int NVN = MODE == 3 ? 2 * D : 1;
+  constexpr int NVN = MODE == 1 ? 2 * D : 1;
   float vnode[SPAN][SZH][NVN];
-  if constexpr (MODE == 3) {
+  if constexpr (MODE == 1) {
 #pragma unroll
     for (int sx = 0; sx < SPAN; ++sx)
 #pragma unroll
@@ -825,7 +826,7 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
       cur = nxt;
     }
   } else {
-    // this wave's run [run_lo, run_hi) of the sorted order (MODE 1)
+    // this wave's run [run_lo, run_hi) of the sorted order
     const int run_lo = wave_min(ce > cs ? cs : 0x7fffffff);
     const int run_hi = wave_max(ce > cs ? ce : 0);
     float *st = stage + wave * (NF * CW);
@@ -853,15 +854,14 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
           const float ta = cell_coord(in.x[a * n + slot], g.origin[a], inv_h);
           b[a] = (int)ta;
           ing = ing && b[a] >= 0 && b[a] < g.num_cells[a];
-          // (MODE 3 keeps the position itself: the first map moves it)
-          st[a * CW + q] = MODE == 3 ? in.x[a * n + slot] : __fsub_rn(ta, (float)b[a]);
+          // (MODE 1 keeps the position itself: the first map moves it)
+          st[a * CW + q] = MODE == 1 ? in.x[a * n + slot] : __fsub_rn(ta, (float)b[a]);
         }
         st[ST::F_LIN * CW + q] = __int_as_float(ing ? (int)lin_index(g, b, D) : -1);
         if (MODE != 2) st[ST::F_M * CW + q] = in.mass[slot];
 #pragma unroll
         for (int a = 0; a < D; ++a)
-          st[(ST::F_V + a) * CW + q] = (MODE == 0 || MODE == 3) ? in.v[a * n + slot]
-                                                                : (MODE == 1 ? out.v[a * n + jj] : rec[a * n + jj]);
+          st[(ST::F_V + a) * CW + q] = MODE != 2 ? in.v[a * n + slot] : rec[a * n + jj];
         if constexpr (MODE == 0) {
           const float mvol = -in.volume[slot];
           const Sym sg = load_sym(in.stress, n, slot);
@@ -910,8 +910,8 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
         }
       };
       // the next particle's inputs are read while the current one is summed
-      // (MODE 3 reads each in turn: its node records take the registers)
-      constexpr bool kPrefetch = MODE != 3;
+      // (MODE 1 reads each in turn: its node records take the registers)
+      constexpr bool kPrefetch = MODE != 1;
       P2GRec cur, nxt;
       float fr[3] = {0.f, 0.f, 0.f}, fr_n[3] = {0.f, 0.f, 0.f};
       int lin = -1, lin_n = -1;
@@ -919,7 +919,7 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
       for (int j = j0; j < j1; ++j) {
         if (!kPrefetch) read(j - w0, cur, fr, lin);
         else if (j + 1 < j1) read(j + 1 - w0, nxt, fr_n, lin_n);
-        if constexpr (MODE == 3) {
+        if constexpr (MODE == 1) {
           // first map of particle j (fr holds its position): the cell's lanes
           // each gather their y slice of the window, the butterfly over the
           // cell's lanes completes the sums, then the re-map deposits m v'
@@ -1419,10 +1419,9 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
 }
 
 // MODE 0: USL G2P (usl_inline.h:245-332) + return map.
-// MODE 1: MUSL first map: x, v of particle j updated from the grid and
-//         written to slot j of `out` (cell order); nothing else.
 // MODE 2: MUSL second map: L, F, V from the re-mapped velocities + return map
-//         (x, v of slot j of `out` were written by the first map).
+//         (x, v of slot j of `out` were written by the first map, which
+//         runs fused with the re-map: p2g_kernel MODE 1).
 // The window reads go through L2: the node velocities (12-24 B per node)
 // of a 1M-particle grid fit in it, and staging a block's node box in LDS
 // measured slower on gfx950 (fewer resident waves for the same reads).
@@ -1437,24 +1436,23 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(pmpm_params prm, pmpm_
   if (j >= n) return;
   const int nn = g.num_cells_total;
   const int src = part.sorted[j];
-  const int dst = (reorder || MODE == 1) ? j : src;
+  const int dst = reorder ? j : src;
   PW<D, SF> w;
   float xp[3];
   particle_weights<D, SF>(prm, g, in.x, n, src, w, xp);
   float vinc[3] = {0.f, 0.f, 0.f}, dvinc[3] = {0.f, 0.f, 0.f};
   float L[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   const NodeSrc ns = node_src(g, nodes);
+  static_assert(MODE == 0 || MODE == 2, "G2P modes: USL (0), MUSL second map (2)");
   if (MODE == 0) gather_grid<D, SF, true, true>(g, ns, w, vinc, dvinc, L);
-  else if (MODE == 1) gather_grid<D, SF, true, false>(g, ns, w, vinc, dvinc, L);
   else gather_grid<D, SF, false, true>(g, ns, w, vinc, dvinc, L);
-  if (MODE == 0 || MODE == 1) {
+  if (MODE == 0) {
     const float am = (float)(1.0 - (double)prm.alpha);
 #pragma unroll
     for (int a = 0; a < D; ++a) {
       out.v[a * n + dst] = am * vinc[a] + prm.alpha * (in.v[a * n + src] + dvinc[a]);
       out.x[a * n + dst] = xp[a] + prm.dt * vinc[a];
     }
-    if (MODE == 1) return;
   }
   // F = (I + L dt) F, V = det(F) V0
   float Fo[9], Fn[9];
@@ -1625,8 +1623,8 @@ TileGeom tile_geom(int dim, int sf, const pmpm_grid &g, int gather) {
 }
 static size_t tile_count(const TileGeom &tg) { return (size_t)tg.tiles[0] * tg.tiles[1] * tg.tiles[2]; }
 
-// mode 0: P2G, 1: MUSL re-map, 2: adjoint scatter of the records `rec`,
-// 3: MUSL first map + re-map
+// mode 0: P2G, 1: MUSL first map + re-map, 2: adjoint scatter of the
+// records `rec`
 template <int D, int SF>
 void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
                 const pmpm_spatial &part, const pmpm_nodes &nodes, int mode, float *partials, int32_t *flags,
@@ -1637,10 +1635,7 @@ void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles
     hipLaunchKernelGGL((p2g_kernel<D, SF, 0>), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
                        tg.tiles[1], partials, flags, rec);
   else if (mode == 1)
-    hipLaunchKernelGGL((p2g_kernel<D, SF, 1>), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
-                       tg.tiles[1], partials, flags, rec);
-  else if (mode == 3)
-    hipLaunchKernelGGL((p2g_kernel<D, SF, 3>), dim3((unsigned)tile_count(tg) + kMuslTailBlocks), block, 0, s, prm, g,
+    hipLaunchKernelGGL((p2g_kernel<D, SF, 1>), dim3((unsigned)tile_count(tg) + kMuslTailBlocks), block, 0, s, prm, g,
                        in, out, part, nodes, tg.tiles[0], tg.tiles[1], partials, flags, rec);
   else
     hipLaunchKernelGGL((p2g_kernel<D, SF, 2>), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
@@ -1659,8 +1654,6 @@ void launch_g2p(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles
   const unsigned blocks = (unsigned)ceil_div(in.n, 256);
   if (mode == 0)
     hipLaunchKernelGGL((g2p_kernel<D, SF, 0>), dim3(blocks), dim3(256), 0, s, prm, g, in, out, part, nodes, reorder);
-  else if (mode == 1)
-    hipLaunchKernelGGL((g2p_kernel<D, SF, 1>), dim3(blocks), dim3(256), 0, s, prm, g, in, out, part, nodes, reorder);
   else
     hipLaunchKernelGGL((g2p_kernel<D, SF, 2>), dim3(blocks), dim3(256), 0, s, prm, g, in, out, part, nodes, reorder);
 }
@@ -1748,7 +1741,7 @@ int do_musl_map(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, c
   // the fused pass reads the node velocity records, never mom_nt: clear it
   // for the re-map's wrap-around atomics first
   if ((rc = zero_fill(nodes.mom_nt, (int64_t)g.num_cells_total * prm.dim, s)) != PMPM_OK) return rc;
-  PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, in, out, part, nodes, 3, ctx->partials,
+  PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, in, out, part, nodes, 1, ctx->partials,
                     ctx->tile_flags, s);
   PMPM_LAUNCH_CHECK();
   return PMPM_OK;
