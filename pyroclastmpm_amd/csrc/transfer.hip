@@ -969,9 +969,9 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
               dvinc[a] = __shfl(dvinc[a], lane % G::CX);
             }
             vn[a] = am * vinc[a] + prm.alpha * (cur.v[a] + dvinc[a]);
-            if (a % G::SY == sy) {  // the cell's lanes share the stores
-              out.v[a * n + j] = vn[a];
-              out.x[a * n + j] = fr[a] + prm.dt * vinc[a];
+            if (a % G::SY == sy) {  // back into the particle's LDS slot (read above)
+              st[(ST::F_V + a) * CW + (j - w0)] = vn[a];
+              st[a * CW + (j - w0)] = fr[a] + prm.dt * vinc[a];
             }
           }
           P2GRec rv = cur;
@@ -1001,6 +1001,17 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
         }
       }
       wave_lds_sync();
+      if constexpr (MODE == 1) {
+        // the window's x', v' leave with coalesced stores (written from the
+        // cells' lanes they would land 16 lines per store, partly filled)
+        for (int q = lane; q < wn; q += 64)
+#pragma unroll
+          for (int a = 0; a < D; ++a) {
+            out.x[a * n + w0 + q] = st[a * CW + q];
+            out.v[a * n + w0 + q] = st[(ST::F_V + a) * CW + q];
+          }
+        wave_lds_sync();
+      }
     }
   }
 
