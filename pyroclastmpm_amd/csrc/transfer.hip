@@ -1059,6 +1059,200 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
   }
 }
 
+// P2G of one tile on the matrix cores (3D, cubic splines with node types):
+// the same tiles, slices and results as p2g_kernel MODE 0, with the per-cell
+// sums done by v_mfma_f32_16x16x4_f32. For one cell and its 4 x 4 x 4 node
+// window, with i, j, k the x, y, z node offsets and p the cell's particles,
+//   C1[(i, f)][(j, k)] = sum_p (Nx_i F_f)(p) (Ny_j Nz_k)(p),  F = (m, m v)
+//   C2[(i, a)][(j, k)] = sum_p (dNx_i A_ax)(Ny_j Nz_k) + (Nx_i A_ay)(dNy_j Nz_k)
+//                              + (Nx_i A_az)(Ny_j dNz_k),       A = -V sigma
+// which are the mass, momentum and force deposits (usl_inline.h:114-160) as
+// 16 x 16 products over K = particles, four particles per instruction. In the
+// instruction's layout lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]
+// and receives D[4 (l / 16) + r][l % 16]: it evaluates x and y weights at
+// offset (l % 16) / 4 and the z weight at offset l % 4 for particle l / 16,
+// and owns node (i = l / 16, j = (l % 16) / 4, k = l % 4) of the result.
+// Wave w walks the 16 cells of z layer w one after another: it stages up to
+// 64 of a cell's particles in LDS (lane = particle, coalesced loads; weights'
+// fractions and the products m v, -V sigma computed once per particle), runs
+// the products, and adds each lane's node into the wave's own LDS node block;
+// the four blocks overlap in z and are summed into the tile slice at the end.
+struct MfmaGeom {
+  static constexpr int CELLS = 16;                      // cells per wave (x)
+  static constexpr int BX = CELLS + 3, BY = 4, BZ = 4;  // a wave's node block
+  static constexpr int BN = BX * BY * BZ;
+  static constexpr int NF = 13;                         // staged floats per particle
+};
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <int SF>
+__global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles in,
+                                                       pmpm_spatial part, pmpm_nodes nodes, int tiles_x, int tiles_y,
+                                                       float *__restrict__ partials, int32_t *__restrict__ flags) {
+  using G = P2GGeom<3, SF>;
+  using M = MfmaGeom;
+  static_assert(G::TX == M::CELLS && G::TZ == 4 && G::NX == M::BX && G::NY == M::BY && G::SPAN == 4,
+                "the tile of p2g_kernel<3, SF, 0>");
+  constexpr int NV = 7, NTN = G::NTN, NLO = G::NLO;
+  __shared__ float block[4][NV][M::BN];
+  __shared__ float stage[4][M::NF][64];
+
+  const int n = in.n;
+  const float inv_h = g.inv_cell_size;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int bx = blockIdx.x % tiles_x;
+  const int by = (blockIdx.x / tiles_x) % tiles_y;
+  const int bz = blockIdx.x / (tiles_x * tiles_y);
+  const int cy = by, cz = bz * 4 + wave;  // this wave's cell row
+  // lanes 0..15 hold the ranges of the wave's 16 cells
+  int cs = 0, ce = 0;
+  {
+    const int cx = bx * M::CELLS + (lane & 15);
+    if (lane < 16 && cx < g.num_cells[0] && cy < g.num_cells[1] && cz < g.num_cells[2]) {
+      const int cell = cx + g.num_cells[0] * (cy + g.num_cells[1] * cz);
+      ce = part.cell_end[cell];
+      cs = part.cell_start[cell];
+    }
+  }
+  if (!__syncthreads_or(ce > cs)) {
+    if (t == 0) flags[blockIdx.x] = 0;
+    return;
+  }
+  float *blk = &block[wave][0][0];
+  for (int i = lane; i < NV * M::BN; i += 64) blk[i] = 0.f;
+  float *st = &stage[wave][0][0];
+
+  // lane roles in the products
+  const int s1 = (lane & 15) >> 2, s2 = lane & 3, pk = lane >> 4;
+  // staged fields of row a = s2 of A = -V sigma (xx 7, yy 8, zz 9, xy 10,
+  // xz 11, yz 12); row 3 of the product is padding
+  const int ax = s2 == 1 ? 10 : (s2 == 2 ? 11 : 7);
+  const int ay = s2 == 1 ? 8 : (s2 == 2 ? 12 : 10);
+  const int az = s2 == 1 ? 12 : (s2 == 2 ? 9 : 11);
+  // y and z coefficients of this lane's offsets (piece 3 - s of the node's
+  // type), fixed for the wave's row
+  float yc[4], zc[4];
+  {
+    const int ty = cubic_node_type(cy + NLO + s1, g.num_cells[1], prm.node_types_built);
+    const int tz = cubic_node_type(cz + NLO + s2, g.num_cells[2], prm.node_types_built);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      yc[q] = kCubic[ty][3 - s1][q];
+      zc[q] = kCubic[tz][3 - s2][q];
+    }
+  }
+  // cubic piece at r (offset s of the span), value and derivative; offset 0
+  // at frac = 1 - 2^-24 rounds to r = 2, where the spline is 0 (span_eval)
+  auto piece = [&](const float *c, float r, bool first, float &N, float &dN) {
+    N = ((c[3] * r + c[2]) * r + c[1]) * r + c[0];
+    dN = inv_h * ((3.f * c[3] * r + 2.f * c[2]) * r + c[1]);
+    if (first && !(r < 2.f)) N = dN = 0.f;
+  };
+
+  for (int lx = 0; lx < M::CELLS; ++lx) {  // wave-uniform
+    const int c0 = __shfl(cs, lx), c1 = __shfl(ce, lx);
+    if (c1 <= c0) continue;
+    const int cx = bx * M::CELLS + lx;
+    float xc[4];
+    {
+      const int tx = cubic_node_type(cx + NLO + s1, g.num_cells[0], prm.node_types_built);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) xc[q] = kCubic[tx][3 - s1][q];
+    }
+    v4f C1 = {0.f, 0.f, 0.f, 0.f}, C2 = {0.f, 0.f, 0.f, 0.f};
+    for (int w0 = c0; w0 < c1; w0 += 64) {
+      const int wn = min(64, c1 - w0);
+      if (lane < wn) {
+        const int j = w0 + lane;
+        const int slot = part.sorted[j];
+        float x[3], fr[3];
+        bool own = true;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          x[a] = in.x[a * n + slot];
+          const float ta = cell_coord(x[a], g.origin[a], inv_h);
+          const int ba = (int)ta;
+          own = own && ba == (a == 0 ? cx : (a == 1 ? cy : cz));
+          fr[a] = __fsub_rn(ta, (float)ba);
+        }
+        const float m = in.mass[slot];
+        float v[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] = in.v[a * n + slot];
+        const Sym sg = load_sym(in.stress, n, slot);
+        const float mvol = -in.volume[slot];
+        const Sym A{mvol * sg.xx, mvol * sg.yy, mvol * sg.zz, mvol * sg.xy, mvol * sg.xz, mvol * sg.yz};
+        if (!own) {
+          // listed in this cell but binned elsewhere (a partition of other
+          // positions): deposited over its own window, no product terms here
+#pragma unroll 1
+          for (int sy = 0; sy < 4; ++sy) p2g_stray<3, SF, 0>(prm, g, nodes, in.x, n, slot, sy, m, v, A);
+        }
+        const float keep = own ? 1.f : 0.f;
+        st[0 * 64 + lane] = fr[0];
+        st[1 * 64 + lane] = fr[1];
+        st[2 * 64 + lane] = fr[2];
+        st[3 * 64 + lane] = keep * m;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) st[(4 + a) * 64 + lane] = keep * m * v[a];
+        st[7 * 64 + lane] = keep * A.xx;
+        st[8 * 64 + lane] = keep * A.yy;
+        st[9 * 64 + lane] = keep * A.zz;
+        st[10 * 64 + lane] = keep * A.xy;
+        st[11 * 64 + lane] = keep * A.xz;
+        st[12 * 64 + lane] = keep * A.yz;
+      }
+      wave_lds_sync();
+      for (int k0 = 0; k0 < wn; k0 += 4) {
+        const int q = k0 + pk;
+        const bool valid = q < wn;
+        const int qq = valid ? q : 0;
+        const float fx = st[0 * 64 + qq], fy = st[1 * 64 + qq], fz = st[2 * 64 + qq];
+        // F_f (f = s2): m, m vx, m vy, m vz; A_{a, .} (a = s2; row 3 is padding)
+        const float Ff = st[(3 + s2) * 64 + qq];
+        const float Aax = st[ax * 64 + qq], Aay = st[ay * 64 + qq], Aaz = st[az * 64 + qq];
+        float Nx, dNx, Ny, dNy, Nz, dNz;
+        piece(xc, __fsub_rn(fx, (float)(NLO + s1)), s1 == 0, Nx, dNx);
+        piece(yc, __fsub_rn(fy, (float)(NLO + s1)), s1 == 0, Ny, dNy);
+        piece(zc, __fsub_rn(fz, (float)(NLO + s2)), s2 == 0, Nz, dNz);
+        const float pad = (valid && s2 < 3) ? 1.f : 0.f;
+        const float a1 = valid ? Nx * Ff : 0.f;
+        const float a2x = pad * dNx * Aax, a2y = pad * Nx * Aay, a2z = pad * Nx * Aaz;
+        const float b1 = valid ? Ny * Nz : 0.f;
+        const float b2y = valid ? dNy * Nz : 0.f, b2z = valid ? Ny * dNz : 0.f;
+        C1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, C1, 0, 0, 0);
+        C2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2x, b1, C2, 0, 0, 0);
+        C2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2y, b2y, C2, 0, 0, 0);
+        C2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2z, b2z, C2, 0, 0, 0);
+      }
+      wave_lds_sync();
+    }
+    // node (i, j, k) = (l / 16, (l % 16) / 4, l % 4) of the cell's window
+    const int nb = (lx + pk) + M::BX * (s1 + M::BY * s2);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) blk[r * M::BN + nb] += C1[r];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) blk[(4 + r) * M::BN + nb] += C2[r];
+  }
+  __syncthreads();
+  // slice node (ix, iy, iz): the blocks of waves iz - 3 .. iz
+  float *slice = partials + (int64_t)blockIdx.x * NV * NTN;
+  if (t == 0) flags[blockIdx.x] = 1;
+  for (int i = t; i < NTN; i += 256) {
+    const int ix = i % M::BX, iy = (i / M::BX) % M::BY, iz = i / (M::BX * M::BY);
+    float sum[NV] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const int z = iz - w;
+      if (z < 0 || z >= M::BZ) continue;
+#pragma unroll
+      for (int f = 0; f < NV; ++f) sum[f] += block[w][f][ix + M::BX * (iy + M::BY * z)];
+    }
+#pragma unroll
+    for (int f = 0; f < NV; ++f) slice[f * NTN + i] = sum[f];
+  }
+}
+
 // Cells whose cubic window can wrap into the grid through the linear index:
 // x in {0, nx-2, nx-1} (any y, z) and, in 3D, y in {0, ny-2, ny-1} (other
 // x). Offsets outside the grid along the slowest axis always leave the index
@@ -1642,15 +1836,27 @@ void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles
                 hipStream_t s, const float *rec = nullptr) {
   const TileGeom tg = tile_geom_of<D, SF>(g, 1);
   const dim3 grid((unsigned)tile_count(tg)), block(P2GGeom<D, SF>::THREADS);
-  if (mode == 0)
-    hipLaunchKernelGGL((p2g_kernel<D, SF, 0>), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
-                       tg.tiles[1], partials, flags, rec);
-  else if (mode == 1)
+  // 3D typed cubic P2G on the matrix cores; everything else on the VALU kernel
+  bool mfma = false;
+  if constexpr (D == 3 && SF == PMPM_SF_CUBIC) {
+    mfma = mode == 0 && prm.node_types_built;
+    if (mfma)
+      hipLaunchKernelGGL((p2g_mfma_kernel<SF>), grid, dim3(256), 0, s, prm, g, in, part, nodes, tg.tiles[0],
+                         tg.tiles[1], partials, flags);
+  }
+  // (each mode launches exactly one tile kernel; the adjoint scatter reads
+  // `rec` and is never launched without it)
+  if (mode == 0) {
+    if (!mfma)
+      hipLaunchKernelGGL((p2g_kernel<D, SF, 0>), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
+                         tg.tiles[1], partials, flags, rec);
+  } else if (mode == 1) {
     hipLaunchKernelGGL((p2g_kernel<D, SF, 1>), dim3((unsigned)tile_count(tg) + kMuslTailBlocks), block, 0, s, prm, g,
                        in, out, part, nodes, tg.tiles[0], tg.tiles[1], partials, flags, rec);
-  else
+  } else if (mode == 2 && rec) {
     hipLaunchKernelGGL((p2g_kernel<D, SF, 2>), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
                        tg.tiles[1], partials, flags, rec);
+  }
   const int wcells = wrap_cell_count<D>(g);
   if (mode != 2 && SF == PMPM_SF_CUBIC && prm.node_types_built && prm.wrap_quirk && wcells > 0) {
     if (mode == 0)
