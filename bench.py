@@ -9,9 +9,12 @@ Drucker-Prager (phi 30 deg, psi 0, c 0), MUSL, cubic B-splines, domain
 
 A step is one full MUSL update of all particles: spatial sort, P2G, node
 BCs, first map + re-map, node BCs, second map + return map. Inputs are
-resident in HBM before the timed region. The timed loop drives the C ABI of
-libpmpm_hip.so on one stream with HIP events around every entry point; the
-dominant kernel's roofline uses its own event time.
+resident in HBM before the timed region. `value` times pmpm_step (the C ABI's
+whole-step entry point) on one stream; a second timed region issues the same
+step entry point by entry point (pmpm_partition, pmpm_p2g, pmpm_grid_bc,
+pmpm_musl_map, pmpm_grid_bc, pmpm_musl_g2p) with HIP events on the launch
+stream around each, which gives the per-entry split and the roofline
+kernel's own launch time (pmpm_musl_g2p launches only that kernel).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -34,10 +37,10 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per particle of each entry point (FP32, 3D, cubic,
 # Drucker-Prager, MUSL): the particle fields each one must read and write.
 BYTES_PER_PARTICLE = {
-    "partition": 76,     # hash: x 12 + key 4; 3 radix passes x (key 4 + key/val 8 in + 8 out); ranges 4
+    "partition": 56,     # hash: x 12 in, key 4 out; 10-bit pass 1: key 4 (histogram) + key 4 in, key 4 +
+                         # slot 4 out; pass 2: key 4 (histogram) + key/slot 8 in, 8 out; ranges: key 4
     "p2g": 64,           # sorted 4, x 12, v 12, m 4, V 4, stress 24, + node flush
-    "musl_map": 84,      # first map: sorted 4, x 12, v 12 in; x 12, v 12 out;
-                         # re-map: sorted 4, x 12, m 4, v' 12 in
+    "musl_map": 56,      # fused first map + re-map: sorted 4, x 12, v 12, m 4 in; x 12, v 12 out
     "musl_g2p": 252,     # in: sorted 4 x 12 F 36 V0 4 stress 24 eps 24 m 4 colour 1 flags 1 id 4
                          # out: F 36 V 4 stress 24 eps 24 L 36 m 4 V0 4 colour 1 flags 1 id 4
 }
@@ -308,24 +311,34 @@ def main():
     print(f"bench: column collapse, {n} particles, grid {list(wl.grid.num_cells)}, warmup {args.warmup}, "
           f"steps {args.steps}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
-        wl.step()
+        wl.fused_step()
     torch.cuda.synchronize()
-    ncalls = len(wl.calls())
-    ev = [[[torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)] for _ in range(ncalls)]
-          for _ in range(args.steps)]
+    # timed region 1: the step through pmpm_step (the value)
     t_start, t_end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
     t_start.record()
     for k in range(args.steps):
-        wl.step(ev[k])
+        wl.fused_step()
     t_end.record()
     torch.cuda.synchronize()
     total_ms = t_start.elapsed_time(t_end)
+    ms_per_step = total_ms / args.steps
+    value = n * args.steps / (total_ms / 1e3)
+    # timed region 2: the same step issued entry point by entry point, HIP
+    # events on the launch stream around each (the per-entry split and the
+    # roofline kernel's launch time)
+    ncalls = len(wl.calls())
+    ev = [[[torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)] for _ in range(ncalls)]
+          for _ in range(args.steps)]
+    e_start, e_end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e_start.record()
+    for k in range(args.steps):
+        wl.step(ev[k])
+    e_end.record()
+    torch.cuda.synchronize()
+    entries_ms_per_step = e_start.elapsed_time(e_end) / args.steps
     names = [c[0] for c in wl.calls()]
     per_call = {nm: sum(ev[k][i][0].elapsed_time(ev[k][i][1]) for k in range(args.steps)) / args.steps
                 for i, nm in enumerate(names)}
-    ms_per_step = total_ms / args.steps
-    value = n * args.steps / (total_ms / 1e3)
     print("bench: per-entry-point mean ms: " + json.dumps({k: round(v, 4) for k, v in per_call.items()}),
           file=sys.stderr, flush=True)
     dom = ROOFLINE_ENTRY
@@ -352,7 +365,8 @@ def main():
         "config": {"workload": "3D Drucker-Prager MUSL column collapse, cubic B-spline, "
                                f"{n} particles (27 per cell), grid 101x51x51, dt 1e-5",
                    "particles": n, "cell_size": args.cell_size, "solver": "MUSL", "shape_function": "cubic",
-                   "material": "Drucker-Prager phi=30 psi=0 c=0", "per_entry_point_ms": per_call},
+                   "material": "Drucker-Prager phi=30 psi=0 c=0", "entry_point": "pmpm_step",
+                   "per_entry_point_ms": per_call, "entry_by_entry_ms_per_step": round(entries_ms_per_step, 4)},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
