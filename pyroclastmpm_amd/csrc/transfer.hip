@@ -2047,11 +2047,12 @@ extern "C" int pmpm_stress_update(const pmpm_params *prm, int32_t mat_id, pmpm_p
 //                            v' = (1 - a) vpic + a (v + dvp), x' = x + dt vpic,
 //                            F' = (I + dt L) F, (sigma', eps') = return map
 // The adjoint runs these backwards: g2p_adjoint_kernel (per particle: return
-// map VJP, F, v, x, and the position adjoint through the weights), the
-// adjoint scatter (p2g_kernel MODE 2) of the PIC/FLIP/L adjoints to the
-// nodes, grid_adjoint_kernel (per node: velocities, BCs, forward moment), and
+// map VJP, F, v, and the records of the PIC/FLIP/L adjoints), the adjoint
+// scatter (p2g_kernel MODE 2) of those records to the nodes,
+// grid_adjoint_kernel (per node: velocities, BCs, forward moment), and
 // p2g_adjoint_kernel (per particle: gather of the node adjoints into v, sigma,
-// V = det(F) V0 and x). The weights' position dependence enters through
+// V = det(F) V0 and x, together with the position adjoint of the G2P weights
+// over the same window). The weights' position dependence enters through
 // grad w and its Hessian.
 // ===========================================================================
 namespace pmpm {
@@ -2089,70 +2090,6 @@ __device__ inline void entry_derivs(const float *N, const float *dN, const float
   H[3] = dN[0] * dN[1] * N[2];
   H[4] = dN[0] * N[1] * dN[2];
   H[5] = N[0] * dN[1] * dN[2];
-}
-
-// Position adjoint through the G2P weights (same window and per-axis checks
-// as gather_grid): xbar_c += sum_i grad_c w (vnt . vpb + dv . dvb)
-//                                 + sum_ab d2w/dx_c dx_a Lb_ab vnt_b.
-template <int D, int SF>
-__device__ inline void g2p_weights_adjoint(const pmpm_params &prm, const pmpm_grid &g, const pmpm_nodes &nodes,
-                                           const PW<D, SF> &w, const float *vpb, const float *dvb, const float *Lb,
-                                           float *xbar) {
-  constexpr int E = Spline<SF>::ENTRIES;
-  constexpr int EY = D > 1 ? E : 1;
-  constexpr int EZ = D > 2 ? E : 1;
-  const int nn = g.num_cells_total;
-  float d2w[3][4];
-  window_d2<D, SF>(prm, g, w, d2w);
-  float N[3][4], dN[3][4], d2N[3][4];
-  int off[3][4];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const int stride = a == 0 ? 1 : (a == 1 ? g.num_cells[0] : g.num_cells[0] * g.num_cells[1]);
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      if (a < D && e < E) {
-        const int i = w.bin[a] + w.first[a] + e;
-        const bool ok = i >= 0 && i < g.num_cells[a];
-        off[a][e] = min(max(i, 0), g.num_cells[a] - 1) * stride;
-        N[a][e] = ok ? w.N[a][e] : 0.f;
-        dN[a][e] = ok ? w.dN[a][e] : 0.f;
-        d2N[a][e] = ok ? d2w[a][e] : 0.f;
-      } else {
-        off[a][e] = 0;
-        N[a][e] = 1.f;
-        dN[a][e] = 0.f;
-        d2N[a][e] = 0.f;
-      }
-    }
-  }
-#pragma unroll
-  for (int ey = 0; ey < EY; ++ey)
-#pragma unroll
-    for (int ex = 0; ex < E; ++ex)
-#pragma unroll
-      for (int ez = 0; ez < EZ; ++ez) {
-        const int node = off[0][ex] + off[1][ey] + off[2][ez];
-        const float n3[3] = {N[0][ex], N[1][ey], N[2][ez]}, d3[3] = {dN[0][ex], dN[1][ey], dN[2][ez]},
-                    h3[3] = {d2N[0][ex], d2N[1][ey], d2N[2][ez]};
-        float wgt, gr[3], H[6];
-        entry_derivs(n3, d3, h3, wgt, gr, H);
-        float vnt[3] = {0.f, 0.f, 0.f}, dvn[3] = {0.f, 0.f, 0.f}, s = 0.f, u[3] = {0.f, 0.f, 0.f};
-        load_node_vel<D, true>(nodes.vel, node, vnt, dvn);
-#pragma unroll
-        for (int b = 0; b < D; ++b) s += vnt[b] * vpb[b] + dvn[b] * dvb[b];
-#pragma unroll
-        for (int a = 0; a < D; ++a)
-#pragma unroll
-          for (int b = 0; b < D; ++b) u[a] += Lb[3 * a + b] * vnt[b];
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-          float hu = 0.f;
-#pragma unroll
-          for (int a = 0; a < D; ++a) hu += H[sym_idx(c, a)] * u[a];
-          xbar[c] += gr[c] * s + hu;
-        }
-      }
 }
 
 // Wave sum of v (all 64 lanes take part).
@@ -2264,7 +2201,8 @@ __global__ __launch_bounds__(256) void g2p_adjoint_kernel(pmpm_params prm, pmpm_
       dvb[a] = al * vo[a];
       xb[a] = xo[a];
     }
-    g2p_weights_adjoint<D, SF>(prm, g, nodes, w, vpb, dvb, Lb, xb);
+    // (the position adjoint through the weights is added by p2g_adjoint_kernel,
+    // which walks the same window)
 
 #pragma unroll
     for (int a = 0; a < D; ++a) {
@@ -2377,11 +2315,17 @@ template <int D> __device__ inline void cofactor(const float *F, float *C) {
 
 // Per particle: gather of the node adjoints through the P2G window (the same
 // entries as the forward deposit, linear-index wrap-around included) into the
-// adjoints of v, sigma, F (through V = det(F) V0) and x; added to ib.
+// adjoints of v, sigma, F (through V = det(F) V0) and x, plus the position
+// adjoint through the G2P weights, which walk the same window:
+//   xbar_c += sum_i grad_c w (vnt . vpb + dv . dvb) + sum_ab d2w/dx_c dx_a Lb_ab vnt_b
+// with vpb, dvb, Lb the records g2p_adjoint_kernel wrote. The in-grid entries
+// run branch-free (clamped node, zero weights outside, as gather_grid); the
+// wrap-around entries of boundary particles, which only the deposit reaches,
+// take a separate compact loop. Adds to ib.
 template <int D, int SF>
 __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles st,
-                                                          pmpm_adjoint ib, pmpm_spatial part,
-                                                          const float *__restrict__ nbar) {
+                                                          pmpm_adjoint ib, pmpm_spatial part, pmpm_nodes nodes,
+                                                          const float *__restrict__ nbar, const float *__restrict__ rec) {
   constexpr int E = Spline<SF>::ENTRIES;
   constexpr int EY = D > 1 ? E : 1;
   constexpr int EZ = D > 2 ? E : 1;
@@ -2404,76 +2348,125 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = 0; b < 3; ++b) S[a][b] = sym_at(s, a, b);
+  float vpb[3] = {0.f, 0.f, 0.f}, dvb[3] = {0.f, 0.f, 0.f}, Lb[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    vpb[a] = rec[a * n + j];
+    dvb[a] = rec[(3 + a) * n + j];
+  }
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int b = 0; b < D; ++b) Lb[3 * a + b] = rec[(6 + 3 * a + b) * n + j];
   float vb[3] = {0.f, 0.f, 0.f}, xb[3] = {0.f, 0.f, 0.f}, Mb[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f},
                                                                          {0.f, 0.f, 0.f}};
   float Vb = 0.f;
+  // one entry's contributions (g2p = 1 adds the G2P weights' position term)
+  auto entry = [&](int node, const float *N3, const float *dN3, const float *d2N3, bool g2p) {
+    float wgt, gr[3], H[6];
+    entry_derivs(N3, dN3, d2N3, wgt, gr, H);
+    const float mb = nbar[node];
+    float Pb[3] = {0.f, 0.f, 0.f}, fb[3] = {0.f, 0.f, 0.f}, vP = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      Pb[a] = nbar[(1 + a) * nn + node];
+      fb[a] = nbar[(1 + D + a) * nn + node];
+      vP += v[a] * Pb[a];
+    }
+    float sf[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int b = 0; b < D; ++b)
+#pragma unroll
+      for (int a = 0; a < D; ++a) sf[b] += S[a][b] * fb[a];
+    float gs = m * (mb + vP), u[3] = {0.f, 0.f, 0.f};
+    if (g2p) {
+      float vnt[3] = {0.f, 0.f, 0.f}, dvn[3] = {0.f, 0.f, 0.f};
+      load_node_vel<D, true>(nodes.vel, node, vnt, dvn);
+#pragma unroll
+      for (int b = 0; b < D; ++b) gs += vnt[b] * vpb[b] + dvn[b] * dvb[b];
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) u[a] += Lb[3 * a + b] * vnt[b];
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      vb[a] += m * wgt * Pb[a];
+      Vb -= sf[a] * gr[a];
+#pragma unroll
+      for (int b = 0; b < D; ++b) Mb[a][b] -= V * fb[a] * gr[b];
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      float hs = 0.f;
+#pragma unroll
+      for (int b = 0; b < D; ++b) hs += H[sym_idx(c, b)] * (u[b] - V * sf[b]);
+      xb[c] += gr[c] * gs + hs;
+    }
+  };
+  // in-grid entries
+  float N[3][4], dN[3][4], d2N[3][4];
+  int off[3][4];
+  bool leaves = false;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int stride = a == 0 ? 1 : (a == 1 ? g.num_cells[0] : g.num_cells[0] * g.num_cells[1]);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (a < D && e < E) {
+        const int i = w.bin[a] + w.first[a] + e;
+        const bool ok = i >= 0 && i < g.num_cells[a];
+        leaves = leaves || !ok;
+        off[a][e] = min(max(i, 0), g.num_cells[a] - 1) * stride;
+        N[a][e] = ok ? w.N[a][e] : 0.f;
+        dN[a][e] = ok ? w.dN[a][e] : 0.f;
+        d2N[a][e] = ok ? d2w[a][e] : 0.f;
+      } else {
+        off[a][e] = 0;
+        N[a][e] = 1.f;
+        dN[a][e] = 0.f;
+        d2N[a][e] = 0.f;
+      }
+    }
+  }
 #pragma unroll
   for (int ey = 0; ey < EY; ++ey)
 #pragma unroll
     for (int ex = 0; ex < E; ++ex)
 #pragma unroll
       for (int ez = 0; ez < EZ; ++ez) {
-        const int e3[3] = {ex, ey, ez};
-        int nb[3] = {0, 0, 0}, off[3] = {0, 0, 0};
-        bool inside = true;
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-          off[a] = w.first[a] + e3[a];
-          nb[a] = w.bin[a] + off[a];
-          inside = inside && nb[a] >= 0 && nb[a] < g.num_cells[a];
-        }
-        const int64_t lin = lin_index(g, nb, D);
-        if (lin < 0 || lin >= nn || (!inside && !prm.wrap_quirk)) continue;
-        float N[3] = {1.f, 1.f, 1.f}, dN[3] = {0.f, 0.f, 0.f}, d2N[3] = {0.f, 0.f, 0.f};
-        if (inside) {
-#pragma unroll
-          for (int a = 0; a < D; ++a) {
-            N[a] = w.N[a][e3[a]];
-            dN[a] = w.dN[a][e3[a]];
-            d2N[a] = d2w[a][e3[a]];
-          }
-        } else {  // the wrapped node's types, as the forward deposit
-          int ids[3];
-          node_ids<D>(g, lin, ids);
-#pragma unroll
-          for (int a = 0; a < D; ++a) {
-            const int type = cubic_node_type(ids[a], g.num_cells[a], prm.node_types_built);
-            const float r = __fsub_rn(__fsub_rn(w.t[a], (float)w.bin[a]), (float)off[a]);
-            spline_eval<SF>(r, type, g.inv_cell_size, N[a], dN[a]);
-            d2N[a] = spline_d2<SF>(r, type, g.inv_cell_size);
-          }
-        }
-        const int node = (int)lin;
-        float wgt, gr[3], H[6];
-        entry_derivs(N, dN, d2N, wgt, gr, H);
-        const float mb = nbar[node];
-        float Pb[3] = {0.f, 0.f, 0.f}, fb[3] = {0.f, 0.f, 0.f}, vP = 0.f;
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-          Pb[a] = nbar[(1 + a) * nn + node];
-          fb[a] = nbar[(1 + D + a) * nn + node];
-          vP += v[a] * Pb[a];
-        }
-        float sf[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-        for (int b = 0; b < D; ++b)
-#pragma unroll
-          for (int a = 0; a < D; ++a) sf[b] += S[a][b] * fb[a];
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-          vb[a] += m * wgt * Pb[a];
-          Vb -= sf[a] * gr[a];
-#pragma unroll
-          for (int b = 0; b < D; ++b) Mb[a][b] -= V * fb[a] * gr[b];
-        }
-#pragma unroll
-        for (int c = 0; c < D; ++c) {
-          float hs = 0.f;
-#pragma unroll
-          for (int b = 0; b < D; ++b) hs += H[sym_idx(c, b)] * sf[b];
-          xb[c] += m * gr[c] * (mb + vP) - V * hs;
-        }
+        const float n3[3] = {N[0][ex], N[1][ey], N[2][ez]}, d3[3] = {dN[0][ex], dN[1][ey], dN[2][ez]},
+                    h3[3] = {d2N[0][ex], d2N[1][ey], d2N[2][ez]};
+        entry(off[0][ex] + off[1][ey] + off[2][ez], n3, d3, h3, true);
       }
+  // wrap-around entries (deposit only; the wrapped node's types)
+  if (leaves && prm.wrap_quirk) {
+#pragma unroll 1
+    for (int e = 0; e < E * EY * EZ; ++e) {
+      const int e3[3] = {e % E, D > 1 ? (e / E) % EY : 0, D > 2 ? e / (E * EY) : 0};
+      int nb[3] = {0, 0, 0}, offs[3] = {0, 0, 0};
+      bool inside = true;
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        offs[a] = w.first[a] + e3[a];
+        nb[a] = w.bin[a] + offs[a];
+        inside = inside && nb[a] >= 0 && nb[a] < g.num_cells[a];
+      }
+      const int64_t lin = lin_index(g, nb, D);
+      if (inside || lin < 0 || lin >= nn) continue;
+      int ids[3];
+      node_ids<D>(g, lin, ids);
+      float n3[3] = {1.f, 1.f, 1.f}, d3[3] = {0.f, 0.f, 0.f}, h3[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        const int type = cubic_node_type(ids[a], g.num_cells[a], prm.node_types_built);
+        const float r = __fsub_rn(__fsub_rn(w.t[a], (float)w.bin[a]), (float)offs[a]);
+        spline_eval<SF>(r, type, g.inv_cell_size, n3[a], d3[a]);
+        h3[a] = spline_d2<SF>(r, type, g.inv_cell_size);
+      }
+      entry((int)lin, n3, d3, h3, false);
+    }
+  }
   float F[9], C[9];
 #pragma unroll
   for (int k = 0; k < D * D; ++k) F[k] = st.F[k * n + src];
@@ -2503,7 +2496,8 @@ void launch_adjoint_particles(const pmpm_params &prm, const pmpm_grid &g, const 
     hipLaunchKernelGGL((g2p_adjoint_kernel<D, SF>), dim3(blocks), dim3(256), 0, s, prm, g, st, ob, ib, part, nodes,
                        rec, pbar);
   else
-    hipLaunchKernelGGL((p2g_adjoint_kernel<D, SF>), dim3(blocks), dim3(256), 0, s, prm, g, st, ib, part, nbar);
+    hipLaunchKernelGGL((p2g_adjoint_kernel<D, SF>), dim3(blocks), dim3(256), 0, s, prm, g, st, ib, part, nodes, nbar,
+                       rec);
 }
 
 }  // namespace pmpm
