@@ -268,15 +268,19 @@ typedef struct pmpm_adjoint {
  * step's output `out_bar` (in the output's order: slot j is the j-th particle
  * of the cell order of `state`, as pmpm_step writes it), writes the adjoint of
  * `state` into `in_bar` (state's order) and adds the gradient with respect to
- * each material's p[0..3] to param_bar[4 * material + k]. The step's
- * partition, P2G and grid update are recomputed into `part` and `nodes`. The
- * active sets (return-map branch, slip clamps, mass cutoff) are those of the
- * forward step; no smoothing. Supported: linear elastic and Drucker-Prager
- * materials, gravity and node-domain conditions; anything else returns
- * PMPM_ERR_UNSUPPORTED. */
+ * each material's p[0..3] to param_bar[4 * material + k]. With recompute = 1
+ * the step's partition, P2G and grid update are recomputed into `part` and
+ * `nodes`; with recompute = 0 they are read as they are, and must be what
+ * pmpm_step(musl = 0) left there when it stepped `state` with the same
+ * parameters and grid (a caller that keeps one part/nodes set per step of a
+ * recomputed segment saves that work). The active sets (return-map branch,
+ * slip clamps, mass cutoff) are those of the forward step; no smoothing.
+ * Supported: linear elastic and Drucker-Prager materials, gravity and
+ * node-domain conditions; anything else returns PMPM_ERR_UNSUPPORTED. */
 int pmpm_adjoint_step(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid,
                       const pmpm_particles *state, const pmpm_adjoint *out_bar, pmpm_adjoint *in_bar,
-                      float *param_bar, pmpm_spatial *part, pmpm_nodes *nodes, pmpm_stream_t stream);
+                      float *param_bar, int32_t recompute, pmpm_spatial *part, pmpm_nodes *nodes,
+                      pmpm_stream_t stream);
 
 /* ---- batched scenes (no reference counterpart: the reference runs one
  * scene per process through its globals, global_settings.cpp; this is the

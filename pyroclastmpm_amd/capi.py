@@ -104,8 +104,8 @@ PROTOTYPES = {
     "pmpm_stress_update": (C.c_int, [P(Params), C.c_int32, P(Particles), vp]),
     "pmpm_step": (C.c_int, [vp, P(Params), P(Grid), C.c_int32, P(Particles), P(Particles), P(Spatial), P(Nodes),
                             vp]),
-    "pmpm_adjoint_step": (C.c_int, [vp, P(Params), P(Grid), P(Particles), P(Adjoint), P(Adjoint), vp, P(Spatial),
-                                    P(Nodes), vp]),
+    "pmpm_adjoint_step": (C.c_int, [vp, P(Params), P(Grid), P(Particles), P(Adjoint), P(Adjoint), vp, C.c_int32,
+                                    P(Spatial), P(Nodes), vp]),
     "pmpm_batch_create": (C.c_int, [P(vp), C.c_int32]),
     "pmpm_batch_destroy": (C.c_int, [vp]),
     "pmpm_batch_assign": (C.c_int, [P(C.c_float), C.c_int32, C.c_int32, P(C.c_int32)]),

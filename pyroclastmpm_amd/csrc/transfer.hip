@@ -2504,9 +2504,10 @@ void launch_adjoint_particles(const pmpm_params &prm, const pmpm_grid &g, const 
 
 extern "C" int pmpm_adjoint_step(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid,
                                  const pmpm_particles *state, const pmpm_adjoint *out_bar, pmpm_adjoint *in_bar,
-                                 float *param_bar, pmpm_spatial *part, pmpm_nodes *nodes, pmpm_stream_t stream) {
+                                 float *param_bar, int32_t recompute, pmpm_spatial *part, pmpm_nodes *nodes,
+                                 pmpm_stream_t stream) {
   if (!ctx || !params_ok(prm) || !grid || !particles_ok(state) || !out_bar || !in_bar || !param_bar || !part ||
-      !nodes)
+      !nodes || (recompute != 0 && recompute != 1))
     return PMPM_ERR_INVALID;
   const int n = state->n;
   if (n == 0) return PMPM_OK;
@@ -2524,11 +2525,16 @@ extern "C" int pmpm_adjoint_step(pmpm_context *ctx, const pmpm_params *prm, cons
   hipStream_t s = (hipStream_t)stream;
   const pmpm_grid &g = *grid;
   const int D = prm->dim;
-  // the forward step's partition, P2G and grid update
-  int rc = pmpm_partition(ctx, D, grid, state->x, n, part, stream);
-  if (rc != PMPM_OK) return rc;
-  if ((rc = do_p2g(ctx, *prm, g, *state, *part, *nodes, s)) != PMPM_OK) return rc;
-  if ((rc = launch_grid(ctx, *prm, g, *nodes, 1, 0, prm->num_bcs, 0, 1, s)) != PMPM_OK) return rc;
+  // the forward step's partition, P2G and grid update (kept by the caller
+  // from its own pmpm_step when recompute == 0)
+  int rc;
+  if (recompute) {
+    if ((rc = pmpm_partition(ctx, D, grid, state->x, n, part, stream)) != PMPM_OK) return rc;
+    if ((rc = do_p2g(ctx, *prm, g, *state, *part, *nodes, s)) != PMPM_OK) return rc;
+    if ((rc = launch_grid(ctx, *prm, g, *nodes, 1, 0, prm->num_bcs, 0, 1, s)) != PMPM_OK) return rc;
+  } else if ((rc = reserve_tiles(ctx, *prm, g)) != PMPM_OK) {
+    return rc;
+  }
   if ((rc = ensure_buffer(ctx->adj_rec, ctx->adj_rec_cap, (size_t)15 * n)) != PMPM_OK) return rc;
   if ((rc = ensure_buffer(ctx->adj_nodes, ctx->adj_nodes_cap, (size_t)(1 + 2 * D) * g.num_cells_total)) != PMPM_OK)
     return rc;

@@ -51,6 +51,22 @@ class ParticleSet:
         return o
 
 
+class StepBuffers:
+    """The partition (pmpm_spatial) and node arrays (pmpm_nodes) one step
+    uses."""
+
+    def __init__(self, n, nn, dim, device="cuda"):
+        i32 = lambda m: torch.zeros(m, dtype=torch.int32, device=device)
+        self.sp = {"hash": i32(n), "hash_sorted": i32(n), "sorted": i32(n), "cell_start": i32(nn),
+                   "cell_end": i32(nn)}
+        self.spatial = capi.Spatial(*[self.sp[k].data_ptr() for k in ["hash", "hash_sorted", "sorted",
+                                                                      "cell_start", "cell_end"]], None)
+        f = lambda m: torch.zeros(m, dtype=torch.float32, device=device)
+        self.nd = {"mass": f(nn), "mom": f(dim * nn), "mom_nt": f(dim * nn),
+                   "vel": f(capi.node_vel_stride(dim) * nn)}
+        self.nodes = capi.Nodes(*[self.nd[k].data_ptr() for k in ["mass", "mom", "mom_nt", "vel"]])
+
+
 class AdjointSet:
     """Adjoints of the particle state (x, v, stress, F, eps_e), as pmpm_adjoint."""
 
@@ -116,17 +132,11 @@ class DiffMPM:
         p0.t["mass"].fill_(float(np.float32(density) * vol))
         self.initial = p0
         self.gravity = list(gravity) if gravity is not None else [0.0] * (dim - 1) + [-9.81]
-        i32 = lambda m: torch.zeros(m, dtype=torch.int32, device=device)
-        self.sp = {"hash": i32(n), "hash_sorted": i32(n), "sorted": i32(n), "cell_start": i32(nn),
-                   "cell_end": i32(nn)}
-        self.spatial = capi.Spatial(*[self.sp[k].data_ptr() for k in ["hash", "hash_sorted", "sorted",
-                                                                      "cell_start", "cell_end"]], None)
-        f = lambda m: torch.zeros(m, dtype=torch.float32, device=device)
-        self.nd = {"mass": f(nn), "mom": f(dim * nn), "mom_nt": f(dim * nn),
-                   "vel": f(capi.node_vel_stride(dim) * nn)}
-        self.nodes = capi.Nodes(*[self.nd[k].data_ptr() for k in ["mass", "mom", "mom_nt", "vel"]])
-        self.ctx = capi.Context(n, nn)
         self.device = device
+        self.work = StepBuffers(n, nn, dim, device)
+        self.sp, self.spatial, self.nd, self.nodes = self.work.sp, self.work.spatial, self.work.nd, self.work.nodes
+        self.ctx = capi.Context(n, nn)
+        self._segment = None  # per-step states and buffers of a recomputed segment, made on first use
 
     # -------------------------------------------------------------- params
     def params(self, theta):
@@ -154,16 +164,21 @@ class DiffMPM:
     def _stream(self):
         return torch.cuda.current_stream().cuda_stream
 
-    def step(self, prm, a, b):
+    def step(self, prm, a, b, work=None):
+        w = work or self.work
         capi.check(capi.lib().pmpm_step(self.ctx.ptr, C.byref(prm), C.byref(self.grid), 0, C.byref(a.view()),
-                                        C.byref(b.view()), C.byref(self.spatial), C.byref(self.nodes),
+                                        C.byref(b.view()), C.byref(w.spatial), C.byref(w.nodes),
                                         self._stream()), "pmpm_step")
 
-    def adjoint_step(self, prm, state, out_bar, in_bar, pbar):
+    def adjoint_step(self, prm, state, out_bar, in_bar, pbar, work=None, recompute=True):
+        """Adjoint of the step from `state`. With recompute=False, `work` must
+        hold the partition and nodes that step(prm, state, ., work) left."""
+        w = work or self.work
         capi.check(capi.lib().pmpm_adjoint_step(self.ctx.ptr, C.byref(prm), C.byref(self.grid),
                                                 C.byref(state.view()), C.byref(out_bar.view()),
-                                                C.byref(in_bar.view()), pbar.data_ptr(), C.byref(self.spatial),
-                                                C.byref(self.nodes), self._stream()), "pmpm_adjoint_step")
+                                                C.byref(in_bar.view()), pbar.data_ptr(), int(bool(recompute)),
+                                                C.byref(w.spatial), C.byref(w.nodes), self._stream()),
+                   "pmpm_adjoint_step")
 
     def forward(self, theta, steps):
         """Runs `steps` USL steps from the initial state; returns the final
@@ -185,25 +200,37 @@ class DiffMPM:
         x[state.t["id"].long()] = state.t["x"].t()
         return x
 
+    def _segment_buffers(self):
+        """K - 1 particle sets and K partition/node sets, reused by every
+        segment: step k of a segment reads state k (state 0 is the checkpoint
+        itself, which a USL step only reads) and keeps its partition and nodes
+        in work k for the adjoint of the same step."""
+        if self._segment is None:
+            nn = self.grid.num_cells_total
+            states = [ParticleSet(self.n, self.dim, self.device) for _ in range(self.K - 1)]
+            work = [self.work] + [StepBuffers(self.n, nn, self.dim, self.device) for _ in range(self.K - 1)]
+            self._segment = (states, work)
+        return self._segment
+
     def backward(self, theta, steps, ckpts, final, gx):
         """Adjoint of the final positions (original order) -> d/d(K, G, M, Pt)."""
         prm = self.params(theta)
-        gbar = AdjointSet(self.n, self.dim, self.device)
+        # every adjoint step writes all of in_bar: two sets, used in turn
+        gbar, ibar = AdjointSet(self.n, self.dim, self.device), AdjointSet(self.n, self.dim, self.device)
         gbar.t["x"].copy_(gx[final.t["id"].long()].t())
         pbar = torch.zeros(4, dtype=torch.float32, device=self.device)
+        pool, work = self._segment_buffers()
         starts = sorted(ckpts.keys())
         for s0 in reversed(starts):
             s1 = min(s0 + self.K, steps)
-            states = [ckpts[s0].clone()]
-            for _ in range(s0, s1 - 1):
-                nxt = ParticleSet(self.n, self.dim, self.device)
-                self.step(prm, states[-1], nxt)
-                states.append(nxt)
-            for t in reversed(range(s0, s1)):
-                ibar = AdjointSet(self.n, self.dim, self.device)
-                self.adjoint_step(prm, states[t - s0], gbar, ibar, pbar)
-                gbar = ibar
-            del states
+            states = [ckpts[s0]] + pool[:s1 - s0 - 1]
+            for k in range(s1 - s0 - 1):
+                self.step(prm, states[k], states[k + 1], work[k])
+            # the segment's last step was not run forward here: its adjoint
+            # recomputes the partition, P2G and grid update itself
+            for k in reversed(range(s1 - s0)):
+                self.adjoint_step(prm, states[k], gbar, ibar, pbar, work[k], recompute=k == s1 - s0 - 1)
+                gbar, ibar = ibar, gbar
         return pbar.double().cpu().numpy()
 
     def grad_theta(self, theta, pbar):

@@ -240,3 +240,40 @@ def test_multi_step_parameter_gradient_matches_finite_differences(oracle_mod):
         assert abs(g_ad[i] - g_64[i]) <= 1e-3 * abs(g_64[i]) + 1e-15, (i, g_ad, g_64)
         assert abs(g_ad[i] - g_fd[i]) <= 1e-1 * abs(g_fd[i]) + 1e-15, (i, g_ad, g_fd)
     assert all(abs(v) > 0 for v in g_64), g_64  # the column yields: every parameter matters
+
+
+def test_adjoint_with_kept_step_buffers_matches_recompute():
+    """pmpm_adjoint_step(recompute=0) on the partition and nodes pmpm_step left
+    equals recompute=1 (field-scaled relative error <= 1e-5: the wrap-around
+    deposits are float atomics, so the two node sets may differ in the last
+    bits)."""
+    from pyroclastmpm_amd.diff import AdjointSet, DiffMPM, StepBuffers, ParticleSet
+
+    spec = S.column(h=0.04, per_axis=2, dt=1e-4, jitter=0.1, seed=2, musl=False)
+    model = DiffMPM(3, spec.origin, spec.end, 0.04, spec.positions, 8, dt=1e-4)
+    theta = (30.0, 20.0, 1e6)
+    _banner(f"adjoint kept buffers vs recompute: 3D DP column, n={model.n}, 12 warm-up steps")
+    state, _ = model.forward(theta, 12)
+    prm = model.params(theta)
+    n = model.n
+    kept = StepBuffers(n, model.grid.num_cells_total, 3)
+    nxt = ParticleSet(n, 3)
+    model.step(prm, state, nxt, kept)
+    g = torch.Generator(device="cuda").manual_seed(4)
+    ob = AdjointSet(n, 3)
+    for k, v in ob.t.items():
+        v.copy_(torch.randn(v.shape, device="cuda", generator=g))
+    res = []
+    for recompute in (True, False):
+        ib = AdjointSet(n, 3)
+        pbar = torch.zeros(4, dtype=torch.float32, device="cuda")
+        model.adjoint_step(prm, state, ob, ib, pbar, work=None if recompute else kept, recompute=recompute)
+        torch.cuda.synchronize()
+        res.append((ib, pbar))
+    for k in ob.t:
+        a, b = res[1][0].t[k], res[0][0].t[k]
+        e = float((a - b).abs().max()) / max(float(b.abs().max()), 1e-30)
+        assert e <= 1e-5, (k, e)
+    pa, pb = res[1][1], res[0][1]
+    assert float((pa - pb).abs().max()) <= 1e-5 * float(pb.abs().max()) + 1e-30, (pa, pb)
+    assert float(pb.abs().max()) > 0
