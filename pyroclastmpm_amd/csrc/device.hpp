@@ -90,6 +90,77 @@ template <int SF> __device__ inline float spline_d2(float r, int type, float inv
   return h2 * (2.f * c[2] + 6.f * c[3] * r);
 }
 
+// Coefficients c0 + c1 r + c2 r^2 + c3 r^3 of cubic piece 3 - S for node
+// types 1-3 (the kCubic rows, chosen with selects instead of a table load).
+// Node offset S (node bin - 1 + S) of a particle with frac = t - bin in
+// [0, 1) lies on that piece: r = frac + 1 - S. Type 2 (node 1) never sits at
+// S = 3 of such a particle (it would need bin = -2).
+template <int S> __device__ inline void cubic_span_coeffs(int type, float c[4]) {
+  const bool t1 = type == 1, t2 = type == 2;
+  if (S == 0) {  // piece 3, identical for types 1-3
+    c[0] = 4.f / 3.f; c[1] = -2.f; c[2] = 1.f; c[3] = -1.f / 6.f;
+  } else if (S == 1) {  // piece 2
+    c[0] = t1 ? 1.f : 2.f / 3.f; c[1] = t1 ? -1.f : 0.f; c[2] = t1 ? 0.f : -1.f; c[3] = t1 ? 1.f / 6.f : 0.5f;
+  } else if (S == 2) {  // piece 1
+    c[0] = t1 ? 1.f : 2.f / 3.f; c[1] = t1 ? 1.f : 0.f; c[2] = t1 ? 0.f : -1.f;
+    c[3] = t1 ? -1.f / 6.f : (t2 ? -1.f / 3.f : -0.5f);
+  } else {  // piece 0, zero for type 2
+    c[0] = t2 ? 0.f : 4.f / 3.f; c[1] = t2 ? 0.f : 2.f; c[2] = t2 ? 0.f : 1.f; c[3] = t2 ? 0.f : 1.f / 6.f;
+  }
+}
+
+// Weight, derivative and (if d2N) second derivative of node offset S from
+// the span form. At frac = 1 - 2^-24 rounding puts r on the next knot: at
+// S = 0 (r = 2) the weight is 0 as in cubic_eval; at S = 3 (r = -1) both
+// pieces meet there (the spline is C2), so the values agree to rounding.
+template <int S>
+__device__ inline void cubic_span(float frac, int type, float inv_h, float &N, float &dN, float *d2N = nullptr) {
+  float c[4];
+  cubic_span_coeffs<S>(type, c);
+  const float r = __fsub_rn(frac, (float)(S - 1));
+  N = ((c[3] * r + c[2]) * r + c[1]) * r + c[0];
+  dN = inv_h * ((3.f * c[3] * r + 2.f * c[2]) * r + c[1]);
+  if (d2N) *d2N = inv_h * inv_h * (2.f * c[2] + 6.f * c[3] * r);
+  if (S == 0 && !(r < 2.f)) {
+    N = 0.f;
+    dN = 0.f;
+    if (d2N) *d2N = 0.f;
+  }
+}
+
+// The four cubic window entries of one axis (nodes bin - 1 .. bin + 2). The
+// span form serves frac in [0, 1), every particle at t >= 0; a particle left
+// of the origin (t < 0, where trunc gives frac in (-1, 0)) takes the table
+// form, whose pieces then shift by one.
+__device__ inline void cubic_axis(float frac, int bin, int n_axis, int types_built, float inv_h, float *N, float *dN,
+                                  float *d2N = nullptr) {
+  if (!types_built) {  // untyped nodes: kCubic row 0
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      N[e] = 0.f;
+      dN[e] = 0.f;
+      if (d2N) d2N[e] = 0.f;
+    }
+    return;
+  }
+  int type[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) type[e] = cubic_node_type(bin - 1 + e, n_axis, 1);
+  if (frac >= 0.f) {
+    cubic_span<0>(frac, type[0], inv_h, N[0], dN[0], d2N ? d2N + 0 : nullptr);
+    cubic_span<1>(frac, type[1], inv_h, N[1], dN[1], d2N ? d2N + 1 : nullptr);
+    cubic_span<2>(frac, type[2], inv_h, N[2], dN[2], d2N ? d2N + 2 : nullptr);
+    cubic_span<3>(frac, type[3], inv_h, N[3], dN[3], d2N ? d2N + 3 : nullptr);
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const float r = __fsub_rn(frac, (float)(e - 1));
+      cubic_eval(r, type[e], inv_h, N[e], dN[e]);
+      if (d2N) d2N[e] = spline_d2<PMPM_SF_CUBIC>(r, type[e], inv_h);
+    }
+  }
+}
+
 // t = (x - origin) * inv_h rounded after each operation, exactly as the
 // reference evaluates it (no fused multiply-add: the cell index and the
 // relative coordinates must come out bit-identical).
@@ -117,11 +188,7 @@ __device__ inline void axis_weights(float t, int bin, int n_axis, int types_buil
     for (int e = 0; e < 3; ++e) quad_eval(__fsub_rn(frac, (float)(first + e)), inv_h, N[e], dN[e]);
   } else {
     first = -1;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int o = e - 1;
-      cubic_eval(__fsub_rn(frac, (float)o), cubic_node_type(bin + o, n_axis, types_built), inv_h, N[e], dN[e]);
-    }
+    cubic_axis(frac, bin, n_axis, types_built, inv_h, N, dN);
   }
 }
 

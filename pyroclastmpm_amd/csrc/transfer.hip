@@ -86,16 +86,18 @@ template <int D, int SF> struct PW {
 template <int D, int SF>
 __device__ inline void particle_weights(const pmpm_params &prm, const pmpm_grid &g, const float *x, int n, int slot,
                                         PW<D, SF> &w, float *xp) {
+  // all coordinates in flight before the first weight
+#pragma unroll
+  for (int a = 0; a < 3; ++a) xp[a] = a < D ? x[a * n + slot] : 0.f;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     if (a < D) {
-      xp[a] = x[a * n + slot];
       w.t[a] = cell_coord(xp[a], g.origin[a], g.inv_cell_size);
       w.bin[a] = (int)w.t[a];
       axis_weights<SF>(w.t[a], w.bin[a], g.num_cells[a], prm.node_types_built, g.inv_cell_size, w.first[a], w.N[a],
                        w.dN[a]);
     } else {
-      xp[a] = 0.f; w.t[a] = 0.f; w.bin[a] = 0; w.first[a] = 0;
+      w.t[a] = 0.f; w.bin[a] = 0; w.first[a] = 0;
 #pragma unroll
       for (int e = 0; e < 4; ++e) { w.N[a][e] = 1.f; w.dN[a][e] = 0.f; }
     }
@@ -296,29 +298,11 @@ __device__ inline void scatter_entry_global(const pmpm_params &prm, const pmpm_g
 // pieces meet there (the spline is C2), so the value agrees to rounding.
 template <int SF, int S>
 __device__ inline void span_eval(float frac, int type, float inv_h, float &N, float &dN) {
-  const float r = __fsub_rn(frac, (float)(Spline<SF>::NLO + S));
   if (SF != PMPM_SF_CUBIC) {
-    spline_eval<SF>(r, 0, inv_h, N, dN);
+    spline_eval<SF>(__fsub_rn(frac, (float)(Spline<SF>::NLO + S)), 0, inv_h, N, dN);
     return;
   }
-  const bool t1 = type == 1, t2 = type == 2;
-  float c0, c1, c2, c3;
-  if (S == 0) {  // piece 3, identical for types 1-3
-    c0 = 4.f / 3.f; c1 = -2.f; c2 = 1.f; c3 = -1.f / 6.f;
-  } else if (S == 1) {  // piece 2
-    c0 = t1 ? 1.f : 2.f / 3.f; c1 = t1 ? -1.f : 0.f; c2 = t1 ? 0.f : -1.f; c3 = t1 ? 1.f / 6.f : 0.5f;
-  } else if (S == 2) {  // piece 1
-    c0 = t1 ? 1.f : 2.f / 3.f; c1 = t1 ? 1.f : 0.f; c2 = t1 ? 0.f : -1.f;
-    c3 = t1 ? -1.f / 6.f : (t2 ? -1.f / 3.f : -0.5f);
-  } else {  // piece 0, zero for type 2
-    c0 = t2 ? 0.f : 4.f / 3.f; c1 = t2 ? 0.f : 2.f; c2 = t2 ? 0.f : 1.f; c3 = t2 ? 0.f : 1.f / 6.f;
-  }
-  N = ((c3 * r + c2) * r + c1) * r + c0;
-  dN = inv_h * ((3.f * c3 * r + 2.f * c2) * r + c1);
-  if (S == 0) {
-    N = r < 2.f ? N : 0.f;
-    dN = r < 2.f ? dN : 0.f;
-  }
+  cubic_span<S>(frac, type, inv_h, N, dN);
 }
 
 // span_eval for the NS consecutive offsets S0 .. S0 + NS - 1.
@@ -2063,17 +2047,23 @@ template <int D, int SF>
 __device__ inline void window_d2(const pmpm_params &prm, const pmpm_grid &g, const PW<D, SF> &w, float d2N[3][4]) {
   constexpr int E = Spline<SF>::ENTRIES;
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
+  for (int a = 0; a < 3; ++a) {
+    const float frac = __fsub_rn(w.t[a], (float)w.bin[a]);
+    if (SF == PMPM_SF_CUBIC && a < D) {
+      float N[4], dN[4];
+      cubic_axis(frac, w.bin[a], g.num_cells[a], prm.node_types_built, g.inv_cell_size, N, dN, d2N[a]);
+      continue;
+    }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       d2N[a][e] = 0.f;
       if (a < D && e < E) {
-        const float frac = __fsub_rn(w.t[a], (float)w.bin[a]);
         const int o = w.first[a] + e;
         const int type = cubic_node_type(w.bin[a] + o, g.num_cells[a], prm.node_types_built);
         d2N[a][e] = spline_d2<SF>(__fsub_rn(frac, (float)o), type, g.inv_cell_size);
       }
     }
+  }
 }
 
 // Weight, gradient and Hessian (xx, yy, zz, xy, xz, yz) of one window entry
