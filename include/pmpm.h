@@ -53,6 +53,11 @@ enum { PMPM_PF_RIGID = 1, PMPM_PF_ACTIVE = 2 };
 #define PMPM_MAX_MATERIALS 8
 #define PMPM_MAX_BCS 8
 
+/* Largest node count of a grid (num_cells_total < 2^27): the gathers address
+ * the node velocity records (32 B per node in 3D) with 32-bit byte offsets.
+ * Entry points given a larger grid return PMPM_ERR_INVALID. */
+#define PMPM_MAX_GRID_NODES (1 << 27)
+
 /* Background grid (types_common.h:250-291 Grid). num_cells = nodes per axis. */
 typedef struct pmpm_grid {
   float origin[3];

@@ -48,6 +48,12 @@ constexpr int kMaxRadix = 1 << kMaxRadixBits;
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// A grid the kernels can address: at least one node, fewer than
+// PMPM_MAX_GRID_NODES (32-bit byte offsets into the node records).
+inline bool grid_ok(const pmpm_grid *g) {
+  return g && g->num_cells_total > 0 && g->num_cells_total < PMPM_MAX_GRID_NODES;
+}
+
 // Records the failing HIP call of the calling thread (pmpm_last_error).
 void note_hip_error(hipError_t e, const char *what);
 inline bool hip_ok(hipError_t e, const char *what) {

@@ -307,7 +307,7 @@ extern "C" int pmpm_context_out_of_grid(pmpm_context *ctx, pmpm_stream_t stream,
 
 extern "C" int pmpm_partition(pmpm_context *ctx, int32_t dim, const pmpm_grid *grid, const float *x, int32_t n,
                               pmpm_spatial *part, pmpm_stream_t stream) {
-  if (!ctx || !grid || !part || dim < 1 || dim > 3 || n < 0) return PMPM_ERR_INVALID;
+  if (!ctx || !grid_ok(grid) || !part || dim < 1 || dim > 3 || n < 0) return PMPM_ERR_INVALID;
   if (n > 0 && (!x || !part->hash || !part->hash_sorted || !part->sorted)) return PMPM_ERR_INVALID;
   if (!part->cell_start || !part->cell_end || grid->num_cells_total <= 0) return PMPM_ERR_INVALID;
   if (n > ctx->cap_particles) return PMPM_ERR_WORKSPACE;

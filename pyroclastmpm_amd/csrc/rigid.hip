@@ -156,7 +156,7 @@ extern "C" int pmpm_rigid_levelset(const pmpm_params *prm, const pmpm_grid *grid
                                    const pmpm_spatial *part, pmpm_nodes *nodes, const float *rotation,
                                    const float *com, const float *dpsi_orig, float *rigid_dx,
                                    pmpm_stream_t stream) {
-  if (!prm || !grid || !p || !part || !nodes || !rotation || !com) return PMPM_ERR_INVALID;
+  if (!prm || !grid_ok(grid) || !p || !part || !nodes || !rotation || !com) return PMPM_ERR_INVALID;
   if (prm->dim < 2 || prm->dim > 3) return PMPM_ERR_INVALID;  // the reference exits in 1D
   if (p->n > 0 && !dpsi_orig) return PMPM_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;

@@ -22,7 +22,7 @@ using namespace pmpm;
 extern "C" int pmpm_step(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid, int32_t musl,
                          const pmpm_particles *a, pmpm_particles *b, pmpm_spatial *part, pmpm_nodes *nodes,
                          pmpm_stream_t stream) {
-  if (!ctx || !prm || !grid || !a || !b || !part || !nodes) return PMPM_ERR_INVALID;
+  if (!ctx || !prm || !grid_ok(grid) || !a || !b || !part || !nodes) return PMPM_ERR_INVALID;
   if (prm->dim < 1 || prm->dim > 3 || prm->shape_fn < 0 || prm->shape_fn > 2) return PMPM_ERR_INVALID;
   if (prm->num_bcs < 0 || prm->num_bcs > PMPM_MAX_BCS) return PMPM_ERR_INVALID;
   if (a->n != b->n || a->x == b->x) return PMPM_ERR_INVALID;

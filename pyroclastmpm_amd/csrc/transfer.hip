@@ -110,21 +110,25 @@ __device__ inline void particle_weights(const pmpm_params &prm, const pmpm_grid 
 template <int D> struct NodeVel {
   static constexpr int NP = D == 3 ? 8 : 2 * D;
 };
+// The byte offset of a record is a 32-bit value (grids hold fewer than
+// PMPM_MAX_GRID_NODES nodes), so a load is the uniform base plus one VGPR.
 template <int D, bool WANT_DV>
 __device__ inline void load_node_vel(const float *__restrict__ vel, int node, float *v, float *dv) {
+  const char *base = reinterpret_cast<const char *>(vel);
+  const uint32_t at = (uint32_t)node * (uint32_t)(NodeVel<D>::NP * sizeof(float));
   if constexpr (D == 3) {
-    const float4 a = *reinterpret_cast<const float4 *>(vel + 8 * (int64_t)node);
+    const float4 a = *reinterpret_cast<const float4 *>(base + at);
     v[0] = a.x; v[1] = a.y; v[2] = a.z;
     if constexpr (WANT_DV) {
-      const float4 b = *reinterpret_cast<const float4 *>(vel + 8 * (int64_t)node + 4);
+      const float4 b = *reinterpret_cast<const float4 *>(base + at + 16u);
       dv[0] = a.w; dv[1] = b.x; dv[2] = b.y;
     }
   } else if constexpr (D == 2) {
-    const float4 a = *reinterpret_cast<const float4 *>(vel + 4 * (int64_t)node);
+    const float4 a = *reinterpret_cast<const float4 *>(base + at);
     v[0] = a.x; v[1] = a.y;
     if constexpr (WANT_DV) { dv[0] = a.z; dv[1] = a.w; }
   } else {
-    const float2 a = *reinterpret_cast<const float2 *>(vel + 2 * (int64_t)node);
+    const float2 a = *reinterpret_cast<const float2 *>(base + at);
     v[0] = a.x;
     if constexpr (WANT_DV) dv[0] = a.y;
   }
@@ -150,11 +154,12 @@ __device__ inline NodeSrc node_src(const pmpm_grid &g, const pmpm_nodes &nodes) 
   return NodeSrc{nodes.vel, {1, g.num_cells[0], g.num_cells[0] * g.num_cells[1]}};
 }
 
-// Grid gather in the reference's window order (y outer, x, z inner). The
-// reference skips entries outside the grid on any axis (usl_inline.h:281-293);
-// here such entries get zero weights and a clamped, valid node index, which
-// leaves every sum unchanged and keeps the loop free of branches. Nodes below
-// the mass cutoff carry zero velocities, which is equivalent to skipping them.
+// Grid gather (1D/2D in the reference's window order, y outer, x inner; 3D in
+// the separable order below). The reference skips entries outside the grid
+// on any axis (usl_inline.h:281-293); here such entries get zero weights and
+// a clamped, valid node index, which leaves every sum unchanged and keeps the
+// loop free of branches. Nodes below the mass cutoff carry zero velocities,
+// which is equivalent to skipping them.
 template <int D, int SF, bool WANT_V, bool WANT_L>
 __device__ inline void gather_grid(const pmpm_grid &g, const NodeSrc &src, const PW<D, SF> &w, float *vinc,
                                    float *dvinc, float *L) {
@@ -180,6 +185,62 @@ __device__ inline void gather_grid(const pmpm_grid &g, const NodeSrc &src, const
         dN[a][e] = 0.f;
       }
     }
+  }
+  if constexpr (D == 3) {
+    // Separable form: the weights are products of per-axis factors, so the
+    // sums contract x first (a row's x nodes are adjacent records), then y,
+    // then z: per row S = sum_x Nx v, Sd = sum_x dNx v; per z layer
+    // U0 = sum_y Ny Sd, U1 = sum_y dNy S, U2 = sum_y Ny S; then
+    // L[x][.] += Nz U0, L[y][.] += Nz U1, L[z][.] += dNz U2, v += Nz U2.
+    // The next row's records are loaded while the current row is summed.
+    constexpr int NR = EY * EZ;
+    float rv[2][E][3], rd[2][E][3];
+    auto load_row = [&](int r, int buf) {
+      const int row = off[1][r % EY] + off[2][r / EY];
+#pragma unroll
+      for (int ex = 0; ex < E; ++ex) load_node_vel<D, WANT_V>(src.vel, row + off[0][ex], rv[buf][ex], rd[buf][ex]);
+    };
+    load_row(0, 0);
+    float U0[3] = {0.f, 0.f, 0.f}, U1[3] = {0.f, 0.f, 0.f}, U2[3] = {0.f, 0.f, 0.f}, Ud[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int ey = r % EY, ez = r / EY, cb = r & 1;
+      if (r + 1 < NR) load_row(r + 1, cb ^ 1);
+      float Sx[3] = {0.f, 0.f, 0.f}, Sdx[3] = {0.f, 0.f, 0.f}, Sxd[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ex = 0; ex < E; ++ex)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          Sx[b] += N[0][ex] * rv[cb][ex][b];
+          if (WANT_L) Sdx[b] += dN[0][ex] * rv[cb][ex][b];
+          if (WANT_V) Sxd[b] += N[0][ex] * rd[cb][ex][b];
+        }
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        U2[b] += N[1][ey] * Sx[b];
+        if (WANT_L) {
+          U0[b] += N[1][ey] * Sdx[b];
+          U1[b] += dN[1][ey] * Sx[b];
+        }
+        if (WANT_V) Ud[b] += N[1][ey] * Sxd[b];
+      }
+      if (ey == EY - 1) {  // z layer complete
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          if (WANT_L) {
+            L[b] += N[2][ez] * U0[b];
+            L[3 + b] += N[2][ez] * U1[b];
+            L[6 + b] += dN[2][ez] * U2[b];
+          }
+          if (WANT_V) {
+            vinc[b] += N[2][ez] * U2[b];
+            dvinc[b] += N[2][ez] * Ud[b];
+          }
+          U0[b] = U1[b] = U2[b] = Ud[b] = 0.f;
+        }
+      }
+    }
+    return;
   }
 #pragma unroll
   for (int ey = 0; ey < EY; ++ey) {
@@ -1954,7 +2015,7 @@ using namespace pmpm;
 
 extern "C" int pmpm_shape_function(int32_t dim, int32_t shape_fn, int32_t node_types_built, const pmpm_grid *grid,
                                    const float *x, int32_t n, float *psi, float *dpsi, pmpm_stream_t stream) {
-  if (!grid || dim < 1 || dim > 3 || shape_fn < 0 || shape_fn > 2 || n < 0) return PMPM_ERR_INVALID;
+  if (!grid_ok(grid) || dim < 1 || dim > 3 || shape_fn < 0 || shape_fn > 2 || n < 0) return PMPM_ERR_INVALID;
   if (n == 0) return PMPM_OK;
   if (!x || !psi || !dpsi) return PMPM_ERR_INVALID;
   PMPM_DISPATCH_DSF(dim, shape_fn, launch_shape, *grid, node_types_built, x, n, psi, dpsi, (hipStream_t)stream);
@@ -1964,7 +2025,7 @@ extern "C" int pmpm_shape_function(int32_t dim, int32_t shape_fn, int32_t node_t
 
 extern "C" int pmpm_p2g(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles *p,
                         const pmpm_spatial *part, pmpm_nodes *nodes, pmpm_stream_t stream) {
-  if (!ctx || !params_ok(prm) || !grid || !particles_ok(p) || !part || !nodes) return PMPM_ERR_INVALID;
+  if (!ctx || !params_ok(prm) || !grid_ok(grid) || !particles_ok(p) || !part || !nodes) return PMPM_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
   int rc = do_p2g(ctx, *prm, *grid, *p, *part, *nodes, s);
   if (rc != PMPM_OK) return rc;
@@ -1974,14 +2035,14 @@ extern "C" int pmpm_p2g(pmpm_context *ctx, const pmpm_params *prm, const pmpm_gr
 
 extern "C" int pmpm_grid_bc(const pmpm_params *prm, const pmpm_grid *grid, pmpm_nodes *nodes, int32_t bc_begin,
                             int32_t bc_end, int32_t remap_pass, pmpm_stream_t stream) {
-  if (!params_ok(prm) || !grid || !nodes) return PMPM_ERR_INVALID;
+  if (!params_ok(prm) || !grid_ok(grid) || !nodes) return PMPM_ERR_INVALID;
   if (bc_begin < 0 || bc_end > prm->num_bcs || bc_begin > bc_end) return PMPM_ERR_INVALID;
   return launch_grid(nullptr, *prm, *grid, *nodes, 0, bc_begin, bc_end, remap_pass ? 1 : 0, 0, (hipStream_t)stream);
 }
 
 extern "C" int pmpm_g2p(const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles *in, pmpm_particles *out,
                         const pmpm_spatial *part, const pmpm_nodes *nodes, pmpm_stream_t stream) {
-  if (!params_ok(prm) || !grid || !particles_ok(in) || !particles_ok(out) || !part || !nodes) return PMPM_ERR_INVALID;
+  if (!params_ok(prm) || !grid_ok(grid) || !particles_ok(in) || !particles_ok(out) || !part || !nodes) return PMPM_ERR_INVALID;
   if (out->n != in->n) return PMPM_ERR_INVALID;
   return do_g2p(*prm, *grid, *in, *out, *part, *nodes, 0, (hipStream_t)stream);
 }
@@ -1989,7 +2050,7 @@ extern "C" int pmpm_g2p(const pmpm_params *prm, const pmpm_grid *grid, const pmp
 extern "C" int pmpm_musl_map(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid,
                              const pmpm_particles *in, pmpm_particles *out, const pmpm_spatial *part,
                              pmpm_nodes *nodes, pmpm_stream_t stream) {
-  if (!ctx || !params_ok(prm) || !grid || !particles_ok(in) || !particles_ok(out) || !part || !nodes)
+  if (!ctx || !params_ok(prm) || !grid_ok(grid) || !particles_ok(in) || !particles_ok(out) || !part || !nodes)
     return PMPM_ERR_INVALID;
   if (out->n != in->n || out->x == in->x) return PMPM_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
@@ -2002,7 +2063,7 @@ extern "C" int pmpm_musl_map(pmpm_context *ctx, const pmpm_params *prm, const pm
 extern "C" int pmpm_musl_g2p(const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles *in,
                              pmpm_particles *out, const pmpm_spatial *part, const pmpm_nodes *nodes,
                              pmpm_stream_t stream) {
-  if (!params_ok(prm) || !grid || !particles_ok(in) || !particles_ok(out) || !part || !nodes) return PMPM_ERR_INVALID;
+  if (!params_ok(prm) || !grid_ok(grid) || !particles_ok(in) || !particles_ok(out) || !part || !nodes) return PMPM_ERR_INVALID;
   if (out->n != in->n) return PMPM_ERR_INVALID;
   return do_g2p(*prm, *grid, *in, *out, *part, *nodes, 2, (hipStream_t)stream);
 }
@@ -2496,7 +2557,7 @@ extern "C" int pmpm_adjoint_step(pmpm_context *ctx, const pmpm_params *prm, cons
                                  const pmpm_particles *state, const pmpm_adjoint *out_bar, pmpm_adjoint *in_bar,
                                  float *param_bar, int32_t recompute, pmpm_spatial *part, pmpm_nodes *nodes,
                                  pmpm_stream_t stream) {
-  if (!ctx || !params_ok(prm) || !grid || !particles_ok(state) || !out_bar || !in_bar || !param_bar || !part ||
+  if (!ctx || !params_ok(prm) || !grid_ok(grid) || !particles_ok(state) || !out_bar || !in_bar || !param_bar || !part ||
       !nodes || (recompute != 0 && recompute != 1))
     return PMPM_ERR_INVALID;
   const int n = state->n;
