@@ -144,14 +144,32 @@ template <int D> __device__ inline void store_node_vel(float *__restrict__ vel, 
     *reinterpret_cast<float2 *>(vel + 2 * (int64_t)node) = make_float2(v[0], dv[0]);
   }
 }
-// Where a gather reads node velocities from: the records and the grid's node
-// strides.
+// Where a gather reads node velocities from: the global records
+// (pmpm_nodes.vel) with the grid's node strides, or a box of nodes staged in
+// LDS (box4 = v_nt and dv.x, box2 = dv.y and dv.z per node) with the box's
+// strides and first node per axis.
 struct NodeSrc {
-  const float *vel;  // pmpm_nodes.vel
-  int stride[3];     // node (i, j, k) sits at sum id_a stride_a
+  const float *vel;
+  const float4 *box4;
+  const float2 *box2;
+  int stride[3];  // node (i, j, k) sits at sum (id_a - lo_a) stride_a
+  int lo[3];
 };
 __device__ inline NodeSrc node_src(const pmpm_grid &g, const pmpm_nodes &nodes) {
-  return NodeSrc{nodes.vel, {1, g.num_cells[0], g.num_cells[0] * g.num_cells[1]}};
+  return NodeSrc{nodes.vel, nullptr, nullptr, {1, g.num_cells[0], g.num_cells[0] * g.num_cells[1]}, {0, 0, 0}};
+}
+template <int D, bool WANT_DV, bool BOX>
+__device__ inline void load_src(const NodeSrc &src, int node, float *v, float *dv) {
+  if constexpr (BOX) {
+    const float4 a = src.box4[node];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z;
+    if constexpr (WANT_DV) {
+      const float2 b = src.box2[node];
+      dv[0] = a.w; dv[1] = b.x; dv[2] = b.y;
+    }
+  } else {
+    load_node_vel<D, WANT_DV>(src.vel, node, v, dv);
+  }
 }
 
 // Grid gather (1D/2D in the reference's window order, y outer, x inner; 3D in
@@ -160,9 +178,10 @@ __device__ inline NodeSrc node_src(const pmpm_grid &g, const pmpm_nodes &nodes) 
 // a clamped, valid node index, which leaves every sum unchanged and keeps the
 // loop free of branches. Nodes below the mass cutoff carry zero velocities,
 // which is equivalent to skipping them.
-template <int D, int SF, bool WANT_V, bool WANT_L>
+template <int D, int SF, bool WANT_V, bool WANT_L, bool BOX = false>
 __device__ inline void gather_grid(const pmpm_grid &g, const NodeSrc &src, const PW<D, SF> &w, float *vinc,
                                    float *dvinc, float *L) {
+  static_assert(!BOX || D == 3, "LDS node boxes are 3D");
   constexpr int E = Spline<SF>::ENTRIES;
   constexpr int EY = D > 1 ? E : 1;
   constexpr int EZ = D > 2 ? E : 1;
@@ -176,7 +195,7 @@ __device__ inline void gather_grid(const pmpm_grid &g, const NodeSrc &src, const
       if (a < D && e < E) {
         const int i = w.bin[a] + w.first[a] + e;
         const bool ok = i >= 0 && i < g.num_cells[a];
-        off[a][e] = min(max(i, 0), g.num_cells[a] - 1) * stride;
+        off[a][e] = (min(max(i, 0), g.num_cells[a] - 1) - src.lo[a]) * stride;
         N[a][e] = ok ? w.N[a][e] : 0.f;
         dN[a][e] = ok ? w.dN[a][e] : 0.f;
       } else {
@@ -198,7 +217,7 @@ __device__ inline void gather_grid(const pmpm_grid &g, const NodeSrc &src, const
     auto load_row = [&](int r, int buf) {
       const int row = off[1][r % EY] + off[2][r / EY];
 #pragma unroll
-      for (int ex = 0; ex < E; ++ex) load_node_vel<D, WANT_V>(src.vel, row + off[0][ex], rv[buf][ex], rd[buf][ex]);
+      for (int ex = 0; ex < E; ++ex) load_src<D, WANT_V, BOX>(src, row + off[0][ex], rv[buf][ex], rd[buf][ex]);
     };
     load_row(0, 0);
     float U0[3] = {0.f, 0.f, 0.f}, U1[3] = {0.f, 0.f, 0.f}, U2[3] = {0.f, 0.f, 0.f}, Ud[3] = {0.f, 0.f, 0.f};
@@ -1676,26 +1695,110 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
 // of a 1M-particle grid fit in it, and staging a block's node box in LDS
 // measured slower on gfx950 (fewer resident waves for the same reads).
 constexpr int kG2PThreads = 256;
+// Nodes of the LDS box of one G2P workgroup (24 KB of float4 records). A
+// workgroup's 256 particles are consecutive in cell order: about ten cells of
+// one or two x rows at 27 particles per cell, so their windows span a few
+// hundred nodes; a workgroup whose box is larger (its cells cross a z layer,
+// or particles outside the grid) gathers from the global records instead.
+constexpr int kG2PBoxNodes = 1536;
+
+// Stages the node box of the workgroup's particles (bins `bin` of the valid
+// lanes, windows bin - 1 .. bin + 2 per axis, clamped to the grid) in LDS and
+// points `src` at it; returns false, with `src` unchanged, when the box holds
+// more than kG2PBoxNodes nodes or no particle. Every thread of the workgroup
+// calls it (it synchronises the workgroup).
+template <bool WANT_DV>
+__device__ inline bool stage_node_box(const pmpm_grid &g, const pmpm_nodes &nodes, const int *bin, bool valid,
+                                      float4 *box4, float2 *box2, int *s_red, NodeSrc &src) {
+  constexpr int NW = kG2PThreads / 64;
+  int mn[3], mx[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    mn[a] = wave_min(valid ? bin[a] : INT_MAX);
+    mx[a] = wave_max(valid ? bin[a] : INT_MIN);
+  }
+  const int wv = (int)threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      s_red[wv * 6 + a] = mn[a];
+      s_red[wv * 6 + 3 + a] = mx[a];
+    }
+  }
+  __syncthreads();
+  int lo[3], ext[3];
+  int64_t total = 1;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    int m0 = INT_MAX, m1 = INT_MIN;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      m0 = min(m0, s_red[k * 6 + a]);
+      m1 = max(m1, s_red[k * 6 + 3 + a]);
+    }
+    if (m0 > m1) return false;  // no valid particle
+    // int64: bins of particles far outside the grid must not overflow
+    lo[a] = (int)max((int64_t)m0 - 1, (int64_t)0);
+    const int hi = (int)min((int64_t)m1 + 2, (int64_t)g.num_cells[a] - 1);
+    if (lo[a] > hi) return false;
+    ext[a] = hi - lo[a] + 1;
+    total *= ext[a];
+  }
+  if (total > kG2PBoxNodes) return false;
+  const int nb = (int)total, exy = ext[0] * ext[1];
+  const float *vel = nodes.vel;
+  for (int k = (int)threadIdx.x; k < nb; k += kG2PThreads) {
+    const int kz = k / exy, r = k - kz * exy, ky = r / ext[0], kx = r - ky * ext[0];
+    const uint32_t node = (uint32_t)(lo[0] + kx) +
+                          (uint32_t)g.num_cells[0] * ((uint32_t)(lo[1] + ky) + (uint32_t)g.num_cells[1] * (uint32_t)(lo[2] + kz));
+    const float4 *rec = reinterpret_cast<const float4 *>(vel + 8 * (size_t)node);
+    box4[k] = rec[0];
+    if (WANT_DV) {
+      const float4 b = rec[1];
+      box2[k] = make_float2(b.x, b.y);
+    }
+  }
+  __syncthreads();
+  src.box4 = box4;
+  src.box2 = box2;
+  src.stride[0] = 1;
+  src.stride[1] = ext[0];
+  src.stride[2] = exy;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) src.lo[a] = lo[a];
+  return true;
+}
 
 template <int D, int SF, int MODE>
 __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles in,
                                                           pmpm_particles out, pmpm_spatial part, pmpm_nodes nodes,
                                                           int reorder) {
+  static_assert(MODE == 0 || MODE == 2, "G2P modes: USL (0), MUSL second map (2)");
   const int n = in.n;
   const int j = blockIdx.x * kG2PThreads + threadIdx.x;
-  if (j >= n) return;
+  const bool valid = j < n;
+  if (D != 3 && !valid) return;  // the 3D workgroup synchronises below
   const int nn = g.num_cells_total;
-  const int src = part.sorted[j];
+  const int src = valid ? part.sorted[j] : 0;
   const int dst = reorder ? j : src;
   PW<D, SF> w;
   float xp[3];
-  particle_weights<D, SF>(prm, g, in.x, n, src, w, xp);
+  if (valid) particle_weights<D, SF>(prm, g, in.x, n, src, w, xp);
   float vinc[3] = {0.f, 0.f, 0.f}, dvinc[3] = {0.f, 0.f, 0.f};
   float L[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  const NodeSrc ns = node_src(g, nodes);
-  static_assert(MODE == 0 || MODE == 2, "G2P modes: USL (0), MUSL second map (2)");
-  if (MODE == 0) gather_grid<D, SF, true, true>(g, ns, w, vinc, dvinc, L);
-  else gather_grid<D, SF, false, true>(g, ns, w, vinc, dvinc, L);
+  NodeSrc ns = node_src(g, nodes);
+  if constexpr (D == 3) {
+    __shared__ float4 box4[kG2PBoxNodes];
+    __shared__ float2 box2[MODE == 0 ? kG2PBoxNodes : 1];
+    __shared__ int s_red[kG2PThreads / 64 * 6];
+    const int bins[3] = {valid ? w.bin[0] : 0, valid ? w.bin[1] : 0, valid ? w.bin[2] : 0};
+    const bool boxed = stage_node_box<MODE == 0>(g, nodes, bins, valid, box4, box2, s_red, ns);
+    if (!valid) return;
+    if (boxed) gather_grid<D, SF, MODE == 0, true, true>(g, ns, w, vinc, dvinc, L);
+    else gather_grid<D, SF, MODE == 0, true, false>(g, ns, w, vinc, dvinc, L);
+  } else {
+    gather_grid<D, SF, MODE == 0, true>(g, ns, w, vinc, dvinc, L);
+  }
   if (MODE == 0) {
     const float am = (float)(1.0 - (double)prm.alpha);
 #pragma unroll
