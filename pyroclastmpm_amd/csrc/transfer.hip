@@ -145,14 +145,16 @@ template <int D> __device__ inline void store_node_vel(float *__restrict__ vel, 
   }
 }
 // Where a gather reads node velocities from: the global records
-// (pmpm_nodes.vel) with the grid's node strides, or a box of nodes staged in
-// LDS (box4 = v_nt and dv.x, box2 = dv.y and dv.z per node) with the box's
-// strides and first node per axis.
+// (pmpm_nodes.vel) with the grid's node strides (node (i, j, k) at
+// sum id_a stride_a), or a run's node rows staged in LDS (box4 = v_nt and
+// dv.x, box2 = dv.y and dv.z per node): window entry (x, sy, sz) of a
+// particle at x bin b sits at (b - 1 + x - lo[0]) + sy stride[1] + sz stride[2]
+// with sy, sz = 0..3 the entry's node offset + 1 along y and z.
 struct NodeSrc {
   const float *vel;
   const float4 *box4;
   const float2 *box2;
-  int stride[3];  // node (i, j, k) sits at sum (id_a - lo_a) stride_a
+  int stride[3];
   int lo[3];
 };
 __device__ inline NodeSrc node_src(const pmpm_grid &g, const pmpm_nodes &nodes) {
@@ -195,7 +197,8 @@ __device__ inline void gather_grid(const pmpm_grid &g, const NodeSrc &src, const
       if (a < D && e < E) {
         const int i = w.bin[a] + w.first[a] + e;
         const bool ok = i >= 0 && i < g.num_cells[a];
-        off[a][e] = (min(max(i, 0), g.num_cells[a] - 1) - src.lo[a]) * stride;
+        if (BOX) off[a][e] = (a == 0 ? i - src.lo[0] : w.first[a] + 1 + e) * stride;
+        else off[a][e] = min(max(i, 0), g.num_cells[a] - 1) * stride;
         N[a][e] = ok ? w.N[a][e] : 0.f;
         dN[a][e] = ok ? w.dN[a][e] : 0.f;
       } else {
@@ -1695,78 +1698,123 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
 // of a 1M-particle grid fit in it, and staging a block's node box in LDS
 // measured slower on gfx950 (fewer resident waves for the same reads).
 constexpr int kG2PThreads = 256;
-// Nodes of the LDS box of one G2P workgroup (24 KB of float4 records). A
-// workgroup's 256 particles are consecutive in cell order: about ten cells of
-// one or two x rows at 27 particles per cell, so their windows span a few
-// hundred nodes; a workgroup whose box is larger (its cells cross a z layer,
-// or particles outside the grid) gathers from the global records instead.
-constexpr int kG2PBoxNodes = 1536;
+// LDS node staging of the 3D G2P. A workgroup's 256 particles are
+// consecutive in cell order, so they fall into a few runs: consecutive
+// particles with the same y and z bins whose x bins do not decrease and stay
+// within one 64-aligned block. A run's windows lie in 4 x 4 node rows (y, z
+// bins - 1 .. + 2) over its x range - 1 .. + 2: at most 16 x 67 nodes. The
+// runs are staged in order, as many per round as kG2PStageNodes holds (one
+// round for the usual ten cells of one or two x rows at 27 particles per
+// cell), and every particle gathers from LDS. A staged node is the clamped
+// grid node its slot names: entries outside the grid have zero weight and
+// read a finite value.
+constexpr int kG2PStageNodes = 1536;
+struct G2PRunTable {
+  int y[kG2PThreads], z[kG2PThreads], x0[kG2PThreads], x1[kG2PThreads], base[kG2PThreads];
+  int wave_sum[kG2PThreads / 64];
+};
+template <bool WANT_DV> constexpr int g2p_stage_bytes() {
+  return kG2PStageNodes * (16 + (WANT_DV ? 8 : 0)) + (int)sizeof(G2PRunTable);
+}
 
-// Stages the node box of the workgroup's particles (bins `bin` of the valid
-// lanes, windows bin - 1 .. bin + 2 per axis, clamped to the grid) in LDS and
-// points `src` at it; returns false, with `src` unchanged, when the box holds
-// more than kG2PBoxNodes nodes or no particle. Every thread of the workgroup
-// calls it (it synchronises the workgroup).
-template <bool WANT_DV>
-__device__ inline bool stage_node_box(const pmpm_grid &g, const pmpm_nodes &nodes, const int *bin, bool valid,
-                                      float4 *box4, float2 *box2, int *s_red, NodeSrc &src) {
-  constexpr int NW = kG2PThreads / 64;
-  int mn[3], mx[3];
+// Inclusive sum of v over the workgroup's threads in order; the total goes to
+// *total. Synchronises the workgroup (wave_sum is rewritten on every call: a
+// workgroup barrier must separate two calls).
+__device__ inline int block_inclusive_sum(int v, int *wave_sum, int *total) {
+  const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    mn[a] = wave_min(valid ? bin[a] : INT_MAX);
-    mx[a] = wave_max(valid ? bin[a] : INT_MIN);
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
   }
-  const int wv = (int)threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
+  if (lane == 63) wave_sum[wv] = v;
+  __syncthreads();
+  int before = 0, all = 0;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      s_red[wv * 6 + a] = mn[a];
-      s_red[wv * 6 + 3 + a] = mx[a];
-    }
+  for (int k = 0; k < kG2PThreads / 64; ++k) {
+    const int c = wave_sum[k];
+    before += k < wv ? c : 0;
+    all += c;
+  }
+  *total = all;
+  return v + before;
+}
+
+// Gathers v_nt (and dv with WANT_DV) and L of the particles of lanes
+// 0 .. nv - 1 (weights w) through the LDS staging `lds`
+// (g2p_stage_bytes<WANT_DV>() bytes). Every thread of the workgroup calls it.
+template <int D, int SF, bool WANT_DV>
+__device__ inline void g2p_gather_staged(const pmpm_grid &g, const pmpm_nodes &nodes, const PW<D, SF> &w, int nv,
+                                         char *lds, float *vinc, float *dvinc, float *L) {
+  static_assert(D == 3, "3D staging");
+  constexpr int CAP = kG2PStageNodes;
+  float4 *box4 = reinterpret_cast<float4 *>(lds);
+  float2 *box2 = reinterpret_cast<float2 *>(lds + CAP * 16);
+  G2PRunTable &R = *reinterpret_cast<G2PRunTable *>(lds + CAP * (16 + (WANT_DV ? 8 : 0)));
+  int *rb = reinterpret_cast<int *>(lds);  // lane bins [3][256], in box4's space before staging
+  const int t = (int)threadIdx.x;
+  const bool valid = t < nv;
+  if (valid) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) rb[a * kG2PThreads + t] = w.bin[a];
   }
   __syncthreads();
-  int lo[3], ext[3];
-  int64_t total = 1;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    int m0 = INT_MAX, m1 = INT_MIN;
-#pragma unroll
-    for (int k = 0; k < NW; ++k) {
-      m0 = min(m0, s_red[k * 6 + a]);
-      m1 = max(m1, s_red[k * 6 + 3 + a]);
-    }
-    if (m0 > m1) return false;  // no valid particle
-    // int64: bins of particles far outside the grid must not overflow
-    lo[a] = (int)max((int64_t)m0 - 1, (int64_t)0);
-    const int hi = (int)min((int64_t)m1 + 2, (int64_t)g.num_cells[a] - 1);
-    if (lo[a] > hi) return false;
-    ext[a] = hi - lo[a] + 1;
-    total *= ext[a];
+  // does a run start at lane q (after lane q - 1)?
+  auto starts_at = [&](int q) {
+    const int p0 = rb[q - 1], q0 = rb[q];
+    return rb[kG2PThreads + q - 1] != rb[kG2PThreads + q] || rb[2 * kG2PThreads + q - 1] != rb[2 * kG2PThreads + q] ||
+           q0 < p0 || (q0 >> 6) != (p0 >> 6);
+  };
+  bool start = false, end = false;
+  if (valid) {
+    start = t == 0 || starts_at(t);
+    end = t == nv - 1 || starts_at(t + 1);
   }
-  if (total > kG2PBoxNodes) return false;
-  const int nb = (int)total, exy = ext[0] * ext[1];
-  const float *vel = nodes.vel;
-  for (int k = (int)threadIdx.x; k < nb; k += kG2PThreads) {
-    const int kz = k / exy, r = k - kz * exy, ky = r / ext[0], kx = r - ky * ext[0];
-    const uint32_t node = (uint32_t)(lo[0] + kx) +
-                          (uint32_t)g.num_cells[0] * ((uint32_t)(lo[1] + ky) + (uint32_t)g.num_cells[1] * (uint32_t)(lo[2] + kz));
-    const float4 *rec = reinterpret_cast<const float4 *>(vel + 8 * (size_t)node);
-    box4[k] = rec[0];
-    if (WANT_DV) {
-      const float4 b = rec[1];
-      box2[k] = make_float2(b.x, b.y);
-    }
+  int nruns = 0;
+  const int k = block_inclusive_sum(start ? 1 : 0, R.wave_sum, &nruns) - 1;
+  if (start) {
+    R.y[k] = w.bin[1];
+    R.z[k] = w.bin[2];
+    R.x0[k] = w.bin[0];
   }
+  if (end) R.x1[k] = w.bin[0];
   __syncthreads();
-  src.box4 = box4;
-  src.box2 = box2;
-  src.stride[0] = 1;
-  src.stride[1] = ext[0];
-  src.stride[2] = exy;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) src.lo[a] = lo[a];
-  return true;
+  const int cnt = t < nruns ? 16 * (R.x1[t] - R.x0[t] + 4) : 0;
+  int all = 0;
+  const int incl = block_inclusive_sum(cnt, R.wave_sum, &all);
+  if (t < nruns) R.base[t] = incl - cnt;
+  __syncthreads();
+  const int myk = valid ? k : -1;
+  for (int k0 = 0; k0 < nruns;) {
+    const int b0 = R.base[k0];
+    int k1 = k0 + 1;
+    while (k1 < nruns && R.base[k1] + 16 * (R.x1[k1] - R.x0[k1] + 4) - b0 <= CAP) ++k1;
+    const int tot = R.base[k1 - 1] + 16 * (R.x1[k1 - 1] - R.x0[k1 - 1] + 4) - b0;
+    for (int q = t; q < tot; q += kG2PThreads) {
+      int kk = k0;
+      while (kk + 1 < k1 && R.base[kk + 1] - b0 <= q) ++kk;
+      const int X = R.x1[kk] - R.x0[kk] + 4, qq = q - (R.base[kk] - b0);
+      const int row = qq / X, x = qq - row * X;  // row = sz * 4 + sy
+      const int ix = min(max(R.x0[kk] - 1 + x, 0), g.num_cells[0] - 1);
+      const int iy = min(max(R.y[kk] - 1 + (row & 3), 0), g.num_cells[1] - 1);
+      const int iz = min(max(R.z[kk] - 1 + (row >> 2), 0), g.num_cells[2] - 1);
+      const uint32_t node = (uint32_t)ix + (uint32_t)g.num_cells[0] * ((uint32_t)iy + (uint32_t)g.num_cells[1] * (uint32_t)iz);
+      const float4 *rec = reinterpret_cast<const float4 *>(nodes.vel + 8 * (size_t)node);
+      box4[q] = rec[0];
+      if (WANT_DV) {
+        const float4 b = rec[1];
+        box2[q] = make_float2(b.x, b.y);
+      }
+    }
+    __syncthreads();
+    if (myk >= k0 && myk < k1) {
+      const int off = R.base[myk] - b0, X = R.x1[myk] - R.x0[myk] + 4;
+      NodeSrc src{nodes.vel, box4 + off, box2 + off, {1, X, 4 * X}, {R.x0[myk] - 1, 0, 0}};
+      gather_grid<D, SF, WANT_DV, true, true>(g, src, w, vinc, dvinc, L);
+    }
+    k0 = k1;
+    __syncthreads();  // the next round overwrites the staging
+  }
 }
 
 template <int D, int SF, int MODE>
@@ -1786,18 +1834,13 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(pmpm_params prm, pmpm_
   if (valid) particle_weights<D, SF>(prm, g, in.x, n, src, w, xp);
   float vinc[3] = {0.f, 0.f, 0.f}, dvinc[3] = {0.f, 0.f, 0.f};
   float L[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  NodeSrc ns = node_src(g, nodes);
   if constexpr (D == 3) {
-    __shared__ float4 box4[kG2PBoxNodes];
-    __shared__ float2 box2[MODE == 0 ? kG2PBoxNodes : 1];
-    __shared__ int s_red[kG2PThreads / 64 * 6];
-    const int bins[3] = {valid ? w.bin[0] : 0, valid ? w.bin[1] : 0, valid ? w.bin[2] : 0};
-    const bool boxed = stage_node_box<MODE == 0>(g, nodes, bins, valid, box4, box2, s_red, ns);
+    __shared__ __attribute__((aligned(16))) char lds[g2p_stage_bytes<MODE == 0>()];
+    const int nv = min(n - (int)blockIdx.x * kG2PThreads, kG2PThreads);
+    g2p_gather_staged<D, SF, MODE == 0>(g, nodes, w, nv, lds, vinc, dvinc, L);
     if (!valid) return;
-    if (boxed) gather_grid<D, SF, MODE == 0, true, true>(g, ns, w, vinc, dvinc, L);
-    else gather_grid<D, SF, MODE == 0, true, false>(g, ns, w, vinc, dvinc, L);
   } else {
-    gather_grid<D, SF, MODE == 0, true>(g, ns, w, vinc, dvinc, L);
+    gather_grid<D, SF, MODE == 0, true>(g, node_src(g, nodes), w, vinc, dvinc, L);
   }
   if (MODE == 0) {
     const float am = (float)(1.0 - (double)prm.alpha);
