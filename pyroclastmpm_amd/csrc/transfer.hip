@@ -881,7 +881,7 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
       for (int a = 0; a < D; ++a) {
         const float ta = cell_coord(cur.x[a], g.origin[a], inv_h);
         const int ba = (int)ta;
-        own = own && ba == c[a];
+        own = own && ba == c[a] && ta >= 0.f;  // (t < 0: table form, stray path)
         frac[a] = __fsub_rn(ta, (float)ba);
       }
       if (own) {
@@ -920,7 +920,7 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
         for (int a = 0; a < D; ++a) {
           const float ta = cell_coord(in.x[a * n + slot], g.origin[a], inv_h);
           b[a] = (int)ta;
-          ing = ing && b[a] >= 0 && b[a] < g.num_cells[a];
+          ing = ing && ta >= 0.f && b[a] < g.num_cells[a];  // (t < 0: stray path)
           // (MODE 1 keeps the position itself: the first map moves it)
           st[a * CW + q] = MODE == 1 ? in.x[a * n + slot] : __fsub_rn(ta, (float)b[a]);
         }
@@ -1239,7 +1239,7 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
           x[a] = in.x[a * n + slot];
           const float ta = cell_coord(x[a], g.origin[a], inv_h);
           const int ba = (int)ta;
-          own = own && ba == (a == 0 ? cx : (a == 1 ? cy : cz));
+          own = own && ta >= 0.f && ba == (a == 0 ? cx : (a == 1 ? cy : cz));
           fr[a] = __fsub_rn(ta, (float)ba);
         }
         const float m = in.mass[slot];
@@ -1421,7 +1421,7 @@ __global__ __launch_bounds__(64) void p2g_wrap_kernel(pmpm_params prm, pmpm_grid
       for (int a = 0; a < D; ++a) {
         const float ta = cell_coord(in.x[a * n + slot], g.origin[a], inv_h);
         const int ba = (int)ta;
-        own = own && ba == c[a];
+        own = own && ta >= 0.f && ba == c[a];
         s_frac[a][lane] = __fsub_rn(ta, (float)ba);
       }
       s_own[lane] = own;

@@ -189,6 +189,29 @@ def test_column_dp_usl_quadratic_parity(oracle_mod):
     _compare_run(oracle_mod, spec, 10, 1e-4, "column DP USL quadratic")
 
 
+def _scattered_3d(musl, seed=11):
+    """Sparse random particles (about 2.4 per cell, so a G2P workgroup's
+    particles span a dozen cell rows and its node rows are staged in several
+    rounds), including particles up to one cell left of the origin on x and y
+    (truncation puts them in cell 0 with a negative fraction: table-form
+    weights, the stray path of P2G)."""
+    rng = np.random.default_rng(seed)
+    n = 2400
+    pos = rng.uniform(0.02, 0.98, size=(n, 3))
+    pos[:24, 0] = rng.uniform(-0.09, -0.001, 24)
+    pos[24:40, 1] = rng.uniform(-0.09, -0.001, 16)
+    vel = rng.normal(0.0, 0.2, size=(n, 3))
+    return S.Spec(dim=3, origin=[0.0, 0.0, 0.0], end=[1.0, 1.0, 1.0], cell_size=0.1, shape_function="cubic",
+                  dt=1e-4, ppc=2, positions=pos, velocities=vel,
+                  materials=[("LinearElastic", {"density": 1000.0, "E": 1e5, "pois": 0.3})],
+                  bcs=[("Gravity", {"gravity": [0.0, 0.0, -9.81]})], musl=musl)
+
+
+@pytest.mark.parametrize("musl", [False, True])
+def test_scattered_3d_cubic_parity(oracle_mod, musl):
+    _compare_run(oracle_mod, _scattered_3d(musl), 5, 1e-4, f"scattered 3D cubic LE {'MUSL' if musl else 'USL'}")
+
+
 def test_scoop_mui_sdf_parity(oracle_mod):
     spec = S.scoop(h=0.05, per_axis=2, dt=1e-4)
     _compare_run(oracle_mod, spec, 5, 1e-4, "scoop mu(I) quadratic + SDF box")
