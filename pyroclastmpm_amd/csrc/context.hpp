@@ -93,5 +93,8 @@ struct ZeroLists {  // buffers of 32-bit words to clear in one launch
   }
 };
 int zero_fill_lists(const ZeroLists &z, hipStream_t s);
+// pmpm_partition whose first kernel also clears the buffers of `extra`.
+int partition_clearing(pmpm_context *ctx, int dim, const pmpm_grid *grid, const float *x, int n, pmpm_spatial *part,
+                       hipStream_t s, const ZeroLists &extra);
 
 }  // namespace pmpm

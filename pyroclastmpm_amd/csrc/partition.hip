@@ -52,12 +52,22 @@ int zero_fill_lists(const ZeroLists &z, hipStream_t s) {
 
 // bins = trunc((x - origin) * inv_h) per axis (int cast truncates toward 0, as
 // Eigen's cast<int>), hash = x + y*nx + z*nx*ny. Hashes outside
-// [0, num_cells_total) are counted and clamped to num_cells_total so they sort
-// last and belong to no cell.
+// [0, num_cells_total) are clamped to num_cells_total so they sort last and
+// belong to no cell. The same launch clears the cell ranges (and the `clear`
+// buffers) for the range pass that follows.
 template <int D>
 __global__ void hash_kernel(const float *__restrict__ x, int n, pmpm_grid g, uint32_t *__restrict__ hash,
-                            int32_t *__restrict__ bins, uint32_t *__restrict__ out_of_grid) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+                            int32_t *__restrict__ bins, int32_t *__restrict__ cstart, int32_t *__restrict__ cend,
+                            ZeroLists clear) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  for (int64_t k = t; k < g.num_cells_total; k += stride) {
+    cstart[k] = 0;
+    cend[k] = 0;
+  }
+  for (int q = 0; q < clear.count; ++q)
+    for (int64_t k = t; k < clear.n[q]; k += stride) clear.p[q][k] = 0u;
+  const int i = (int)t;
   if (i >= n) return;
   int b[3] = {0, 0, 0};
 #pragma unroll
@@ -68,13 +78,7 @@ __global__ void hash_kernel(const float *__restrict__ x, int n, pmpm_grid g, uin
   int64_t h = b[0];
   if (D > 1) h += (int64_t)b[1] * g.num_cells[0];
   if (D > 2) h += (int64_t)b[2] * g.num_cells[0] * g.num_cells[1];
-  uint32_t key;
-  if (h < 0 || h >= g.num_cells_total) {
-    key = (uint32_t)g.num_cells_total;
-    atomicAdd(out_of_grid, 1u);
-  } else {
-    key = (uint32_t)h;
-  }
+  const uint32_t key = (h < 0 || h >= g.num_cells_total) ? (uint32_t)g.num_cells_total : (uint32_t)h;
   hash[i] = key;
   if (bins) {
 #pragma unroll
@@ -203,12 +207,19 @@ __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
 
 // Cell ranges over the sorted hashes (spatialpartition_inline.h:85-115);
 // empty cells keep the 0 of the preceding clear (spatialpartition.cpp:81-82).
+// The out-of-grid particles sort last: the first of them (or the last
+// particle, when there are none) writes their count.
 __global__ void cell_range_kernel(const uint32_t *__restrict__ hs, int n, uint32_t ncells,
-                                  int32_t *__restrict__ cstart, int32_t *__restrict__ cend) {
+                                  int32_t *__restrict__ cstart, int32_t *__restrict__ cend,
+                                  uint32_t *__restrict__ out_of_grid) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t h = hs[j];
-  if (h >= ncells) return;
+  if (h >= ncells) {
+    if (j == 0 || hs[j - 1] < ncells) *out_of_grid = (uint32_t)(n - j);
+    return;
+  }
+  if (j == n - 1) *out_of_grid = 0u;
   if (j == 0 || hs[j - 1] != h) cstart[h] = j;
   if (j == n - 1 || hs[j + 1] != h) cend[h] = j + 1;
 }
@@ -305,27 +316,33 @@ extern "C" int pmpm_context_out_of_grid(pmpm_context *ctx, pmpm_stream_t stream,
   return PMPM_OK;
 }
 
-extern "C" int pmpm_partition(pmpm_context *ctx, int32_t dim, const pmpm_grid *grid, const float *x, int32_t n,
-                              pmpm_spatial *part, pmpm_stream_t stream) {
+int pmpm::partition_clearing(pmpm_context *ctx, int dim, const pmpm_grid *grid, const float *x, int n,
+                             pmpm_spatial *part, hipStream_t s, const ZeroLists &extra) {
   if (!ctx || !grid_ok(grid) || !part || dim < 1 || dim > 3 || n < 0) return PMPM_ERR_INVALID;
   if (n > 0 && (!x || !part->hash || !part->hash_sorted || !part->sorted)) return PMPM_ERR_INVALID;
   if (!part->cell_start || !part->cell_end || grid->num_cells_total <= 0) return PMPM_ERR_INVALID;
   if (n > ctx->cap_particles) return PMPM_ERR_WORKSPACE;
-  hipStream_t s = (hipStream_t)stream;
   const pmpm_grid g = *grid;
   const uint32_t ncells = (uint32_t)g.num_cells_total;
   int rc;
-  ZeroLists z{};
-  z.add(ctx->counters, 1);
-  z.add(part->cell_start, ncells);
-  z.add(part->cell_end, ncells);
-  if ((rc = zero_fill_lists(z, s)) != PMPM_OK) return rc;
-  if (n == 0) return PMPM_OK;
-  const unsigned blocks = (unsigned)ceil_div(n, 256);
+  if (n == 0) {
+    ZeroLists z = extra;
+    z.add(ctx->counters, 1);
+    z.add(part->cell_start, ncells);
+    z.add(part->cell_end, ncells);
+    return zero_fill_lists(z, s);
+  }
+  // enough threads for the particles, and for the clears at 4 words each
+  int64_t words = ncells;
+  for (int q = 0; q < extra.count; ++q) words = extra.n[q] > words ? extra.n[q] : words;
+  int64_t nb = ceil_div(n, 256);
+  const int64_t nb_clear = ceil_div(words, 256 * 4);
+  nb = nb_clear > nb ? (nb_clear < 16384 ? nb_clear : (nb > 16384 ? nb : 16384)) : nb;
+  const unsigned blocks = (unsigned)nb;
   switch (dim) {
-    case 1: hipLaunchKernelGGL(hash_kernel<1>, dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, ctx->counters); break;
-    case 2: hipLaunchKernelGGL(hash_kernel<2>, dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, ctx->counters); break;
-    default: hipLaunchKernelGGL(hash_kernel<3>, dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, ctx->counters); break;
+    case 1: hipLaunchKernelGGL(hash_kernel<1>, dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, part->cell_start, part->cell_end, extra); break;
+    case 2: hipLaunchKernelGGL(hash_kernel<2>, dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, part->cell_start, part->cell_end, extra); break;
+    default: hipLaunchKernelGGL(hash_kernel<3>, dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, part->cell_start, part->cell_end, extra); break;
   }
   PMPM_LAUNCH_CHECK();
   // keys are in [0, ncells] (ncells = out-of-grid bucket)
@@ -365,8 +382,13 @@ extern "C" int pmpm_partition(pmpm_context *ctx, int32_t dim, const pmpm_grid *g
     kin = kout;
     vin = vout;
   }
-  hipLaunchKernelGGL(cell_range_kernel, dim3(blocks), dim3(256), 0, s, part->hash_sorted, n, ncells, part->cell_start,
-                     part->cell_end);
+  hipLaunchKernelGGL(cell_range_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, part->hash_sorted, n, ncells,
+                     part->cell_start, part->cell_end, ctx->counters);
   PMPM_LAUNCH_CHECK();
   return PMPM_OK;
+}
+
+extern "C" int pmpm_partition(pmpm_context *ctx, int32_t dim, const pmpm_grid *grid, const float *x, int32_t n,
+                              pmpm_spatial *part, pmpm_stream_t stream) {
+  return partition_clearing(ctx, dim, grid, x, n, part, (hipStream_t)stream, ZeroLists{});
 }

@@ -8,13 +8,14 @@
 
 namespace pmpm {
 int do_p2g(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &p,
-           const pmpm_spatial &part, const pmpm_nodes &nodes, hipStream_t s);
+           const pmpm_spatial &part, const pmpm_nodes &nodes, hipStream_t s, bool zeroed);
 int do_g2p(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
            const pmpm_spatial &part, const pmpm_nodes &nodes, int mode, hipStream_t s);
 int do_musl_map(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in,
-                const pmpm_particles &out, const pmpm_spatial &part, const pmpm_nodes &nodes, hipStream_t s);
+                const pmpm_particles &out, const pmpm_spatial &part, const pmpm_nodes &nodes, hipStream_t s,
+                bool zeroed);
 int launch_grid(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const pmpm_nodes &nodes, int finalize,
-                int b0, int b1, int remap, int gather, hipStream_t s);
+                int b0, int b1, int remap, int gather, hipStream_t s, int clear_mom_nt);
 }  // namespace pmpm
 
 using namespace pmpm;
@@ -27,13 +28,20 @@ extern "C" int pmpm_step(pmpm_context *ctx, const pmpm_params *prm, const pmpm_g
   if (prm->num_bcs < 0 || prm->num_bcs > PMPM_MAX_BCS) return PMPM_ERR_INVALID;
   if (a->n != b->n || a->x == b->x) return PMPM_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
-  int rc = pmpm_partition(ctx, prm->dim, grid, a->x, a->n, part, stream);
+  // the partition's first kernel also clears the node arrays P2G sums into,
+  // and in MUSL the first grid pass leaves mom_nt cleared for the re-map
+  const int64_t nn = grid->num_cells_total;
+  ZeroLists z{};
+  z.add(nodes->mass, nn);
+  z.add(nodes->mom, nn * prm->dim);
+  z.add(nodes->mom_nt, nn * prm->dim);
+  int rc = partition_clearing(ctx, prm->dim, grid, a->x, a->n, part, s, z);
   if (rc != PMPM_OK) return rc;
-  if ((rc = do_p2g(ctx, *prm, *grid, *a, *part, *nodes, s)) != PMPM_OK) return rc;
-  if ((rc = launch_grid(ctx, *prm, *grid, *nodes, 1, 0, prm->num_bcs, 0, 1, s)) != PMPM_OK) return rc;
+  if ((rc = do_p2g(ctx, *prm, *grid, *a, *part, *nodes, s, true)) != PMPM_OK) return rc;
+  if ((rc = launch_grid(ctx, *prm, *grid, *nodes, 1, 0, prm->num_bcs, 0, 1, s, musl ? 1 : 0)) != PMPM_OK) return rc;
   if (!musl) return do_g2p(*prm, *grid, *a, *b, *part, *nodes, 0, s);
-  if ((rc = do_musl_map(ctx, *prm, *grid, *a, *b, *part, *nodes, s)) != PMPM_OK) return rc;
-  if ((rc = launch_grid(ctx, *prm, *grid, *nodes, 0, 0, prm->num_bcs, 1, 2, s)) != PMPM_OK) return rc;
+  if ((rc = do_musl_map(ctx, *prm, *grid, *a, *b, *part, *nodes, s, true)) != PMPM_OK) return rc;
+  if ((rc = launch_grid(ctx, *prm, *grid, *nodes, 0, 0, prm->num_bcs, 1, 2, s, 0)) != PMPM_OK) return rc;
   return do_g2p(*prm, *grid, *a, *b, *part, *nodes, 2, s);
 }
 

@@ -1534,10 +1534,13 @@ __device__ inline void gather_partials(const TileGeom &tg, const float *partials
 // [bc_begin, bc_end) in list order, then the node velocities G2P reads.
 // remap_pass: mom_nt holds the MUSL re-mapped momentum; only kinematic
 // conditions apply (gravity already acted through the particle velocities).
+// clear_mom_nt: mom_nt is left at zero instead of the forward moment (the
+// MUSL step's first pass: the re-map accumulates into it next, and nothing
+// reads the forward moment but the velocity records written here).
 template <int D>
 __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int finalize, int bc_begin, int bc_end,
                             int remap_pass, TileGeom tg, const float *__restrict__ partials,
-                            const int32_t *__restrict__ flags) {
+                            const int32_t *__restrict__ flags, int clear_mom_nt) {
   const int nn = g.num_cells_total;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nn) return;
@@ -1677,7 +1680,7 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
 #pragma unroll
   for (int a = 0; a < D; ++a) {
     nodes.mom[a * nn + i] = P[a];
-    nodes.mom_nt[a * nn + i] = Q[a];
+    nodes.mom_nt[a * nn + i] = clear_mom_nt ? 0.f : Q[a];
   }
   const bool live = m > prm.small_mass_cutoff;
   float vnt[3] = {0.f, 0.f, 0.f}, dv[3] = {0.f, 0.f, 0.f};
@@ -2074,15 +2077,15 @@ void launch_shape(const pmpm_grid &g, int types_built, const float *x, int n, fl
 // gather: 0 = node arrays only, 1 = add the P2G tile partials, 2 = add the
 // MUSL re-map tile partials (ctx holds them).
 int launch_grid(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const pmpm_nodes &nodes, int finalize,
-                int b0, int b1, int remap, int gather, hipStream_t s) {
+                int b0, int b1, int remap, int gather, hipStream_t s, int clear_mom_nt = 0) {
   const unsigned blocks = (unsigned)ceil_div(g.num_cells_total, 256);
   const TileGeom tg = tile_geom(prm.dim, prm.shape_fn, g, gather);
   const float *partials = gather ? ctx->partials : nullptr;
   const int32_t *flags = gather ? ctx->tile_flags : nullptr;
   switch (prm.dim) {
-    case 1: hipLaunchKernelGGL(grid_kernel<1>, dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags); break;
-    case 2: hipLaunchKernelGGL(grid_kernel<2>, dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags); break;
-    case 3: hipLaunchKernelGGL(grid_kernel<3>, dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags); break;
+    case 1: hipLaunchKernelGGL(grid_kernel<1>, dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt); break;
+    case 2: hipLaunchKernelGGL(grid_kernel<2>, dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt); break;
+    case 3: hipLaunchKernelGGL(grid_kernel<3>, dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt); break;
     default: return PMPM_ERR_INVALID;
   }
   PMPM_LAUNCH_CHECK();
@@ -2120,11 +2123,12 @@ static int reserve_tiles(pmpm_context *ctx, const pmpm_params &prm, const pmpm_g
 
 // Scatter into the tile partials (+ wrap-around deposits into the cleared
 // node arrays); a grid_kernel pass with gather = 1 completes the P2G.
+// (zeroed: the caller has already cleared mass, mom and mom_nt)
 int do_p2g(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &p,
-           const pmpm_spatial &part, const pmpm_nodes &nodes, hipStream_t s) {
+           const pmpm_spatial &part, const pmpm_nodes &nodes, hipStream_t s, bool zeroed = false) {
   int rc = reserve_tiles(ctx, prm, g);
   if (rc != PMPM_OK) return rc;
-  if ((rc = do_zero_nodes(prm, g, nodes, s)) != PMPM_OK) return rc;
+  if (!zeroed && (rc = do_zero_nodes(prm, g, nodes, s)) != PMPM_OK) return rc;
   PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, p, p, part, nodes, 0, ctx->partials, ctx->tile_flags,
                     s);
   PMPM_LAUNCH_CHECK();
@@ -2142,13 +2146,15 @@ int do_g2p(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in,
 
 // First map (x, v into `out` in cell order) + re-map of m v into the tile
 // partials; a grid_kernel pass with gather = 2 completes mom_nt.
+// (zeroed: mom_nt is already zero, left so by the grid pass before)
 int do_musl_map(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in,
-                const pmpm_particles &out, const pmpm_spatial &part, const pmpm_nodes &nodes, hipStream_t s) {
+                const pmpm_particles &out, const pmpm_spatial &part, const pmpm_nodes &nodes, hipStream_t s,
+                bool zeroed = false) {
   int rc = reserve_tiles(ctx, prm, g);
   if (rc != PMPM_OK) return rc;
   // the fused pass reads the node velocity records, never mom_nt: clear it
   // for the re-map's wrap-around atomics first
-  if ((rc = zero_fill(nodes.mom_nt, (int64_t)g.num_cells_total * prm.dim, s)) != PMPM_OK) return rc;
+  if (!zeroed && (rc = zero_fill(nodes.mom_nt, (int64_t)g.num_cells_total * prm.dim, s)) != PMPM_OK) return rc;
   PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, in, out, part, nodes, 1, ctx->partials,
                     ctx->tile_flags, s);
   PMPM_LAUNCH_CHECK();
