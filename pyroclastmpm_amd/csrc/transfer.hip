@@ -1697,9 +1697,8 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
 // MODE 2: MUSL second map: L, F, V from the re-mapped velocities + return map
 //         (x, v of slot j of `out` were written by the first map, which
 //         runs fused with the re-map: p2g_kernel MODE 1).
-// The window reads go through L2: the node velocities (12-24 B per node)
-// of a 1M-particle grid fit in it, and staging a block's node box in LDS
-// measured slower on gfx950 (fewer resident waves for the same reads).
+// 3D gathers read node rows staged in LDS per workgroup (see
+// g2p_gather_staged); 1D/2D gathers read the node records through L2.
 constexpr int kG2PThreads = 256;
 // LDS node staging of the 3D G2P. A workgroup's 256 particles are
 // consecutive in cell order, so they fall into a few runs: consecutive
@@ -1711,7 +1710,8 @@ constexpr int kG2PThreads = 256;
 // cell), and every particle gathers from LDS. A staged node is the clamped
 // grid node its slot names: entries outside the grid have zero weight and
 // read a finite value.
-constexpr int kG2PStageNodes = 1536;
+constexpr int kG2PStageNodes = 1088;  // one run of the widest x range (16 x 67 nodes) always fits
+static_assert(kG2PStageNodes >= 16 * (64 + 3), "a run spans at most 64 x bins");
 struct G2PRunTable {
   int y[kG2PThreads], z[kG2PThreads], x0[kG2PThreads], x1[kG2PThreads], base[kG2PThreads];
   int wave_sum[kG2PThreads / 64];
