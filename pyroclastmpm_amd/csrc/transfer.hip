@@ -1152,15 +1152,20 @@ struct MfmaGeom {
 };
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-template <int SF>
+// REMAP: the MUSL re-map instead (p2g_kernel MODE 1's deposit): m v' with v'
+// from slot j of `out` (written by the first map, g2p_kernel MODE 1) and the
+// weights of the old positions; the slice holds the 3 momentum fields.
+template <int SF, bool REMAP = false>
 __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles in,
-                                                       pmpm_spatial part, pmpm_nodes nodes, int tiles_x, int tiles_y,
-                                                       float *__restrict__ partials, int32_t *__restrict__ flags) {
+                                                       pmpm_particles out, pmpm_spatial part, pmpm_nodes nodes,
+                                                       int tiles_x, int tiles_y, float *__restrict__ partials,
+                                                       int32_t *__restrict__ flags) {
   using G = P2GGeom<3, SF>;
   using M = MfmaGeom;
   static_assert(G::TX == M::CELLS && G::TZ == 4 && G::NX == M::BX && G::NY == M::BY && G::SPAN == 4,
                 "the tile of p2g_kernel<3, SF, 0>");
-  constexpr int NV = 7, NTN = G::NTN, NLO = G::NLO;
+  constexpr int NV = REMAP ? 3 : 7, NTN = G::NTN, NLO = G::NLO;
+  static_assert(NV == p2g_nv<3, REMAP ? 1 : 0>(), "slice fields of the matching p2g_kernel mode");
   __shared__ float block[4][NV][M::BN];
   __shared__ float stage[4][M::NF][64];
 
@@ -1245,29 +1250,34 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
         const float m = in.mass[slot];
         float v[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) v[a] = in.v[a * n + slot];
-        const Sym sg = load_sym(in.stress, n, slot);
-        const float mvol = -in.volume[slot];
-        const Sym A{mvol * sg.xx, mvol * sg.yy, mvol * sg.zz, mvol * sg.xy, mvol * sg.xz, mvol * sg.yz};
+        for (int a = 0; a < 3; ++a) v[a] = REMAP ? out.v[a * n + j] : in.v[a * n + slot];
+        Sym A = sym_zero();
+        if (!REMAP) {
+          const Sym sg = load_sym(in.stress, n, slot);
+          const float mvol = -in.volume[slot];
+          A = Sym{mvol * sg.xx, mvol * sg.yy, mvol * sg.zz, mvol * sg.xy, mvol * sg.xz, mvol * sg.yz};
+        }
         if (!own) {
           // listed in this cell but binned elsewhere (a partition of other
           // positions): deposited over its own window, no product terms here
 #pragma unroll 1
-          for (int sy = 0; sy < 4; ++sy) p2g_stray<3, SF, 0>(prm, g, nodes, in.x, n, slot, sy, m, v, A);
+          for (int sy = 0; sy < 4; ++sy) p2g_stray<3, SF, REMAP ? 1 : 0>(prm, g, nodes, in.x, n, slot, sy, m, v, A);
         }
         const float keep = own ? 1.f : 0.f;
         st[0 * 64 + lane] = fr[0];
         st[1 * 64 + lane] = fr[1];
         st[2 * 64 + lane] = fr[2];
-        st[3 * 64 + lane] = keep * m;
+        st[3 * 64 + lane] = REMAP ? 0.f : keep * m;  // (the re-map's row 0 is padding)
 #pragma unroll
         for (int a = 0; a < 3; ++a) st[(4 + a) * 64 + lane] = keep * m * v[a];
-        st[7 * 64 + lane] = keep * A.xx;
-        st[8 * 64 + lane] = keep * A.yy;
-        st[9 * 64 + lane] = keep * A.zz;
-        st[10 * 64 + lane] = keep * A.xy;
-        st[11 * 64 + lane] = keep * A.xz;
-        st[12 * 64 + lane] = keep * A.yz;
+        if (!REMAP) {
+          st[7 * 64 + lane] = keep * A.xx;
+          st[8 * 64 + lane] = keep * A.yy;
+          st[9 * 64 + lane] = keep * A.zz;
+          st[10 * 64 + lane] = keep * A.xy;
+          st[11 * 64 + lane] = keep * A.xz;
+          st[12 * 64 + lane] = keep * A.yz;
+        }
       }
       wave_lds_sync();
       for (int k0 = 0; k0 < wn; k0 += 4) {
@@ -1277,7 +1287,8 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
         const float fx = st[0 * 64 + qq], fy = st[1 * 64 + qq], fz = st[2 * 64 + qq];
         // F_f (f = s2): m, m vx, m vy, m vz; A_{a, .} (a = s2; row 3 is padding)
         const float Ff = st[(3 + s2) * 64 + qq];
-        const float Aax = st[ax * 64 + qq], Aay = st[ay * 64 + qq], Aaz = st[az * 64 + qq];
+        const float Aax = REMAP ? 0.f : st[ax * 64 + qq], Aay = REMAP ? 0.f : st[ay * 64 + qq];
+        const float Aaz = REMAP ? 0.f : st[az * 64 + qq];
         float Nx, dNx, Ny, dNy, Nz, dNz;
         piece(xc, __fsub_rn(fx, (float)(NLO + s1)), s1 == 0, Nx, dNx);
         piece(yc, __fsub_rn(fy, (float)(NLO + s1)), s1 == 0, Ny, dNy);
@@ -1288,18 +1299,25 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
         const float b1 = valid ? Ny * Nz : 0.f;
         const float b2y = valid ? dNy * Nz : 0.f, b2z = valid ? Ny * dNz : 0.f;
         C1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, C1, 0, 0, 0);
-        C2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2x, b1, C2, 0, 0, 0);
-        C2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2y, b2y, C2, 0, 0, 0);
-        C2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2z, b2z, C2, 0, 0, 0);
+        if (!REMAP) {
+          C2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2x, b1, C2, 0, 0, 0);
+          C2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2y, b2y, C2, 0, 0, 0);
+          C2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2z, b2z, C2, 0, 0, 0);
+        }
       }
       wave_lds_sync();
     }
     // node (i, j, k) = (l / 16, (l % 16) / 4, l % 4) of the cell's window
     const int nb = (lx + pk) + M::BX * (s1 + M::BY * s2);
+    if (REMAP) {  // rows 1..3: m v'
 #pragma unroll
-    for (int r = 0; r < 4; ++r) blk[r * M::BN + nb] += C1[r];
+      for (int r = 1; r < 4; ++r) blk[(r - 1) * M::BN + nb] += C1[r];
+    } else {
 #pragma unroll
-    for (int r = 0; r < 3; ++r) blk[(4 + r) * M::BN + nb] += C2[r];
+      for (int r = 0; r < 4; ++r) blk[r * M::BN + nb] += C1[r];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) blk[(4 + r) * M::BN + nb] += C2[r];
+    }
   }
   __syncthreads();
   // slice node (ix, iy, iz): the blocks of waves iz - 3 .. iz
@@ -1307,7 +1325,7 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
   if (t == 0) flags[blockIdx.x] = 1;
   for (int i = t; i < NTN; i += 256) {
     const int ix = i % M::BX, iy = (i / M::BX) % M::BY, iz = i / (M::BX * M::BY);
-    float sum[NV] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float sum[NV] = {};
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
       const int z = iz - w;
@@ -1743,10 +1761,10 @@ __device__ inline int block_inclusive_sum(int v, int *wave_sum, int *total) {
   return v + before;
 }
 
-// Gathers v_nt (and dv with WANT_DV) and L of the particles of lanes
+// Gathers v_nt (and dv with WANT_DV, L with WANT_L) of the particles of lanes
 // 0 .. nv - 1 (weights w) through the LDS staging `lds`
 // (g2p_stage_bytes<WANT_DV>() bytes). Every thread of the workgroup calls it.
-template <int D, int SF, bool WANT_DV>
+template <int D, int SF, bool WANT_DV, bool WANT_L = true>
 __device__ inline void g2p_gather_staged(const pmpm_grid &g, const pmpm_nodes &nodes, const PW<D, SF> &w, int nv,
                                          char *lds, float *vinc, float *dvinc, float *L) {
   static_assert(D == 3, "3D staging");
@@ -1813,7 +1831,7 @@ __device__ inline void g2p_gather_staged(const pmpm_grid &g, const pmpm_nodes &n
     if (myk >= k0 && myk < k1) {
       const int off = R.base[myk] - b0, X = R.x1[myk] - R.x0[myk] + 4;
       NodeSrc src{nodes.vel, box4 + off, box2 + off, {1, X, 4 * X}, {R.x0[myk] - 1, 0, 0}};
-      gather_grid<D, SF, WANT_DV, true, true>(g, src, w, vinc, dvinc, L);
+      gather_grid<D, SF, WANT_DV, WANT_L, true>(g, src, w, vinc, dvinc, L);
     }
     k0 = k1;
     __syncthreads();  // the next round overwrites the staging
@@ -1824,7 +1842,8 @@ template <int D, int SF, int MODE>
 __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles in,
                                                           pmpm_particles out, pmpm_spatial part, pmpm_nodes nodes,
                                                           int reorder) {
-  static_assert(MODE == 0 || MODE == 2, "G2P modes: USL (0), MUSL second map (2)");
+  static_assert(MODE >= 0 && MODE <= 2, "G2P modes: USL (0), MUSL first map (1), MUSL second map (2)");
+  constexpr bool kV = MODE != 2, kL = MODE != 1;  // what the gather sums
   const int n = in.n;
   const int j = blockIdx.x * kG2PThreads + threadIdx.x;
   const bool valid = j < n;
@@ -1838,20 +1857,23 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(pmpm_params prm, pmpm_
   float vinc[3] = {0.f, 0.f, 0.f}, dvinc[3] = {0.f, 0.f, 0.f};
   float L[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   if constexpr (D == 3) {
-    __shared__ __attribute__((aligned(16))) char lds[g2p_stage_bytes<MODE == 0>()];
+    __shared__ __attribute__((aligned(16))) char lds[g2p_stage_bytes<kV>()];
     const int nv = min(n - (int)blockIdx.x * kG2PThreads, kG2PThreads);
-    g2p_gather_staged<D, SF, MODE == 0>(g, nodes, w, nv, lds, vinc, dvinc, L);
+    g2p_gather_staged<D, SF, kV, kL>(g, nodes, w, nv, lds, vinc, dvinc, L);
     if (!valid) return;
   } else {
-    gather_grid<D, SF, MODE == 0, true>(g, node_src(g, nodes), w, vinc, dvinc, L);
+    gather_grid<D, SF, kV, kL>(g, node_src(g, nodes), w, vinc, dvinc, L);
   }
-  if (MODE == 0) {
+  if (MODE != 2) {
+    // MODE 1 writes slot j of `out` (cell order), as the second map reads it
+    const int to = MODE == 1 ? j : dst;
     const float am = (float)(1.0 - (double)prm.alpha);
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      out.v[a * n + dst] = am * vinc[a] + prm.alpha * (in.v[a * n + src] + dvinc[a]);
-      out.x[a * n + dst] = xp[a] + prm.dt * vinc[a];
+      out.v[a * n + to] = am * vinc[a] + prm.alpha * (in.v[a * n + src] + dvinc[a]);
+      out.x[a * n + to] = xp[a] + prm.dt * vinc[a];
     }
+    if (MODE == 1) return;
   }
   // F = (I + L dt) F, V = det(F) V0
   float Fo[9], Fn[9];
@@ -2030,13 +2052,21 @@ void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles
                 hipStream_t s, const float *rec = nullptr) {
   const TileGeom tg = tile_geom_of<D, SF>(g, 1);
   const dim3 grid((unsigned)tile_count(tg)), block(P2GGeom<D, SF>::THREADS);
-  // 3D typed cubic P2G on the matrix cores; everything else on the VALU kernel
+  // 3D typed cubic P2G and MUSL re-map on the matrix cores (the re-map after
+  // a G2P-style first map); everything else on the VALU kernel
   bool mfma = false;
   if constexpr (D == 3 && SF == PMPM_SF_CUBIC) {
-    mfma = mode == 0 && prm.node_types_built;
-    if (mfma)
-      hipLaunchKernelGGL((p2g_mfma_kernel<SF>), grid, dim3(256), 0, s, prm, g, in, part, nodes, tg.tiles[0],
-                         tg.tiles[1], partials, flags);
+    mfma = (mode == 0 || mode == 1) && prm.node_types_built;
+    if (mfma && mode == 0) {
+      hipLaunchKernelGGL((p2g_mfma_kernel<SF, false>), grid, dim3(256), 0, s, prm, g, in, out, part, nodes,
+                         tg.tiles[0], tg.tiles[1], partials, flags);
+    } else if (mfma && mode == 1) {
+      if (in.n > 0)
+        hipLaunchKernelGGL((g2p_kernel<D, SF, 1>), dim3((unsigned)ceil_div(in.n, kG2PThreads)), dim3(kG2PThreads), 0,
+                           s, prm, g, in, out, part, nodes, 1);
+      hipLaunchKernelGGL((p2g_mfma_kernel<SF, true>), grid, dim3(256), 0, s, prm, g, in, out, part, nodes,
+                         tg.tiles[0], tg.tiles[1], partials, flags);
+    }
   }
   // (each mode launches exactly one tile kernel; the adjoint scatter reads
   // `rec` and is never launched without it)
@@ -2045,8 +2075,9 @@ void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles
       hipLaunchKernelGGL((p2g_kernel<D, SF, 0>), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
                          tg.tiles[1], partials, flags, rec);
   } else if (mode == 1) {
-    hipLaunchKernelGGL((p2g_kernel<D, SF, 1>), dim3((unsigned)tile_count(tg) + kMuslTailBlocks), block, 0, s, prm, g,
-                       in, out, part, nodes, tg.tiles[0], tg.tiles[1], partials, flags, rec);
+    if (!mfma)
+      hipLaunchKernelGGL((p2g_kernel<D, SF, 1>), dim3((unsigned)tile_count(tg) + kMuslTailBlocks), block, 0, s, prm,
+                         g, in, out, part, nodes, tg.tiles[0], tg.tiles[1], partials, flags, rec);
   } else if (mode == 2 && rec) {
     hipLaunchKernelGGL((p2g_kernel<D, SF, 2>), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
                        tg.tiles[1], partials, flags, rec);
