@@ -1147,7 +1147,10 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
 struct MfmaGeom {
   static constexpr int CELLS = 16;                      // cells per wave (x)
   static constexpr int BX = CELLS + 3, BY = 4, BZ = 4;  // a wave's node block
-  static constexpr int BN = BX * BY * BZ;
+  // x pitch of the block in LDS: the 64 lanes of a flush, offsets
+  // pk + 20 s1 + 80 s2, fall into 64 distinct banks (19 would pair some up)
+  static constexpr int BXP = BX + 1;
+  static constexpr int BN = BXP * BY * BZ;
   static constexpr int NF = 13;                         // staged floats per particle
 };
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -1308,7 +1311,7 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
       wave_lds_sync();
     }
     // node (i, j, k) = (l / 16, (l % 16) / 4, l % 4) of the cell's window
-    const int nb = (lx + pk) + M::BX * (s1 + M::BY * s2);
+    const int nb = (lx + pk) + M::BXP * (s1 + M::BY * s2);
     if (REMAP) {  // rows 1..3: m v'
 #pragma unroll
       for (int r = 1; r < 4; ++r) blk[(r - 1) * M::BN + nb] += C1[r];
@@ -1331,7 +1334,7 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
       const int z = iz - w;
       if (z < 0 || z >= M::BZ) continue;
 #pragma unroll
-      for (int f = 0; f < NV; ++f) sum[f] += block[w][f][ix + M::BX * (iy + M::BY * z)];
+      for (int f = 0; f < NV; ++f) sum[f] += block[w][f][ix + M::BXP * (iy + M::BY * z)];
     }
 #pragma unroll
     for (int f = 0; f < NV; ++f) slice[f * NTN + i] = sum[f];
