@@ -148,22 +148,26 @@ __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
   static_assert(R <= kSortBlock, "one digit per thread");
   __shared__ uint32_t base[R];
   __shared__ uint32_t wcnt[W][R];
+  __shared__ uint32_t wsum[W];
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int w = t >> 6;
   const bool owner = t < R;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  // digit bases: exclusive scan of the R column totals
+  // digit bases: exclusive scan of the R column totals (thread d owns digit
+  // d): a shuffle scan within each wave, then the wave totals before it
   const uint32_t tot = owner ? totals[t] : 0u;
-  if (owner) base[t] = tot;
-  __syncthreads();
-  for (int off = 1; off < R; off <<= 1) {
-    const uint32_t v = (owner && t >= off) ? base[t - off] : 0u;
-    __syncthreads();
-    if (owner) base[t] += v;
-    __syncthreads();
+  uint32_t incl = tot;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += u;
   }
-  if (owner) base[t] = base[t] - tot + hist[(int64_t)t * nblocks + blockIdx.x];
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  uint32_t before = 0u;
+  for (int q = 0; q < w; ++q) before += wsum[q];
+  if (owner) base[t] = before + incl - tot + hist[(int64_t)t * nblocks + blockIdx.x];
   const int64_t tile = (int64_t)blockIdx.x * kSortTile;
   for (int r = 0; r < kSortItems; ++r) {
     if (owner) {
