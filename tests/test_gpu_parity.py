@@ -184,6 +184,14 @@ def test_column_dp_musl_cubic_parity(oracle_mod):
     _compare_run(oracle_mod, spec, 10, 1e-4, "column DP MUSL cubic")
 
 
+@pytest.mark.parametrize("sf", ["quadratic", "linear"])
+def test_column_dp_musl_fused_map_parity(oracle_mod, sf):
+    """3D MUSL with a spline other than typed cubic: the fused first map +
+    re-map pass (p2g_kernel MODE 1) and its trailing workgroups."""
+    spec = S.column(h=0.05, per_axis=2, dt=1e-4, shape_function=sf, musl=True)
+    _compare_run(oracle_mod, spec, 10, 1e-4, f"column DP MUSL {sf}")
+
+
 def test_column_dp_usl_quadratic_parity(oracle_mod):
     spec = S.column(h=0.05, per_axis=2, dt=1e-4, shape_function="quadratic", musl=False)
     _compare_run(oracle_mod, spec, 10, 1e-4, "column DP USL quadratic")
