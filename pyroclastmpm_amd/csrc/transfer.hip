@@ -2349,15 +2349,21 @@ __global__ __launch_bounds__(256) void g2p_adjoint_kernel(pmpm_params prm, pmpm_
   const bool valid = j < n;
   float pb[4] = {0.f, 0.f, 0.f, 0.f};
   int color = -1;
+  const int src = valid ? part.sorted[j] : 0;
+  PW<D, SF> w;
+  float xp[3];
+  if (valid) particle_weights<D, SF>(prm, g, st.x, n, src, w, xp);
+  float vpic[3] = {0.f, 0.f, 0.f}, dvp[3] = {0.f, 0.f, 0.f};
+  float L[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if constexpr (D == 3) {  // the forward G2P's gather, node rows staged in LDS
+    static_assert(kG2PThreads == 256, "the workgroup of the staged gather");
+    __shared__ __attribute__((aligned(16))) char lds[g2p_stage_bytes<true>()];
+    const int nv = min(n - (int)blockIdx.x * kG2PThreads, kG2PThreads);
+    g2p_gather_staged<D, SF, true, true>(g, nodes, w, nv, lds, vpic, dvp, L);
+  } else if (valid) {
+    gather_grid<D, SF, true, true>(g, node_src(g, nodes), w, vpic, dvp, L);
+  }
   if (valid) {
-    const int src = part.sorted[j];
-    PW<D, SF> w;
-    float xp[3];
-    particle_weights<D, SF>(prm, g, st.x, n, src, w, xp);
-    const NodeSrc ns = node_src(g, nodes);
-    float vpic[3] = {0.f, 0.f, 0.f}, dvp[3] = {0.f, 0.f, 0.f};
-    float L[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    gather_grid<D, SF, true, true>(g, ns, w, vpic, dvp, L);
 
     float xo[3] = {0.f, 0.f, 0.f}, vo[3] = {0.f, 0.f, 0.f}, Fo[9], F[9], so6[6], eo6[6], s6[6], e6[6], sr6[6];
 #pragma unroll
@@ -2568,8 +2574,11 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
   constexpr int EZ = D > 2 ? E : 1;
   const int n = st.n;
   const int nn = g.num_cells_total;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+  // four lanes per particle, one per y offset of the window (the lanes of a
+  // particle are consecutive, so their sums combine with two shuffles)
+  const int j = (int)blockIdx.x * ((int)blockDim.x / 4) + (int)threadIdx.x / 4;
+  const int sub = (int)threadIdx.x & 3;
+  if (j >= n) return;  // (all four lanes of a particle together)
   const int src = part.sorted[j];
   PW<D, SF> w;
   float xp[3];
@@ -2666,16 +2675,33 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
       }
     }
   }
+  // this lane's y offset (none past the window: zero weights, node 0)
+  auto pick = [&](const auto *a) { return sub == 0 ? a[0] : (sub == 1 ? a[1] : (sub == 2 ? a[2] : a[3])); };
+  const bool has_y = sub < EY;
+  const int oy = has_y ? pick(off[1]) : 0;
+  const float Ny = has_y ? pick(N[1]) : 0.f, dNy = has_y ? pick(dN[1]) : 0.f, d2Ny = has_y ? pick(d2N[1]) : 0.f;
 #pragma unroll
-  for (int ey = 0; ey < EY; ++ey)
+  for (int ex = 0; ex < E; ++ex)
 #pragma unroll
-    for (int ex = 0; ex < E; ++ex)
+    for (int ez = 0; ez < EZ; ++ez) {
+      const float n3[3] = {N[0][ex], Ny, N[2][ez]}, d3[3] = {dN[0][ex], dNy, dN[2][ez]},
+                  h3[3] = {d2N[0][ex], d2Ny, d2N[2][ez]};
+      entry(off[0][ex] + oy + off[2][ez], n3, d3, h3, true);
+    }
+  // the particle's sums over its four lanes; lane 0 goes on alone
+  auto sum4 = [](float v) {
+    v += __shfl_xor(v, 1, 64);
+    return v + __shfl_xor(v, 2, 64);
+  };
 #pragma unroll
-      for (int ez = 0; ez < EZ; ++ez) {
-        const float n3[3] = {N[0][ex], N[1][ey], N[2][ez]}, d3[3] = {dN[0][ex], dN[1][ey], dN[2][ez]},
-                    h3[3] = {d2N[0][ex], d2N[1][ey], d2N[2][ez]};
-        entry(off[0][ex] + off[1][ey] + off[2][ez], n3, d3, h3, true);
-      }
+  for (int a = 0; a < 3; ++a) {
+    xb[a] = sum4(xb[a]);
+    vb[a] = sum4(vb[a]);
+#pragma unroll
+    for (int b = 0; b < 3; ++b) Mb[a][b] = sum4(Mb[a][b]);
+  }
+  Vb = sum4(Vb);
+  if (sub != 0) return;
   // wrap-around entries (deposit only; the wrapped node's types)
   if (leaves && prm.wrap_quirk) {
 #pragma unroll 1
@@ -2732,9 +2758,9 @@ void launch_adjoint_particles(const pmpm_params &prm, const pmpm_grid &g, const 
   if (which == 0)
     hipLaunchKernelGGL((g2p_adjoint_kernel<D, SF>), dim3(blocks), dim3(256), 0, s, prm, g, st, ob, ib, part, nodes,
                        rec, pbar);
-  else
-    hipLaunchKernelGGL((p2g_adjoint_kernel<D, SF>), dim3(blocks), dim3(256), 0, s, prm, g, st, ib, part, nodes, nbar,
-                       rec);
+  else  // four lanes per particle
+    hipLaunchKernelGGL((p2g_adjoint_kernel<D, SF>), dim3((unsigned)ceil_div(st.n, 64)), dim3(256), 0, s, prm, g, st,
+                       ib, part, nodes, nbar, rec);
 }
 
 }  // namespace pmpm
