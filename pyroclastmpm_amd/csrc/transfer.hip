@@ -1505,9 +1505,10 @@ struct TileGeom {
 // flags of all candidate tiles are loaded first; then every candidate's loads
 // are issued without a branch (a dead or out-of-range candidate reads tile
 // 0's first entries and adds zero), so a node waits for two loads in a row,
-// not one per tile.
+// not one per tile. Returns whether any candidate tile was live (a node no
+// live tile covers skips the loads).
 template <int D, int NV>
-__device__ inline void gather_partials(const TileGeom &tg, const float *partials, const int32_t *flags,
+__device__ inline bool gather_partials(const TileGeom &tg, const float *partials, const int32_t *flags,
                                        const int *ids, float *acc) {
   constexpr int MX = kCoverX, MY = D > 1 ? kCoverY : 1, MZ = D > 2 ? kCoverZ : 1, MC = MX * MY * MZ;
   int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
@@ -1540,6 +1541,10 @@ __device__ inline void gather_partials(const TileGeom &tg, const float *partials
         if (D > 2) li += tg.N[0] * tg.N[1] * (ids[2] - (bz * tg.T[2] + tg.nlo));
         off[c] = (int64_t)tile * NV * ntn + li;
       }
+  bool any = false;
+#pragma unroll
+  for (int c = 0; c < MC; ++c) any = any || live[c];
+  if (!any) return false;
 #pragma unroll
   for (int c = 0; c < MC; ++c) {
     const float *s = partials + (live[c] ? off[c] : 0);
@@ -1549,6 +1554,7 @@ __device__ inline void gather_partials(const TileGeom &tg, const float *partials
 #pragma unroll
     for (int k = 0; k < NV; ++k) acc[k] += live[c] ? v[k] : 0.f;
   }
+  return true;
 }
 
 // Per node: cutoff + forward moment (finalize), boundary conditions
