@@ -68,6 +68,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the multi-scene sweep")
     ap.add_argument("--batched-scenes", type=int, default=64)
     ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the differentiated multi-scene sweep")
+    ap.add_argument("--ensemble-steps", type=int, default=40)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="before the workload, run 3 plain device copies of MIB MiB (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -285,6 +287,64 @@ def batched_throughput(count, steps):
             "config": "3D Drucker-Prager columns, MUSL, cubic, 8 per cell, dt 1e-5, seed 1"}
 
 
+def ensemble_throughput(count, steps, every=20):
+    """The multi-scene sweep differentiated: `count` columns with the sweep's
+    geometry and materials (seed 1), USL, cubic, dt 1e-5, checkpoints every
+    `every` steps; per-scene loss = mean squared displacement of the final
+    positions; gradient with respect to each scene's (phi, c, E). The
+    ensemble runs its forward through pmpm_batch_step (lanes and graph replay
+    from the library's autotuner) and its reverse pass through
+    pmpm_batch_adjoint_step. Compared with the same scenes run one after
+    another, each through diff.simulate and its backward."""
+    import torch
+    from pyroclastmpm_amd.diff import simulate
+    from pyroclastmpm_amd.ensemble import Ensemble, column_ensemble, simulate_ensemble
+
+    models, thetas = column_ensemble(count, seed=1, dt=1e-5, checkpoint_every=every)
+    x0s = [m.positions(m.initial) for m in models]
+    ens = Ensemble(models, lanes=4)
+    tuning = ens.autotune(thetas)
+    graph = bool(tuning["use_graph"])
+    th = thetas.clone().requires_grad_(True)
+    xs = simulate_ensemble(ens, th, 2, graph=graph)  # sizes the segment pools and adjoint workspaces
+    sum(((x - x0) ** 2).mean() for x, x0 in zip(xs, x0s)).backward()
+    th.grad = None
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    torch.cuda.synchronize()
+    ev[0].record()
+    xs = simulate_ensemble(ens, th, steps, graph=graph)
+    ev[1].record()
+    sum(((x - x0) ** 2).mean() for x, x0 in zip(xs, x0s)).backward()
+    ev[2].record()
+    torch.cuda.synchronize()
+    fwd, bwd = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+    g_ens = th.grad.clone()
+    # the same scenes one after another, each alone
+    rel = 0.0
+    torch.cuda.synchronize()
+    ev[0].record()
+    grads = []
+    for i, m in enumerate(models):
+        t = thetas[i].clone().requires_grad_(True)
+        ((simulate(m, t, steps) - x0s[i]) ** 2).mean().backward()
+        grads.append(t.grad)
+    ev[1].record()
+    torch.cuda.synchronize()
+    seq = ev[0].elapsed_time(ev[1])
+    for i, g in enumerate(grads):
+        rel = max(rel, float(((g_ens[i] - g).abs() / g.abs().clamp_min(1e-30)).max()))
+    n = ens.particles
+    return {"metric": "particle-updates/s, differentiated multi-scene column sweep (forward + adjoint)",
+            "value": round(n * steps / ((fwd + bwd) * 1e-3), 1), "unit": "particle-updates/s",
+            "higher_is_better": True, "scenes": count, "particles": n, "steps": steps, "checkpoint_every": every,
+            "tuning": tuning, "forward_ms_per_step": round(fwd / steps, 4),
+            "backward_ms_per_step": round(bwd / steps, 4), "adjoint_forward_ratio": round(bwd / fwd, 3),
+            "one_after_another_ms_per_step": round(seq / steps, 4),
+            "speedup_vs_one_after_another": round(seq / (fwd + bwd), 3),
+            "max_rel_grad_difference_vs_alone": rel,
+            "config": "3D Drucker-Prager columns, USL, cubic, 8 per cell, dt 1e-5, seed 1"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -382,6 +442,12 @@ def main():
             print("bench: batched " + json.dumps(line["batched"]), file=sys.stderr, flush=True)
         except Exception as e:  # reported, the throughput line stands on its own
             line["batched"] = {"value": None, "error": str(e)}
+    if not args.no_ensemble:
+        try:
+            line["ensemble"] = ensemble_throughput(args.batched_scenes, args.ensemble_steps)
+            print("bench: ensemble " + json.dumps(line["ensemble"]), file=sys.stderr, flush=True)
+        except Exception as e:  # reported, the throughput line stands on its own
+            line["ensemble"] = {"value": None, "error": str(e)}
     print(json.dumps(line), flush=True)
 
 

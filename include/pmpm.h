@@ -356,6 +356,29 @@ int pmpm_batch_autotune(pmpm_batch *batch, const pmpm_scene *scenes, int32_t cou
 /* The batch's current setting. */
 int pmpm_batch_get_tuning(const pmpm_batch *batch, pmpm_batch_tuning *out);
 
+/* One scene's arguments to pmpm_adjoint_step (see there), for the batched
+ * reverse pass of an ensemble of differentiated scenes. */
+typedef struct pmpm_adjoint_scene {
+  pmpm_context *ctx;
+  pmpm_params prm;
+  pmpm_grid grid;
+  pmpm_particles state;
+  pmpm_adjoint out_bar;
+  pmpm_adjoint in_bar;
+  float *param_bar;
+  int32_t recompute;
+  pmpm_spatial part;
+  pmpm_nodes nodes;
+} pmpm_adjoint_scene;
+
+/* The adjoint step of every scene, each exactly as pmpm_adjoint_step runs
+ * it, on the batch's lanes (its current lane count; scenes placed by its
+ * weights, else by particle count), forked from and joined back into
+ * `stream`. Scenes must not share a context, param_bar or buffers. Eager
+ * launches only. Returns the first scene's error, if any. */
+int pmpm_batch_adjoint_step(pmpm_batch *batch, const pmpm_adjoint_scene *scenes, int32_t count,
+                            pmpm_stream_t stream);
+
 /* The last HIP failure an entry point met on the calling thread, as
  * "call: hipGetErrorString (code)", or "" if none (thread-local storage). */
 const char *pmpm_last_error(void);

@@ -36,6 +36,34 @@ def _lattice(lo, hi, spacing, jitter, rng):
     return pts.astype(np.float32)
 
 
+def column_geometry(n_target, h, per_axis=2, jitter=0.1, seed=0):
+    """Particle positions (n, 3) and domain end of a column with the 1.0 x 0.5
+    x 0.5 m benchmark column's proportions (column 0.2 x 0.5 x 0.4 of the
+    domain), scaled to about n_target particles at per_axis^3 per cell of
+    size h."""
+    spacing = h / per_axis
+    s = (n_target * spacing ** 3 / (0.2 * 0.5 * 0.4)) ** (1.0 / 3.0)
+    col = (0.2 * s, 0.5 * s, 0.4 * s)
+    end = (1.0 * s, 0.5 * s, 0.5 * s)
+    return _lattice([0.0, 0.0, 0.0], col, spacing, jitter, np.random.default_rng(seed)), end
+
+
+def sweep_parameters(count=64, seed=1, n_min=16000, n_max=1000000):
+    """(n_target, h, phi, E) of each scene of the sweep, drawn with
+    numpy.random.default_rng(seed): particle counts log-uniform in
+    [n_min, n_max], h uniform in [0.01, 0.02] m, friction angle uniform in
+    [20, 40] deg, Young's modulus uniform in [5e5, 2e6] Pa."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = float(math.exp(rng.uniform(math.log(n_min), math.log(n_max))))
+        h = float(rng.uniform(0.01, 0.02))
+        phi = float(rng.uniform(20.0, 40.0))
+        E = float(rng.uniform(5e5, 2e6))
+        out.append((n, h, phi, E))
+    return out
+
+
 class ColumnScene:
     """One 3D Drucker-Prager column collapse (the benchmark column's physics
     at another size and material), with both particle buffer sets, nodes,
@@ -43,12 +71,7 @@ class ColumnScene:
 
     def __init__(self, n_target, h, phi, E, cohesion=0.0, per_axis=2, dt=1e-5, jitter=0.1, seed=0,
                  device="cuda"):
-        spacing = h / per_axis
-        # the benchmark column's proportions, scaled to about n_target particles
-        s = (n_target * spacing ** 3 / (0.2 * 0.5 * 0.4)) ** (1.0 / 3.0)
-        col = (0.2 * s, 0.5 * s, 0.4 * s)
-        end = (1.0 * s, 0.5 * s, 0.5 * s)
-        pos = _lattice([0.0, 0.0, 0.0], col, spacing, jitter, np.random.default_rng(seed))
+        pos, end = column_geometry(n_target, h, per_axis, jitter, seed)
         self.n = n = pos.shape[0]
         self.h, self.phi, self.E = h, phi, E
         self.grid = capi.make_grid([0.0, 0.0, 0.0], end, h, 3)
@@ -120,15 +143,8 @@ class ColumnScene:
 def column_batch(count=64, seed=1, n_min=16000, n_max=1000000, per_axis=2, dt=1e-5, device="cuda"):
     """The multi-scene sweep: `count` columns, parameters drawn with
     numpy.random.default_rng(seed)."""
-    rng = np.random.default_rng(seed)
-    scenes = []
-    for i in range(count):
-        n = float(math.exp(rng.uniform(math.log(n_min), math.log(n_max))))
-        h = float(rng.uniform(0.01, 0.02))
-        phi = float(rng.uniform(20.0, 40.0))
-        E = float(rng.uniform(5e5, 2e6))
-        scenes.append(ColumnScene(n, h, phi, E, per_axis=per_axis, dt=dt, seed=seed * 1000 + i, device=device))
-    return scenes
+    return [ColumnScene(n, h, phi, E, per_axis=per_axis, dt=dt, seed=seed * 1000 + i, device=device)
+            for i, (n, h, phi, E) in enumerate(sweep_parameters(count, seed, n_min, n_max))]
 
 
 class Batch:

@@ -79,6 +79,12 @@ class Scene(C.Structure):
                 ("nodes", Nodes), ("musl", C.c_int32)]
 
 
+class AdjointScene(C.Structure):
+    _fields_ = [("ctx", vp), ("prm", Params), ("grid", Grid), ("state", Particles), ("out_bar", Adjoint),
+                ("in_bar", Adjoint), ("param_bar", vp), ("recompute", C.c_int32), ("part", Spatial),
+                ("nodes", Nodes)]
+
+
 class BatchTuning(C.Structure):
     _fields_ = [("lanes", C.c_int32), ("use_graph", C.c_int32), ("step_ms", C.c_float)]
 
@@ -115,6 +121,7 @@ PROTOTYPES = {
     "pmpm_batch_replay": (C.c_int, [vp, C.c_int32, vp]),
     "pmpm_batch_autotune": (C.c_int, [vp, P(Scene), C.c_int32, C.c_int32, C.c_int32, vp, P(BatchTuning)]),
     "pmpm_batch_get_tuning": (C.c_int, [vp, P(BatchTuning)]),
+    "pmpm_batch_adjoint_step": (C.c_int, [vp, P(AdjointScene), C.c_int32, vp]),
     "pmpm_last_error": (C.c_char_p, []),
     "pmpm_version": (C.c_char_p, []),
 }

@@ -44,17 +44,23 @@ int scene_ok(const pmpm_scene &s) {
   return s.ctx && s.set[0].n == s.set[1].n && s.set[0].x && s.set[1].x && s.set[0].x != s.set[1].x;
 }
 
-std::vector<float> weights_of(const pmpm_batch *b, const pmpm_scene *scenes, int count) {
+// Per-scene weights for the lane split: the batch's own (measured step
+// times, after autotuning) when it holds one per scene, else particle counts.
+template <class Count>
+std::vector<float> weights_of(const pmpm_batch *b, int count, Count particles) {
   std::vector<float> w(count);
   for (int i = 0; i < count; ++i)
-    w[i] = (int)b->weight.size() == count ? b->weight[i] : (float)scenes[i].set[0].n;
+    w[i] = (int)b->weight.size() == count ? b->weight[i] : (float)particles(i);
   return w;
 }
 
-// Enqueues one step of every scene on the batch's lanes, forked from and
-// joined into `origin` (lane 0 is `origin` itself).
-int enqueue_step(pmpm_batch *b, const pmpm_scene *scenes, int count, int cur, int lanes, hipStream_t origin) {
-  std::vector<float> w = weights_of(b, scenes, count);
+// Forks `lanes` streams from `origin` (lane 0 is `origin` itself), enqueues
+// scene i with enqueue(i, stream of its lane), and joins the lanes back into
+// `origin` with events, also when an enqueue failed (the first failure is
+// returned and the scenes after it are not enqueued).
+template <class Enqueue>
+int fork_join(pmpm_batch *b, const std::vector<float> &w, int lanes, hipStream_t origin, Enqueue enqueue) {
+  const int count = (int)w.size();
   std::vector<int32_t> lane_of(count);
   pmpm_batch_assign(w.data(), count, lanes, lane_of.data());
   hipStream_t s[PMPM_MAX_LANES];
@@ -64,22 +70,27 @@ int enqueue_step(pmpm_batch *b, const pmpm_scene *scenes, int count, int cur, in
     PMPM_HIP_CHECK(hipEventRecord(b->fork, origin));
     for (int l = 1; l < lanes; ++l) PMPM_HIP_CHECK(hipStreamWaitEvent(s[l], b->fork, 0));
   }
-  for (int i = 0; i < count; ++i) {
-    const pmpm_scene &sc = scenes[i];
-    pmpm_spatial part = sc.part;
-    pmpm_nodes nodes = sc.nodes;
-    pmpm_particles out = sc.set[cur ^ 1];
-    const int rc = pmpm_step(sc.ctx, &sc.prm, &sc.grid, sc.musl, &sc.set[cur], &out, &part, &nodes,
-                             (pmpm_stream_t)s[lane_of[i]]);
-    if (rc != PMPM_OK) return rc;
-  }
+  int rc = PMPM_OK;
+  for (int i = 0; i < count && rc == PMPM_OK; ++i) rc = enqueue(i, s[lane_of[i]]);
   if (lanes > 1) {
     for (int l = 1; l < lanes; ++l) {
       PMPM_HIP_CHECK(hipEventRecord(b->join[l], s[l]));
       PMPM_HIP_CHECK(hipStreamWaitEvent(origin, b->join[l], 0));
     }
   }
-  return PMPM_OK;
+  return rc;
+}
+
+// Enqueues one step of every scene on the batch's lanes.
+int enqueue_step(pmpm_batch *b, const pmpm_scene *scenes, int count, int cur, int lanes, hipStream_t origin) {
+  const std::vector<float> w = weights_of(b, count, [&](int i) { return scenes[i].set[0].n; });
+  return fork_join(b, w, lanes, origin, [&](int i, hipStream_t s) {
+    const pmpm_scene &sc = scenes[i];
+    pmpm_spatial part = sc.part;
+    pmpm_nodes nodes = sc.nodes;
+    pmpm_particles out = sc.set[cur ^ 1];
+    return pmpm_step(sc.ctx, &sc.prm, &sc.grid, sc.musl, &sc.set[cur], &out, &part, &nodes, (pmpm_stream_t)s);
+  });
 }
 
 int capture(pmpm_batch *b, const pmpm_scene *scenes, int count, int cur, hipStream_t origin) {
@@ -208,6 +219,22 @@ extern "C" int pmpm_batch_replay(pmpm_batch *b, int32_t cur, pmpm_stream_t strea
   if (!b || (cur != 0 && cur != 1) || !b->graph[cur]) return PMPM_ERR_INVALID;
   PMPM_HIP_CHECK(hipGraphLaunch(b->graph[cur], (hipStream_t)stream));
   return PMPM_OK;
+}
+
+extern "C" int pmpm_batch_adjoint_step(pmpm_batch *b, const pmpm_adjoint_scene *scenes, int32_t count,
+                                       pmpm_stream_t stream) {
+  if (!b || count < 0 || (count > 0 && !scenes)) return PMPM_ERR_INVALID;
+  for (int i = 0; i < count; ++i)
+    if (!scenes[i].ctx || !scenes[i].param_bar) return PMPM_ERR_INVALID;
+  const std::vector<float> w = weights_of(b, count, [&](int i) { return scenes[i].state.n; });
+  return fork_join(b, w, b->tuning.lanes, (hipStream_t)stream, [&](int i, hipStream_t s) {
+    const pmpm_adjoint_scene &sc = scenes[i];
+    pmpm_adjoint in_bar = sc.in_bar;
+    pmpm_spatial part = sc.part;
+    pmpm_nodes nodes = sc.nodes;
+    return pmpm_adjoint_step(sc.ctx, &sc.prm, &sc.grid, &sc.state, &sc.out_bar, &in_bar, sc.param_bar,
+                             sc.recompute, &part, &nodes, (pmpm_stream_t)s);
+  });
 }
 
 extern "C" int pmpm_batch_get_tuning(const pmpm_batch *b, pmpm_batch_tuning *out) {

@@ -28,6 +28,9 @@ struct pmpm_context {
   size_t adj_rec_cap = 0;
   float *adj_nodes = nullptr;
   size_t adj_nodes_cap = 0;
+  // per-workgroup partial sums of the material parameter adjoints
+  float *adj_pb = nullptr;
+  size_t adj_pb_cap = 0;
 };
 
 namespace pmpm {

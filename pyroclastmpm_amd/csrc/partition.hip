@@ -282,6 +282,7 @@ extern "C" int pmpm_context_destroy(pmpm_context *ctx) {
   if (!ctx) return PMPM_OK;
   (void)hipFree(ctx->adj_rec);
   (void)hipFree(ctx->adj_nodes);
+  (void)hipFree(ctx->adj_pb);
   (void)hipFree(ctx->partials);
   (void)hipFree(ctx->tile_flags);
   (void)hipFree(ctx->keys_tmp);

@@ -2470,13 +2470,40 @@ __global__ __launch_bounds__(256) void g2p_adjoint_kernel(pmpm_params prm, pmpm_
 #pragma unroll
     for (int k = 0; k < 9; ++k) rec[(6 + k) * n + j] = Lb[k];
   }
-  // material parameter adjoints: one atomic per wave and material
+  // material parameter adjoints: the workgroup's sum per material and
+  // parameter (waves in a fixed order), reduced over workgroups in a fixed
+  // order by param_bar_reduce_kernel, so the gradient does not depend on
+  // scheduling
+  __shared__ float wsum[4][4 * PMPM_MAX_MATERIALS];
+  const int wave = threadIdx.x >> 6;
   for (int k = 0; k < prm.num_materials; ++k)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const float s = wave_sum(color == k ? pb[q] : 0.f);
-      if ((threadIdx.x & 63) == 0 && s != 0.f) atomicAdd(&pbar[4 * k + q], s);
+      if ((threadIdx.x & 63) == 0) wsum[wave][4 * k + q] = s;
     }
+  __syncthreads();
+  const int e = threadIdx.x;
+  if (e < 4 * prm.num_materials)
+    pbar[(size_t)blockIdx.x * (4 * prm.num_materials) + e] = ((wsum[0][e] + wsum[1][e]) + wsum[2][e]) + wsum[3][e];
+}
+
+// param_bar[e] += the sum over workgroups of part[b * entries + e]: one
+// workgroup per entry, each lane summing a fixed stride of workgroups, then a
+// fixed tree.
+__global__ __launch_bounds__(256) void param_bar_reduce_kernel(const float *__restrict__ part, int groups,
+                                                               int entries, float *__restrict__ param_bar) {
+  __shared__ float red[256];
+  const int e = blockIdx.x;
+  float s = 0.f;
+  for (int b = threadIdx.x; b < groups; b += 256) s += part[(size_t)b * entries + e];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) param_bar[e] += red[0];
 }
 
 // Per node: adjoints of the node velocities (scattered in the tile partials)
@@ -2807,10 +2834,18 @@ extern "C" int pmpm_adjoint_step(pmpm_context *ctx, const pmpm_params *prm, cons
   if ((rc = ensure_buffer(ctx->adj_rec, ctx->adj_rec_cap, (size_t)15 * n)) != PMPM_OK) return rc;
   if ((rc = ensure_buffer(ctx->adj_nodes, ctx->adj_nodes_cap, (size_t)(1 + 2 * D) * g.num_cells_total)) != PMPM_OK)
     return rc;
-  // G2P adjoint, then the scatter of its records to the nodes
+  const int groups = ceil_div(n, 256), entries = 4 * prm->num_materials;
+  if ((rc = ensure_buffer(ctx->adj_pb, ctx->adj_pb_cap, (size_t)groups * entries)) != PMPM_OK) return rc;
+  // G2P adjoint (material parameter adjoints per workgroup, then summed into
+  // param_bar), then the scatter of its records to the nodes
   PMPM_DISPATCH_DSF(D, prm->shape_fn, launch_adjoint_particles, *prm, g, *state, *out_bar, *in_bar, *part, *nodes,
-                    ctx->adj_rec, param_bar, 0, nullptr, s);
+                    ctx->adj_rec, ctx->adj_pb, 0, nullptr, s);
   PMPM_LAUNCH_CHECK();
+  if (entries > 0) {
+    hipLaunchKernelGGL(param_bar_reduce_kernel, dim3((unsigned)entries), dim3(256), 0, s, ctx->adj_pb, groups,
+                       entries, param_bar);
+    PMPM_LAUNCH_CHECK();
+  }
   PMPM_DISPATCH_DSF(D, prm->shape_fn, launch_p2g, *prm, g, *state, *state, *part, *nodes, 2, ctx->partials,
                     ctx->tile_flags, s, ctx->adj_rec);
   PMPM_LAUNCH_CHECK();

@@ -81,3 +81,43 @@ def test_scene_struct_matches_header_layout():
     assert s.prm.offset == C.sizeof(C.c_void_p)
     assert s.set.size == 2 * C.sizeof(capi.Particles)
     assert s.musl.offset == s.nodes.offset + C.sizeof(capi.Nodes)
+
+
+_SCENE_FIELDS = {
+    "pmpm_scene": (capi.Scene, ["ctx", "prm", "grid", "set", "part", "nodes", "musl"]),
+    "pmpm_adjoint_scene": (capi.AdjointScene, ["ctx", "prm", "grid", "state", "out_bar", "in_bar", "param_bar",
+                                               "recompute", "part", "nodes"]),
+}
+
+
+def test_scene_structs_match_header_offsets(tmp_path):
+    # offsetof/sizeof of the header's structs, from a C program built against
+    # include/pmpm.h, against the ctypes mirrors
+    import os
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "pmpm.h"', "int main(void) {"]
+    for name, (_, fields) in _SCENE_FIELDS.items():
+        lines.append(f'  printf("{name} sizeof %zu\\n", sizeof({name}));')
+        for f in fields:
+            lines.append(f'  printf("{name} {f} %zu\\n", offsetof({name}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([gcc, "-std=c99", "-Wall", "-Werror", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for name, (cls, fields) in _SCENE_FIELDS.items():
+        assert got[(name, "sizeof")] == C.sizeof(cls), name
+        for f in fields:
+            assert got[(name, f)] == getattr(cls, f).offset, (name, f)
+
+
+def test_batch_adjoint_rejects_bad_arguments():
+    L = capi.lib()
+    assert L.pmpm_batch_adjoint_step(None, None, 0, None) == capi.PMPM_ERR_INVALID
