@@ -23,7 +23,7 @@ struct pmpm_context {
   int32_t *tile_flags = nullptr;  // 1 = tile had particles this launch
   size_t tile_flags_cap = 0;
   // adjoint step: per-particle scatter records (15 floats, cell order) and
-  // node adjoints (1 + 2 dim floats per node)
+  // node adjoints (one record of 8 floats per node, 4 in 1D)
   float *adj_rec = nullptr;
   size_t adj_rec_cap = 0;
   float *adj_nodes = nullptr;

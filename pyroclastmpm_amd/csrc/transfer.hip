@@ -2506,6 +2506,41 @@ __global__ __launch_bounds__(256) void param_bar_reduce_kernel(const float *__re
   if (threadIdx.x == 0) param_bar[e] += red[0];
 }
 
+// Node adjoints (mass, momentum, force) as one record per node: 8 floats in
+// 2D and 3D (m, P, f, padding), 4 in 1D, so a gather reads one or two
+// 16-byte vectors per node instead of 1 + 2D scalars from separate arrays.
+template <int D> struct NodeAdj { static constexpr int NP = D == 1 ? 4 : 8; };
+template <int D>
+__device__ inline void load_node_adj(const float *__restrict__ nbar, int node, float &mb, float *Pb, float *fb) {
+  const char *base = reinterpret_cast<const char *>(nbar);
+  const uint32_t at = (uint32_t)node * (uint32_t)(NodeAdj<D>::NP * sizeof(float));
+  float r[8];
+  const float4 a = *reinterpret_cast<const float4 *>(base + at);
+  r[0] = a.x; r[1] = a.y; r[2] = a.z; r[3] = a.w;
+  if constexpr (NodeAdj<D>::NP == 8) {
+    const float4 b = *reinterpret_cast<const float4 *>(base + at + 16u);
+    r[4] = b.x; r[5] = b.y; r[6] = b.z; r[7] = b.w;
+  }
+  mb = r[0];
+#pragma unroll
+  for (int a2 = 0; a2 < D; ++a2) {
+    Pb[a2] = r[1 + a2];
+    fb[a2] = r[1 + D + a2];
+  }
+}
+template <int D>
+__device__ inline void store_node_adj(float *__restrict__ nbar, int node, float mb, const float *Pb, const float *fb) {
+  float r[8] = {mb, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    r[1 + a] = Pb[a];
+    r[1 + D + a] = fb[a];
+  }
+  float4 *p = reinterpret_cast<float4 *>(nbar + (size_t)NodeAdj<D>::NP * node);
+  p[0] = make_float4(r[0], r[1], r[2], r[3]);
+  if constexpr (NodeAdj<D>::NP == 8) p[1] = make_float4(r[4], r[5], r[6], r[7]);
+}
+
 // Per node: adjoints of the node velocities (scattered in the tile partials)
 // -> adjoints of mass, moment and force. Boundary conditions in reverse list
 // order with the forward's active sets (read from the post-condition
@@ -2566,12 +2601,7 @@ __global__ void grid_adjoint_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nod
       fb[a] = prm.dt * Qb[a];
     }
   }
-  nbar[i] = mb;
-#pragma unroll
-  for (int a = 0; a < D; ++a) {
-    nbar[(1 + a) * nn + i] = Pb[a];
-    nbar[(1 + D + a) * nn + i] = fb[a];
-  }
+  store_node_adj<D>(nbar, i, mb, Pb, fb);
 }
 
 // d det(F) / dF (row-major D x D).
@@ -2644,14 +2674,10 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
   auto entry = [&](int node, const float *N3, const float *dN3, const float *d2N3, bool g2p) {
     float wgt, gr[3], H[6];
     entry_derivs(N3, dN3, d2N3, wgt, gr, H);
-    const float mb = nbar[node];
-    float Pb[3] = {0.f, 0.f, 0.f}, fb[3] = {0.f, 0.f, 0.f}, vP = 0.f;
+    float mb, Pb[3] = {0.f, 0.f, 0.f}, fb[3] = {0.f, 0.f, 0.f}, vP = 0.f;
+    load_node_adj<D>(nbar, node, mb, Pb, fb);
 #pragma unroll
-    for (int a = 0; a < D; ++a) {
-      Pb[a] = nbar[(1 + a) * nn + node];
-      fb[a] = nbar[(1 + D + a) * nn + node];
-      vP += v[a] * Pb[a];
-    }
+    for (int a = 0; a < D; ++a) vP += v[a] * Pb[a];
     float sf[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int b = 0; b < D; ++b)
@@ -2721,25 +2747,12 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
                   h3[3] = {d2N[0][ex], d2Ny, d2N[2][ez]};
       entry(off[0][ex] + oy + off[2][ez], n3, d3, h3, true);
     }
-  // the particle's sums over its four lanes; lane 0 goes on alone
-  auto sum4 = [](float v) {
-    v += __shfl_xor(v, 1, 64);
-    return v + __shfl_xor(v, 2, 64);
-  };
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    xb[a] = sum4(xb[a]);
-    vb[a] = sum4(vb[a]);
-#pragma unroll
-    for (int b = 0; b < 3; ++b) Mb[a][b] = sum4(Mb[a][b]);
-  }
-  Vb = sum4(Vb);
-  if (sub != 0) return;
-  // wrap-around entries (deposit only; the wrapped node's types)
-  if (leaves && prm.wrap_quirk) {
+  // wrap-around entries (deposit only; the wrapped node's types), split
+  // over the lanes by y offset like the in-grid entries
+  if (leaves && prm.wrap_quirk && has_y) {
 #pragma unroll 1
-    for (int e = 0; e < E * EY * EZ; ++e) {
-      const int e3[3] = {e % E, D > 1 ? (e / E) % EY : 0, D > 2 ? e / (E * EY) : 0};
+    for (int e = 0; e < E * EZ; ++e) {
+      const int e3[3] = {e % E, D > 1 ? sub : 0, D > 2 ? e / E : 0};
       int nb[3] = {0, 0, 0}, offs[3] = {0, 0, 0};
       bool inside = true;
 #pragma unroll
@@ -2763,6 +2776,20 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
       entry((int)lin, n3, d3, h3, false);
     }
   }
+  // the particle's sums over its four lanes; lane 0 goes on alone
+  auto sum4 = [](float v) {
+    v += __shfl_xor(v, 1, 64);
+    return v + __shfl_xor(v, 2, 64);
+  };
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    xb[a] = sum4(xb[a]);
+    vb[a] = sum4(vb[a]);
+#pragma unroll
+    for (int b = 0; b < 3; ++b) Mb[a][b] = sum4(Mb[a][b]);
+  }
+  Vb = sum4(Vb);
+  if (sub != 0) return;
   float F[9], C[9];
 #pragma unroll
   for (int k = 0; k < D * D; ++k) F[k] = st.F[k * n + src];
@@ -2832,7 +2859,8 @@ extern "C" int pmpm_adjoint_step(pmpm_context *ctx, const pmpm_params *prm, cons
     return rc;
   }
   if ((rc = ensure_buffer(ctx->adj_rec, ctx->adj_rec_cap, (size_t)15 * n)) != PMPM_OK) return rc;
-  if ((rc = ensure_buffer(ctx->adj_nodes, ctx->adj_nodes_cap, (size_t)(1 + 2 * D) * g.num_cells_total)) != PMPM_OK)
+  const size_t adj_np = D == 1 ? NodeAdj<1>::NP : NodeAdj<3>::NP;
+  if ((rc = ensure_buffer(ctx->adj_nodes, ctx->adj_nodes_cap, adj_np * g.num_cells_total)) != PMPM_OK)
     return rc;
   const int groups = ceil_div(n, 256), entries = 4 * prm->num_materials;
   if ((rc = ensure_buffer(ctx->adj_pb, ctx->adj_pb_cap, (size_t)groups * entries)) != PMPM_OK) return rc;
