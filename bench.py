@@ -65,6 +65,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward time ratio")
     ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-cell-size", type=float, default=0.02,
+                    help="cell size of the differentiated column (0.02 -> 40,000 particles at 2 per axis)")
+    ap.add_argument("--adjoint-per-axis", type=int, default=2)
     ap.add_argument("--no-batched", action="store_true", help="skip the multi-scene sweep")
     ap.add_argument("--batched-scenes", type=int, default=64)
     ap.add_argument("--batched-steps", type=int, default=10)
@@ -213,10 +216,10 @@ def cpu_baseline(spec, steps):
                       f"{dt:.1f} s"}
 
 
-def adjoint_ratio(steps, every=20):
-    """The differentiable 3D Drucker-Prager column: h = 0.02 (51 x 26 x 26
-    nodes), 8 particles per cell (40,000 particles), USL, cubic, dt = 1e-4,
-    checkpoints every 20 steps. Loss = mean squared distance of the final
+def adjoint_ratio(steps, every=20, h=0.02, per_axis=2):
+    """The differentiable 3D Drucker-Prager column: by default h = 0.02
+    (51 x 26 x 26 nodes), 8 particles per cell (40,000 particles), USL, cubic,
+    dt = 1e-4 (1e-5 below h = 0.02), checkpoints every 20 steps. Loss = mean squared distance of the final
     positions to a target run (phi 32 deg, c 50 Pa, E 1.2e6); gradient with
     respect to (phi, c, E) from (28 deg, 0, 1e6). Returns the backward time per
     step (segment recomputation included) over the forward time per step."""
@@ -224,8 +227,9 @@ def adjoint_ratio(steps, every=20):
     from pyroclastmpm_amd.diff import DiffMPM, simulate
     from tests import scenarios as S
 
-    spec = S.column(h=0.02, per_axis=2, dt=1e-4, jitter=0.1, seed=0, musl=False)
-    model = DiffMPM(3, spec.origin, spec.end, 0.02, spec.positions, 8, dt=1e-4, checkpoint_every=every)
+    dt = 1e-4 if h >= 0.02 else 1e-5
+    spec = S.column(h=h, per_axis=per_axis, dt=dt, jitter=0.1, seed=0, musl=False)
+    model = DiffMPM(3, spec.origin, spec.end, h, spec.positions, per_axis ** 3, dt=dt, checkpoint_every=every)
     with torch.no_grad():
         target = simulate(model, torch.tensor([32.0, 50.0, 1.2e6], dtype=torch.float64), steps)
     theta = torch.tensor([28.0, 0.0, 1e6], dtype=torch.float64, requires_grad=True)
@@ -246,7 +250,7 @@ def adjoint_ratio(steps, every=20):
             "higher_is_better": False, "forward_ms_per_step": round(fwd / steps, 4),
             "backward_ms_per_step": round(bwd / steps, 4), "steps": steps, "checkpoint_every": every,
             "particles": model.n, "loss": float(loss.detach()), "grad": [float(v) for v in theta.grad],
-            "config": "3D Drucker-Prager column, h 0.02, 8 per cell, USL, cubic, dt 1e-4"}
+            "config": f"3D Drucker-Prager column, h {h}, {per_axis ** 3} per cell, USL, cubic, dt {dt}"}
 
 
 def batched_throughput(count, steps):
@@ -432,7 +436,8 @@ def main():
     }
     if not args.no_adjoint:
         try:
-            line["adjoint"] = adjoint_ratio(args.adjoint_steps)
+            line["adjoint"] = adjoint_ratio(args.adjoint_steps, h=args.adjoint_cell_size,
+                                            per_axis=args.adjoint_per_axis)
             print("bench: adjoint " + json.dumps(line["adjoint"]), file=sys.stderr, flush=True)
         except Exception as e:  # reported, the throughput line stands on its own
             line["adjoint"] = {"value": None, "error": str(e)}

@@ -2667,23 +2667,31 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
   for (int a = 0; a < D; ++a)
 #pragma unroll
     for (int b = 0; b < D; ++b) Lb[3 * a + b] = rec[(6 + 3 * a + b) * n + j];
-  float vb[3] = {0.f, 0.f, 0.f}, xb[3] = {0.f, 0.f, 0.f}, Mb[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f},
-                                                                         {0.f, 0.f, 0.f}};
-  float Vb = 0.f;
+  // per node: the deposit adjoints (m, P, f) and, for the G2P weights, the
+  // node velocities; G = sum f (x) grad w gives the stress and volume adjoints
+  // after the loop
+  float Sv[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) Sv[a][b] = V * S[a][b];
+  float vb[3] = {0.f, 0.f, 0.f}, xb[3] = {0.f, 0.f, 0.f}, G[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f},
+                                                                       {0.f, 0.f, 0.f}};
   // one entry's contributions (g2p = 1 adds the G2P weights' position term)
   auto entry = [&](int node, const float *N3, const float *dN3, const float *d2N3, bool g2p) {
     float wgt, gr[3], H[6];
     entry_derivs(N3, dN3, d2N3, wgt, gr, H);
-    float mb, Pb[3] = {0.f, 0.f, 0.f}, fb[3] = {0.f, 0.f, 0.f}, vP = 0.f;
+    float mb, Pb[3] = {0.f, 0.f, 0.f}, fb[3] = {0.f, 0.f, 0.f};
     load_node_adj<D>(nbar, node, mb, Pb, fb);
+    // gs: the weight's scalar factor; wv = L_bar v_nt - V sigma^T f_bar
+    float gs = mb, wv[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int a = 0; a < D; ++a) vP += v[a] * Pb[a];
-    float sf[3] = {0.f, 0.f, 0.f};
+    for (int a = 0; a < D; ++a) gs += v[a] * Pb[a];
+    gs *= m;
 #pragma unroll
     for (int b = 0; b < D; ++b)
 #pragma unroll
-      for (int a = 0; a < D; ++a) sf[b] += S[a][b] * fb[a];
-    float gs = m * (mb + vP), u[3] = {0.f, 0.f, 0.f};
+      for (int a = 0; a < D; ++a) wv[b] -= Sv[a][b] * fb[a];
     if (g2p) {
       float vnt[3] = {0.f, 0.f, 0.f}, dvn[3] = {0.f, 0.f, 0.f};
       load_node_vel<D, true>(nodes.vel, node, vnt, dvn);
@@ -2692,20 +2700,20 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
 #pragma unroll
       for (int a = 0; a < D; ++a)
 #pragma unroll
-        for (int b = 0; b < D; ++b) u[a] += Lb[3 * a + b] * vnt[b];
+        for (int b = 0; b < D; ++b) wv[a] += Lb[3 * a + b] * vnt[b];
     }
+    const float mw = m * wgt;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      vb[a] += m * wgt * Pb[a];
-      Vb -= sf[a] * gr[a];
+      vb[a] += mw * Pb[a];
 #pragma unroll
-      for (int b = 0; b < D; ++b) Mb[a][b] -= V * fb[a] * gr[b];
+      for (int b = 0; b < D; ++b) G[a][b] += fb[a] * gr[b];
     }
 #pragma unroll
     for (int c = 0; c < D; ++c) {
       float hs = 0.f;
 #pragma unroll
-      for (int b = 0; b < D; ++b) hs += H[sym_idx(c, b)] * (u[b] - V * sf[b]);
+      for (int b = 0; b < D; ++b) hs += H[sym_idx(c, b)] * wv[b];
       xb[c] += gr[c] * gs + hs;
     }
   };
@@ -2786,10 +2794,18 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
     xb[a] = sum4(xb[a]);
     vb[a] = sum4(vb[a]);
 #pragma unroll
-    for (int b = 0; b < 3; ++b) Mb[a][b] = sum4(Mb[a][b]);
+    for (int b = 0; b < 3; ++b) G[a][b] = sum4(G[a][b]);
   }
-  Vb = sum4(Vb);
   if (sub != 0) return;
+  // M_bar = -V G, V_bar = -sigma : G
+  float Mb[3][3], Vb = 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      Mb[a][b] = -V * G[a][b];
+      Vb -= S[a][b] * G[a][b];
+    }
   float F[9], C[9];
 #pragma unroll
   for (int k = 0; k < D * D; ++k) F[k] = st.F[k * n + src];
