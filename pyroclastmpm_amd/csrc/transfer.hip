@@ -2637,11 +2637,9 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
   constexpr int EZ = D > 2 ? E : 1;
   const int n = st.n;
   const int nn = g.num_cells_total;
-  // four lanes per particle, one per y offset of the window (the lanes of a
-  // particle are consecutive, so their sums combine with two shuffles)
-  const int j = (int)blockIdx.x * ((int)blockDim.x / 4) + (int)threadIdx.x / 4;
-  const int sub = (int)threadIdx.x & 3;
-  if (j >= n) return;  // (all four lanes of a particle together)
+  // one lane per particle; the window's y offsets in turn
+  const int j = (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x;
+  if (j >= n) return;
   const int src = part.sorted[j];
   PW<D, SF> w;
   float xp[3];
@@ -2742,61 +2740,51 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
       }
     }
   }
-  // this lane's y offset (none past the window: zero weights, node 0)
-  auto pick = [&](const auto *a) { return sub == 0 ? a[0] : (sub == 1 ? a[1] : (sub == 2 ? a[2] : a[3])); };
-  const bool has_y = sub < EY;
-  const int oy = has_y ? pick(off[1]) : 0;
-  const float Ny = has_y ? pick(N[1]) : 0.f, dNy = has_y ? pick(dN[1]) : 0.f, d2Ny = has_y ? pick(d2N[1]) : 0.f;
-#pragma unroll
-  for (int ex = 0; ex < E; ++ex)
-#pragma unroll
-    for (int ez = 0; ez < EZ; ++ez) {
-      const float n3[3] = {N[0][ex], Ny, N[2][ez]}, d3[3] = {dN[0][ex], dNy, dN[2][ez]},
-                  h3[3] = {d2N[0][ex], d2Ny, d2N[2][ez]};
-      entry(off[0][ex] + oy + off[2][ez], n3, d3, h3, true);
-    }
-  // wrap-around entries (deposit only; the wrapped node's types), split
-  // over the lanes by y offset like the in-grid entries
-  if (leaves && prm.wrap_quirk && has_y) {
+  // y offset `sub` of the window (values picked by selects, so the in-grid
+  // entries stay unrolled without indexing registers at run time)
 #pragma unroll 1
-    for (int e = 0; e < E * EZ; ++e) {
-      const int e3[3] = {e % E, D > 1 ? sub : 0, D > 2 ? e / E : 0};
-      int nb[3] = {0, 0, 0}, offs[3] = {0, 0, 0};
-      bool inside = true;
+  for (int sub = 0; sub < EY; ++sub) {
+    auto pick = [&](const auto *a) { return sub == 0 ? a[0] : (sub == 1 ? a[1] : (sub == 2 ? a[2] : a[3])); };
+    const int oy = pick(off[1]);
+    const float Ny = pick(N[1]), dNy = pick(dN[1]), d2Ny = pick(d2N[1]);
 #pragma unroll
-      for (int a = 0; a < D; ++a) {
-        offs[a] = w.first[a] + e3[a];
-        nb[a] = w.bin[a] + offs[a];
-        inside = inside && nb[a] >= 0 && nb[a] < g.num_cells[a];
-      }
-      const int64_t lin = lin_index(g, nb, D);
-      if (inside || lin < 0 || lin >= nn) continue;
-      int ids[3];
-      node_ids<D>(g, lin, ids);
-      float n3[3] = {1.f, 1.f, 1.f}, d3[3] = {0.f, 0.f, 0.f}, h3[3] = {0.f, 0.f, 0.f};
+    for (int ex = 0; ex < E; ++ex)
 #pragma unroll
-      for (int a = 0; a < D; ++a) {
-        const int type = cubic_node_type(ids[a], g.num_cells[a], prm.node_types_built);
-        const float r = __fsub_rn(__fsub_rn(w.t[a], (float)w.bin[a]), (float)offs[a]);
-        spline_eval<SF>(r, type, g.inv_cell_size, n3[a], d3[a]);
-        h3[a] = spline_d2<SF>(r, type, g.inv_cell_size);
+      for (int ez = 0; ez < EZ; ++ez) {
+        const float n3[3] = {N[0][ex], Ny, N[2][ez]}, d3[3] = {dN[0][ex], dNy, dN[2][ez]},
+                    h3[3] = {d2N[0][ex], d2Ny, d2N[2][ez]};
+        entry(off[0][ex] + oy + off[2][ez], n3, d3, h3, true);
       }
-      entry((int)lin, n3, d3, h3, false);
+    // wrap-around entries of this y offset (deposit only; the wrapped node's
+    // types)
+    if (leaves && prm.wrap_quirk) {
+#pragma unroll 1
+      for (int e = 0; e < E * EZ; ++e) {
+        const int e3[3] = {e % E, D > 1 ? sub : 0, D > 2 ? e / E : 0};
+        int nb[3] = {0, 0, 0}, offs[3] = {0, 0, 0};
+        bool inside = true;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          offs[a] = w.first[a] + e3[a];
+          nb[a] = w.bin[a] + offs[a];
+          inside = inside && nb[a] >= 0 && nb[a] < g.num_cells[a];
+        }
+        const int64_t lin = lin_index(g, nb, D);
+        if (inside || lin < 0 || lin >= nn) continue;
+        int ids[3];
+        node_ids<D>(g, lin, ids);
+        float n3[3] = {1.f, 1.f, 1.f}, d3[3] = {0.f, 0.f, 0.f}, h3[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          const int type = cubic_node_type(ids[a], g.num_cells[a], prm.node_types_built);
+          const float r = __fsub_rn(__fsub_rn(w.t[a], (float)w.bin[a]), (float)offs[a]);
+          spline_eval<SF>(r, type, g.inv_cell_size, n3[a], d3[a]);
+          h3[a] = spline_d2<SF>(r, type, g.inv_cell_size);
+        }
+        entry((int)lin, n3, d3, h3, false);
+      }
     }
   }
-  // the particle's sums over its four lanes; lane 0 goes on alone
-  auto sum4 = [](float v) {
-    v += __shfl_xor(v, 1, 64);
-    return v + __shfl_xor(v, 2, 64);
-  };
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    xb[a] = sum4(xb[a]);
-    vb[a] = sum4(vb[a]);
-#pragma unroll
-    for (int b = 0; b < 3; ++b) G[a][b] = sum4(G[a][b]);
-  }
-  if (sub != 0) return;
   // M_bar = -V G, V_bar = -sigma : G
   float Mb[3][3], Vb = 0.f;
 #pragma unroll
@@ -2834,9 +2822,9 @@ void launch_adjoint_particles(const pmpm_params &prm, const pmpm_grid &g, const 
   if (which == 0)
     hipLaunchKernelGGL((g2p_adjoint_kernel<D, SF>), dim3(blocks), dim3(256), 0, s, prm, g, st, ob, ib, part, nodes,
                        rec, pbar);
-  else  // four lanes per particle
-    hipLaunchKernelGGL((p2g_adjoint_kernel<D, SF>), dim3((unsigned)ceil_div(st.n, 64)), dim3(256), 0, s, prm, g, st,
-                       ib, part, nodes, nbar, rec);
+  else
+    hipLaunchKernelGGL((p2g_adjoint_kernel<D, SF>), dim3(blocks), dim3(256), 0, s, prm, g, st, ib, part, nodes, nbar,
+                       rec);
 }
 
 }  // namespace pmpm
