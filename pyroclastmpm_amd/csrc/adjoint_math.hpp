@@ -185,6 +185,210 @@ PMPM_HD void dp_vjp(const float *p, float etab, bool increment, bool history, bo
   }
 }
 
+// The same VJP as dp_vjp in one reverse sweep: dp_map's float program is
+// run forward once, keeping its intermediates, and its adjoint is propagated
+// backwards through the branch the forward took (elastic, cone return, apex).
+// Costs a few forward maps instead of one dual-number pass per input.
+template <int D>
+PMPM_HD void dp_vjp_rev(const float *p, float etab, bool increment, bool history, bool apex_guard, float dt,
+                        const float *L, const float *sref6, const float *eps6, const float *sbar6,
+                        const float *ebar6, float *eps_bar6, float *Lbar, float *pbar) {
+  const float K = p[0], G = p[1], M = p[2], Pt = p[3];
+  // ---- forward (dp_map's operations, T = float)
+  float ep[3][3], de[3][3], W[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      ep[i][j] = (i < D && j < D) ? eps6[sym_idx(i, j)] : 0.f;
+      de[i][j] = 0.f;
+      W[i][j] = 0.f;
+    }
+  if (increment) {
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j) de[i][j] = L[3 * i + j];
+  } else {
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        de[i][j] = 0.5f * (L[3 * i + j] + L[3 * j + i]) * dt;
+        W[i][j] = 0.5f * (L[3 * i + j] - L[3 * j + i]) * dt;
+      }
+    float rot[3][3];
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j) {
+        float we = 0.f, ew = 0.f;
+        for (int k = 0; k < D; ++k) {
+          we = we + W[i][k] * ep[k][j];
+          ew = ew + ep[i][k] * W[k][j];
+        }
+        rot[i][j] = -we + ew;
+      }
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j) de[i][j] = de[i][j] + rot[i][j];
+  }
+  const float pref = (float)((1.0 / 3.0) * (double)(sref6[0] + sref6[1] + sref6[2]));
+  float sr[3][3], et[3][3], dev[3][3], str[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      sr[i][j] = sref6[sym_idx(i, j)] - (i == j ? pref : 0.f);
+      et[i][j] = (i < D && j < D) ? ep[i][j] + de[i][j] : 0.f;
+    }
+  float ev = et[0][0];
+  if (D > 1) ev = ev + et[1][1];
+  if (D > 2) ev = ev + et[2][2];
+  const float ptr = pref + K * ev;
+  const float ev3 = (1.f / 3.f) * ev;
+  float f2 = 0.f;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      dev[i][j] = et[i][j] - (i == j ? ev3 : 0.f);
+      if (D < 3 && i == 2 && j == 2) dev[i][j] = 0.f;
+      str[i][j] = sr[i][j] + 2.f * G * dev[i][j];
+      f2 = f2 + str[i][j] * str[i][j];
+    }
+  const float J2 = 0.5f * f2;
+  const float eta = 0.57735026918962576f * M;
+  const float sqJ2 = sqrtf(J2);
+  const float Phi = sqJ2 + eta * (ptr - Pt);
+  // ---- reverse
+  float strb[3][3] = {}, etb[3][3] = {}, epb[3][3] = {}, deb[3][3] = {};
+  float ptrb = 0.f, Kb = 0.f, Gb = 0.f, Mb = 0.f, Ptb = 0.f, sqJ2b = 0.f;
+  if (Phi <= 0.f) {  // elastic: s = str + ptr I, eps_out = et
+    for (int i = 0; i < 3; ++i)
+      for (int j = i; j < 3; ++j) {
+        strb[i][j] += sbar6[sym_idx(i, j)];
+        if (i == j) ptrb += sbar6[i];
+        if (history && i < D && j < D) etb[i][j] += ebar6[sym_idx(i, j)];
+      }
+  } else {
+    const float den = K * eta * etab + G;
+    const float dg = Phi / den;
+    float pn = ptr - K * etab * dg;
+    const float f = 1.f - (G * dg) / sqJ2;
+    const bool apex = sqJ2 - G * dg < 0.f;
+    float dga = 0.f;
+    if (apex) {
+      if (apex_guard && etab == 0.f) {
+        pn = Pt;
+      } else {
+        dga = (ptr - Pt) / (etab * K);
+        pn = ptr - K * etab * dga;
+      }
+    }
+    // outputs: s = sn + pn I (upper triangle), eps_out = (sn - sr)/(2G) + vol I
+    float snb[3][3] = {}, pnb = 0.f;
+    for (int i = 0; i < 3; ++i)
+      for (int j = i; j < 3; ++j) {
+        snb[i][j] += sbar6[sym_idx(i, j)];
+        if (i == j) pnb += sbar6[i];
+      }
+    if (history) {
+      const float i2G = 1.f / (2.f * G);
+      float i2Gb = 0.f, volb = 0.f;
+      for (int i = 0; i < D; ++i)
+        for (int j = i; j < D; ++j) {
+          const float e = ebar6[sym_idx(i, j)];
+          const float sn = apex ? 0.f : f * str[i][j];
+          snb[i][j] += e * i2G;
+          i2Gb += e * (sn - sr[i][j]);
+          if (i == j) volb += e;
+        }
+      Gb += i2Gb * (-2.f * i2G * i2G);
+      pnb += volb / (3.f * K);
+      Kb += -volb * (pn - pref) / (3.f * K * K);
+    }
+    float dgb = 0.f;
+    if (!apex) {
+      // sn = f str; f = 1 - G dg / sqJ2; pn = ptr - K etab dg
+      float fb = 0.f;
+      for (int i = 0; i < 3; ++i)
+        for (int j = i; j < 3; ++j) {
+          fb += snb[i][j] * str[i][j];
+          strb[i][j] += f * snb[i][j];
+        }
+      Gb += -fb * dg / sqJ2;
+      dgb += -fb * G / sqJ2;
+      sqJ2b += fb * G * dg / (sqJ2 * sqJ2);
+      ptrb += pnb;
+      Kb += -pnb * etab * dg;
+      dgb += -pnb * K * etab;
+    } else if (apex_guard && etab == 0.f) {
+      Ptb += pnb;
+    } else {
+      // pn = ptr - K etab dga, dga = (ptr - Pt) / (etab K)
+      ptrb += pnb;
+      Kb += -pnb * etab * dga;
+      const float dgab = -pnb * K * etab;
+      const float q = 1.f / (etab * K);
+      ptrb += dgab * q;
+      Ptb -= dgab * q;
+      Kb += -dgab * (ptr - Pt) * q / K;
+    }
+    // dg = Phi / den, den = K eta etab + G, Phi = sqJ2 + eta (ptr - Pt)
+    const float Phib = dgb / den;
+    const float denb = -dgb * Phi / (den * den);
+    Kb += denb * eta * etab;
+    float etab_b = denb * K * etab;
+    Gb += denb;
+    sqJ2b += Phib;
+    etab_b += Phib * (ptr - Pt);
+    ptrb += Phib * eta;
+    Ptb -= Phib * eta;
+    Mb += 0.57735026918962576f * etab_b;
+    // sqJ2 = sqrt(f2 / 2)
+    if (sqJ2 > 0.f) {
+      const float f2b = 0.5f * (sqJ2b / (2.f * sqJ2));
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) strb[i][j] += 2.f * str[i][j] * f2b;
+    }
+  }
+  // str = sr + 2G dev; dev = et - ev3 I (zz fixed at 0 below 3D)
+  float ev3b = 0.f;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      if (D < 3 && i == 2 && j == 2) continue;
+      Gb += 2.f * dev[i][j] * strb[i][j];
+      const float devb = 2.f * G * strb[i][j];
+      if (i < D && j < D) etb[i][j] += devb;
+      if (i == j) ev3b -= devb;
+    }
+  float evb = (1.f / 3.f) * ev3b;
+  Kb += ptrb * ev;
+  evb += ptrb * K;
+  for (int i = 0; i < D; ++i) etb[i][i] += evb;
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j) {
+      epb[i][j] += etb[i][j];
+      deb[i][j] += etb[i][j];
+    }
+  if (increment) {
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j) Lbar[3 * i + j] += deb[i][j];
+  } else {
+    float Wb[3][3] = {};
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j) {
+        const float r = deb[i][j];
+        for (int k = 0; k < D; ++k) {
+          Wb[i][k] -= r * ep[k][j];
+          epb[k][j] -= r * W[i][k];
+          epb[i][k] += r * W[k][j];
+          Wb[k][j] += r * ep[i][k];
+        }
+      }
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        const float a = 0.5f * dt * deb[i][j], b = 0.5f * dt * Wb[i][j];
+        Lbar[3 * i + j] += a + b;
+        Lbar[3 * j + i] += a - b;
+      }
+  }
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j) eps_bar6[sym_idx(i, j)] += epb[i][j];
+  pbar[0] += Kb;
+  pbar[1] += Gb;
+  pbar[2] += Mb;
+  pbar[3] += Ptb;
+}
+
 //   le: d/d s6 (6), d/d L (9), d/d p (2: lam, G)
 template <int D>
 PMPM_HD void le_vjp(const float *p, float dt, const float *L, const float *s6, const float *sbar6, float *s_bar6,

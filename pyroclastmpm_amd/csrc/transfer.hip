@@ -18,6 +18,8 @@
 // particle x node is stored.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "adjoint_math.hpp"
 #include "context.hpp"
 #include "device.hpp"
@@ -2408,7 +2410,7 @@ __global__ __launch_bounds__(256) void g2p_adjoint_kernel(pmpm_params prm, pmpm_
         if (st.stress_ref && (m.flags & PMPM_MATF_HAS_STRESS_REF))
 #pragma unroll
           for (int k = 0; k < 6; ++k) sr6[k] = st.stress_ref[k * n + src];
-        dp_vjp<D>(m.p, m.p[4], (m.flags & PMPM_MATF_VELGRAD_IS_INCREMENT) != 0, history,
+        dp_vjp_rev<D>(m.p, m.p[4], (m.flags & PMPM_MATF_VELGRAD_IS_INCREMENT) != 0, history,
                   (m.flags & PMPM_MATF_APEX_GUARD) != 0, prm.dt, L, sr6, e6, so6, eo6, eb6, Lb, pb);
         s_through = false;  // the stress is recomputed from the strain
         e_through = !history;
@@ -2628,7 +2630,7 @@ template <int D> __device__ inline void cofactor(const float *F, float *C) {
 // run branch-free (clamped node, zero weights outside, as gather_grid); the
 // wrap-around entries of boundary particles, which only the deposit reaches,
 // take a separate compact loop. Adds to ib.
-template <int D, int SF>
+template <int D, int SF, int LANES>
 __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles st,
                                                           pmpm_adjoint ib, pmpm_spatial part, pmpm_nodes nodes,
                                                           const float *__restrict__ nbar, const float *__restrict__ rec) {
@@ -2637,9 +2639,14 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
   constexpr int EZ = D > 2 ? E : 1;
   const int n = st.n;
   const int nn = g.num_cells_total;
-  // one lane per particle; the window's y offsets in turn
-  const int j = (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x;
-  if (j >= n) return;
+  // LANES = 1: one lane per particle, the window's y offsets in turn;
+  // LANES = 4: four consecutive lanes per particle, one y offset each, sums
+  // combined by shuffles (for launches too small to fill the chip otherwise)
+  static_assert(LANES == 1 || LANES == 4, "lanes per particle");
+  const int j = LANES == 1 ? (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x
+                           : (int)blockIdx.x * ((int)blockDim.x / 4) + (int)threadIdx.x / 4;
+  const int sub0 = LANES == 1 ? 0 : (int)threadIdx.x & 3;
+  if (j >= n) return;  // (all lanes of a particle together)
   const int src = part.sorted[j];
   PW<D, SF> w;
   float xp[3];
@@ -2743,7 +2750,7 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
   // y offset `sub` of the window (values picked by selects, so the in-grid
   // entries stay unrolled without indexing registers at run time)
 #pragma unroll 1
-  for (int sub = 0; sub < EY; ++sub) {
+  for (int sub = sub0; sub < EY; sub += LANES) {
     auto pick = [&](const auto *a) { return sub == 0 ? a[0] : (sub == 1 ? a[1] : (sub == 2 ? a[2] : a[3])); };
     const int oy = pick(off[1]);
     const float Ny = pick(N[1]), dNy = pick(dN[1]), d2Ny = pick(d2N[1]);
@@ -2785,6 +2792,20 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
       }
     }
   }
+  if constexpr (LANES == 4) {  // the particle's sums over its lanes; lane 0 goes on
+    auto sum4 = [](float v) {
+      v += __shfl_xor(v, 1, 64);
+      return v + __shfl_xor(v, 2, 64);
+    };
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      xb[a] = sum4(xb[a]);
+      vb[a] = sum4(vb[a]);
+#pragma unroll
+      for (int b = 0; b < 3; ++b) G[a][b] = sum4(G[a][b]);
+    }
+    if (sub0 != 0) return;
+  }
   // M_bar = -V G, V_bar = -sigma : G
   float Mb[3][3], Vb = 0.f;
 #pragma unroll
@@ -2813,6 +2834,18 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
     for (int b = a; b < D; ++b) ib.stress[sym_idx(a, b) * n + src] += a == b ? Mb[a][a] : Mb[a][b] + Mb[b][a];
 }
 
+// Below this many particles, one lane per particle leaves fewer than three
+// waves per SIMD (256 CUs x 4 SIMDs x 64 lanes x 3): the adjoint P2G then
+// runs four lanes per particle.
+// PMPM_ADJOINT_P2G_LANES=1 or 4 in the environment overrides the choice (read
+// at every launch), so both layouts can be checked against each other.
+constexpr int kAdjointOneLaneMin = 256 * 4 * 64 * 3;
+inline int adjoint_p2g_lanes(int n) {
+  const char *e = getenv("PMPM_ADJOINT_P2G_LANES");
+  if (e && (e[0] == '1' || e[0] == '4') && e[1] == 0) return e[0] - '0';
+  return n >= kAdjointOneLaneMin ? 1 : 4;
+}
+
 template <int D, int SF>
 void launch_adjoint_particles(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &st,
                               const pmpm_adjoint &ob, const pmpm_adjoint &ib, const pmpm_spatial &part,
@@ -2822,9 +2855,12 @@ void launch_adjoint_particles(const pmpm_params &prm, const pmpm_grid &g, const 
   if (which == 0)
     hipLaunchKernelGGL((g2p_adjoint_kernel<D, SF>), dim3(blocks), dim3(256), 0, s, prm, g, st, ob, ib, part, nodes,
                        rec, pbar);
+  else if (adjoint_p2g_lanes(st.n) == 1)
+    hipLaunchKernelGGL((p2g_adjoint_kernel<D, SF, 1>), dim3(blocks), dim3(256), 0, s, prm, g, st, ib, part, nodes,
+                       nbar, rec);
   else
-    hipLaunchKernelGGL((p2g_adjoint_kernel<D, SF>), dim3(blocks), dim3(256), 0, s, prm, g, st, ib, part, nodes, nbar,
-                       rec);
+    hipLaunchKernelGGL((p2g_adjoint_kernel<D, SF, 4>), dim3((unsigned)ceil_div(st.n, 64)), dim3(256), 0, s, prm, g,
+                       st, ib, part, nodes, nbar, rec);
 }
 
 }  // namespace pmpm

@@ -277,3 +277,41 @@ def test_adjoint_with_kept_step_buffers_matches_recompute():
     pa, pb = res[1][1], res[0][1]
     assert float((pa - pb).abs().max()) <= 1e-5 * float(pb.abs().max()) + 1e-30, (pa, pb)
     assert float(pb.abs().max()) > 0
+
+
+def test_adjoint_p2g_one_and_four_lanes_agree(monkeypatch):
+    """The adjoint P2G's two layouts (one lane per particle for large
+    launches, four lanes per particle below about 196k particles) give the
+    same adjoint step (field-scaled relative error <= 1e-5: the layouts add
+    the window's entries in different orders). PMPM_ADJOINT_P2G_LANES forces
+    the layout."""
+    from pyroclastmpm_amd.diff import AdjointSet, DiffMPM, StepBuffers, ParticleSet
+
+    spec = S.column(h=0.04, per_axis=2, dt=1e-4, jitter=0.1, seed=3, musl=False)
+    model = DiffMPM(3, spec.origin, spec.end, 0.04, spec.positions, 8, dt=1e-4)
+    theta = (30.0, 20.0, 1e6)
+    _banner(f"adjoint P2G one lane vs four lanes: 3D DP column, n={model.n}, 12 warm-up steps")
+    state, _ = model.forward(theta, 12)
+    prm = model.params(theta)
+    n = model.n
+    kept = StepBuffers(n, model.grid.num_cells_total, 3)
+    model.step(prm, state, ParticleSet(n, 3), kept)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    ob = AdjointSet(n, 3)
+    for k, v in ob.t.items():
+        v.copy_(torch.randn(v.shape, device="cuda", generator=g))
+    res = []
+    for lanes in ("1", "4"):
+        monkeypatch.setenv("PMPM_ADJOINT_P2G_LANES", lanes)
+        ib = AdjointSet(n, 3)
+        pbar = torch.zeros(4, dtype=torch.float32, device="cuda")
+        model.adjoint_step(prm, state, ob, ib, pbar, work=kept, recompute=False)
+        torch.cuda.synchronize()
+        res.append((ib, pbar))
+    monkeypatch.delenv("PMPM_ADJOINT_P2G_LANES")
+    for k in ob.t:
+        a, b = res[0][0].t[k], res[1][0].t[k]
+        e = float((a - b).abs().max()) / max(float(b.abs().max()), 1e-30)
+        assert e <= 1e-5, (k, e)
+    assert float(res[1][0].t["x"].abs().max()) > 0
+    assert torch.equal(res[0][1], res[1][1]), (res[0][1], res[1][1])
