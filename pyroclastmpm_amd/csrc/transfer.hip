@@ -2640,12 +2640,13 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
   const int n = st.n;
   const int nn = g.num_cells_total;
   // LANES = 1: one lane per particle, the window's y offsets in turn;
-  // LANES = 4: four consecutive lanes per particle, one y offset each, sums
-  // combined by shuffles (for launches too small to fill the chip otherwise)
-  static_assert(LANES == 1 || LANES == 4, "lanes per particle");
-  const int j = LANES == 1 ? (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x
-                           : (int)blockIdx.x * ((int)blockDim.x / 4) + (int)threadIdx.x / 4;
-  const int sub0 = LANES == 1 ? 0 : (int)threadIdx.x & 3;
+  // LANES = 4: four consecutive lanes per particle, one y offset each;
+  // LANES = 16 (3D): sixteen lanes per particle, one (y, z) row of the window
+  // each. The particle's sums are combined by shuffles; the wider layouts are
+  // for launches too small to fill the chip otherwise.
+  static_assert(LANES == 1 || LANES == 4 || (LANES == 16 && D == 3), "lanes per particle");
+  const int j = (int)blockIdx.x * ((int)blockDim.x / LANES) + (int)threadIdx.x / LANES;
+  const int sub0 = (int)threadIdx.x % LANES;
   if (j >= n) return;  // (all lanes of a particle together)
   const int src = part.sorted[j];
   PW<D, SF> w;
@@ -2747,62 +2748,86 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
       }
     }
   }
-  // y offset `sub` of the window (values picked by selects, so the in-grid
-  // entries stay unrolled without indexing registers at run time)
-#pragma unroll 1
-  for (int sub = sub0; sub < EY; sub += LANES) {
-    auto pick = [&](const auto *a) { return sub == 0 ? a[0] : (sub == 1 ? a[1] : (sub == 2 ? a[2] : a[3])); };
-    const int oy = pick(off[1]);
-    const float Ny = pick(N[1]), dNy = pick(dN[1]), d2Ny = pick(d2N[1]);
+  // one wrap-around entry (deposit only; the wrapped node's types)
+  auto wrap_entry = [&](const int *e3) {
+    int nb[3] = {0, 0, 0}, offs[3] = {0, 0, 0};
+    bool inside = true;
 #pragma unroll
-    for (int ex = 0; ex < E; ++ex)
+    for (int a = 0; a < D; ++a) {
+      offs[a] = w.first[a] + e3[a];
+      nb[a] = w.bin[a] + offs[a];
+      inside = inside && nb[a] >= 0 && nb[a] < g.num_cells[a];
+    }
+    const int64_t lin = lin_index(g, nb, D);
+    if (inside || lin < 0 || lin >= nn) return;
+    int ids[3];
+    node_ids<D>(g, lin, ids);
+    float n3[3] = {1.f, 1.f, 1.f}, d3[3] = {0.f, 0.f, 0.f}, h3[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ez = 0; ez < EZ; ++ez) {
-        const float n3[3] = {N[0][ex], Ny, N[2][ez]}, d3[3] = {dN[0][ex], dNy, dN[2][ez]},
-                    h3[3] = {d2N[0][ex], d2Ny, d2N[2][ez]};
-        entry(off[0][ex] + oy + off[2][ez], n3, d3, h3, true);
+    for (int a = 0; a < D; ++a) {
+      const int type = cubic_node_type(ids[a], g.num_cells[a], prm.node_types_built);
+      const float r = __fsub_rn(__fsub_rn(w.t[a], (float)w.bin[a]), (float)offs[a]);
+      spline_eval<SF>(r, type, g.inv_cell_size, n3[a], d3[a]);
+      h3[a] = spline_d2<SF>(r, type, g.inv_cell_size);
+    }
+    entry((int)lin, n3, d3, h3, false);
+  };
+  // the window's offsets along y (and z) are picked by selects, so the
+  // in-grid entries stay unrolled without indexing registers at run time
+  auto pick = [](const auto *a, int k) { return k == 0 ? a[0] : (k == 1 ? a[1] : (k == 2 ? a[2] : a[3])); };
+  if constexpr (LANES == 16) {
+    const int sy = sub0 & 3, sz = sub0 >> 2;
+    if (sy < EY && sz < EZ) {
+      const int oyz = pick(off[1], sy) + pick(off[2], sz);
+      const float Ny = pick(N[1], sy), dNy = pick(dN[1], sy), d2Ny = pick(d2N[1], sy);
+      const float Nz = pick(N[2], sz), dNz = pick(dN[2], sz), d2Nz = pick(d2N[2], sz);
+#pragma unroll
+      for (int ex = 0; ex < E; ++ex) {
+        const float n3[3] = {N[0][ex], Ny, Nz}, d3[3] = {dN[0][ex], dNy, dNz}, h3[3] = {d2N[0][ex], d2Ny, d2Nz};
+        entry(off[0][ex] + oyz, n3, d3, h3, true);
       }
-    // wrap-around entries of this y offset (deposit only; the wrapped node's
-    // types)
-    if (leaves && prm.wrap_quirk) {
+      if (leaves && prm.wrap_quirk) {
 #pragma unroll 1
-      for (int e = 0; e < E * EZ; ++e) {
-        const int e3[3] = {e % E, D > 1 ? sub : 0, D > 2 ? e / E : 0};
-        int nb[3] = {0, 0, 0}, offs[3] = {0, 0, 0};
-        bool inside = true;
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-          offs[a] = w.first[a] + e3[a];
-          nb[a] = w.bin[a] + offs[a];
-          inside = inside && nb[a] >= 0 && nb[a] < g.num_cells[a];
+        for (int ex = 0; ex < E; ++ex) {
+          const int e3[3] = {ex, sy, sz};
+          wrap_entry(e3);
         }
-        const int64_t lin = lin_index(g, nb, D);
-        if (inside || lin < 0 || lin >= nn) continue;
-        int ids[3];
-        node_ids<D>(g, lin, ids);
-        float n3[3] = {1.f, 1.f, 1.f}, d3[3] = {0.f, 0.f, 0.f}, h3[3] = {0.f, 0.f, 0.f};
+      }
+    }
+  } else {
+#pragma unroll 1
+    for (int sub = sub0; sub < EY; sub += LANES) {
+      const int oy = pick(off[1], sub);
+      const float Ny = pick(N[1], sub), dNy = pick(dN[1], sub), d2Ny = pick(d2N[1], sub);
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-          const int type = cubic_node_type(ids[a], g.num_cells[a], prm.node_types_built);
-          const float r = __fsub_rn(__fsub_rn(w.t[a], (float)w.bin[a]), (float)offs[a]);
-          spline_eval<SF>(r, type, g.inv_cell_size, n3[a], d3[a]);
-          h3[a] = spline_d2<SF>(r, type, g.inv_cell_size);
+      for (int ex = 0; ex < E; ++ex)
+#pragma unroll
+        for (int ez = 0; ez < EZ; ++ez) {
+          const float n3[3] = {N[0][ex], Ny, N[2][ez]}, d3[3] = {dN[0][ex], dNy, dN[2][ez]},
+                      h3[3] = {d2N[0][ex], d2Ny, d2N[2][ez]};
+          entry(off[0][ex] + oy + off[2][ez], n3, d3, h3, true);
         }
-        entry((int)lin, n3, d3, h3, false);
+      if (leaves && prm.wrap_quirk) {
+#pragma unroll 1
+        for (int e = 0; e < E * EZ; ++e) {
+          const int e3[3] = {e % E, D > 1 ? sub : 0, D > 2 ? e / E : 0};
+          wrap_entry(e3);
+        }
       }
     }
   }
-  if constexpr (LANES == 4) {  // the particle's sums over its lanes; lane 0 goes on
-    auto sum4 = [](float v) {
-      v += __shfl_xor(v, 1, 64);
-      return v + __shfl_xor(v, 2, 64);
+  if constexpr (LANES > 1) {  // the particle's sums over its lanes; lane 0 goes on
+    auto sum_lanes = [](float v) {
+#pragma unroll
+      for (int o = 1; o < LANES; o <<= 1) v += __shfl_xor(v, o, 64);
+      return v;
     };
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      xb[a] = sum4(xb[a]);
-      vb[a] = sum4(vb[a]);
+      xb[a] = sum_lanes(xb[a]);
+      vb[a] = sum_lanes(vb[a]);
 #pragma unroll
-      for (int b = 0; b < 3; ++b) G[a][b] = sum4(G[a][b]);
+      for (int b = 0; b < 3; ++b) G[a][b] = sum_lanes(G[a][b]);
     }
     if (sub0 != 0) return;
   }
@@ -2834,16 +2859,21 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
     for (int b = a; b < D; ++b) ib.stress[sym_idx(a, b) * n + src] += a == b ? Mb[a][a] : Mb[a][b] + Mb[b][a];
 }
 
-// Below this many particles, one lane per particle leaves fewer than three
-// waves per SIMD (256 CUs x 4 SIMDs x 64 lanes x 3): the adjoint P2G then
-// runs four lanes per particle.
-// PMPM_ADJOINT_P2G_LANES=1 or 4 in the environment overrides the choice (read
-// at every launch), so both layouts can be checked against each other.
-constexpr int kAdjointOneLaneMin = 256 * 4 * 64 * 3;
-inline int adjoint_p2g_lanes(int n) {
+// Lanes per particle of the adjoint P2G: the fewest that still give three
+// waves per SIMD (256 CUs x 4 SIMDs x 64 lanes x 3 lanes in all), 16 only in
+// 3D. PMPM_ADJOINT_P2G_LANES=1, 4 or 16 in the environment overrides the
+// choice (read at every launch), so the layouts can be checked against each
+// other.
+constexpr int kAdjointLanesWanted = 256 * 4 * 64 * 3;
+inline int adjoint_p2g_lanes(int n, int dim) {
   const char *e = getenv("PMPM_ADJOINT_P2G_LANES");
-  if (e && (e[0] == '1' || e[0] == '4') && e[1] == 0) return e[0] - '0';
-  return n >= kAdjointOneLaneMin ? 1 : 4;
+  if (e) {
+    const int v = atoi(e);
+    if (v == 1 || v == 4 || (v == 16 && dim == 3)) return v;
+  }
+  if (n >= kAdjointLanesWanted) return 1;
+  if ((int64_t)n * 4 >= kAdjointLanesWanted || dim != 3) return 4;
+  return 16;
 }
 
 template <int D, int SF>
@@ -2855,12 +2885,19 @@ void launch_adjoint_particles(const pmpm_params &prm, const pmpm_grid &g, const 
   if (which == 0)
     hipLaunchKernelGGL((g2p_adjoint_kernel<D, SF>), dim3(blocks), dim3(256), 0, s, prm, g, st, ob, ib, part, nodes,
                        rec, pbar);
-  else if (adjoint_p2g_lanes(st.n) == 1)
-    hipLaunchKernelGGL((p2g_adjoint_kernel<D, SF, 1>), dim3(blocks), dim3(256), 0, s, prm, g, st, ib, part, nodes,
-                       nbar, rec);
-  else
-    hipLaunchKernelGGL((p2g_adjoint_kernel<D, SF, 4>), dim3((unsigned)ceil_div(st.n, 64)), dim3(256), 0, s, prm, g,
-                       st, ib, part, nodes, nbar, rec);
+  else {
+    const int lanes = adjoint_p2g_lanes(st.n, D);
+    const dim3 grid((unsigned)ceil_div(st.n, 256 / lanes));
+    if (lanes == 1)
+      hipLaunchKernelGGL((p2g_adjoint_kernel<D, SF, 1>), grid, dim3(256), 0, s, prm, g, st, ib, part, nodes, nbar,
+                         rec);
+    else if (lanes == 4 || D != 3)
+      hipLaunchKernelGGL((p2g_adjoint_kernel<D, SF, 4>), grid, dim3(256), 0, s, prm, g, st, ib, part, nodes, nbar,
+                         rec);
+    else
+      hipLaunchKernelGGL((p2g_adjoint_kernel<D, SF, D == 3 ? 16 : 4>), grid, dim3(256), 0, s, prm, g, st, ib, part,
+                         nodes, nbar, rec);
+  }
 }
 
 }  // namespace pmpm

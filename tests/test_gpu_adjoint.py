@@ -279,18 +279,17 @@ def test_adjoint_with_kept_step_buffers_matches_recompute():
     assert float(pb.abs().max()) > 0
 
 
-def test_adjoint_p2g_one_and_four_lanes_agree(monkeypatch):
-    """The adjoint P2G's two layouts (one lane per particle for large
-    launches, four lanes per particle below about 196k particles) give the
-    same adjoint step (field-scaled relative error <= 1e-5: the layouts add
-    the window's entries in different orders). PMPM_ADJOINT_P2G_LANES forces
-    the layout."""
+def test_adjoint_p2g_lane_layouts_agree(monkeypatch):
+    """The adjoint P2G's layouts (one, four or sixteen lanes per particle,
+    chosen by launch size) give the same adjoint step (field-scaled relative
+    error <= 1e-5: the layouts add the window's entries in different orders).
+    PMPM_ADJOINT_P2G_LANES forces the layout."""
     from pyroclastmpm_amd.diff import AdjointSet, DiffMPM, StepBuffers, ParticleSet
 
     spec = S.column(h=0.04, per_axis=2, dt=1e-4, jitter=0.1, seed=3, musl=False)
     model = DiffMPM(3, spec.origin, spec.end, 0.04, spec.positions, 8, dt=1e-4)
     theta = (30.0, 20.0, 1e6)
-    _banner(f"adjoint P2G one lane vs four lanes: 3D DP column, n={model.n}, 12 warm-up steps")
+    _banner(f"adjoint P2G lanes 1 / 4 / 16: 3D DP column, n={model.n}, 12 warm-up steps")
     state, _ = model.forward(theta, 12)
     prm = model.params(theta)
     n = model.n
@@ -301,7 +300,7 @@ def test_adjoint_p2g_one_and_four_lanes_agree(monkeypatch):
     for k, v in ob.t.items():
         v.copy_(torch.randn(v.shape, device="cuda", generator=g))
     res = []
-    for lanes in ("1", "4"):
+    for lanes in ("1", "4", "16"):
         monkeypatch.setenv("PMPM_ADJOINT_P2G_LANES", lanes)
         ib = AdjointSet(n, 3)
         pbar = torch.zeros(4, dtype=torch.float32, device="cuda")
@@ -309,9 +308,10 @@ def test_adjoint_p2g_one_and_four_lanes_agree(monkeypatch):
         torch.cuda.synchronize()
         res.append((ib, pbar))
     monkeypatch.delenv("PMPM_ADJOINT_P2G_LANES")
-    for k in ob.t:
-        a, b = res[0][0].t[k], res[1][0].t[k]
-        e = float((a - b).abs().max()) / max(float(b.abs().max()), 1e-30)
-        assert e <= 1e-5, (k, e)
-    assert float(res[1][0].t["x"].abs().max()) > 0
-    assert torch.equal(res[0][1], res[1][1]), (res[0][1], res[1][1])
+    for other in res[1:]:
+        for k in ob.t:
+            a, b = res[0][0].t[k], other[0].t[k]
+            e = float((a - b).abs().max()) / max(float(b.abs().max()), 1e-30)
+            assert e <= 1e-5, (k, e)
+        assert torch.equal(res[0][1], other[1]), (res[0][1], other[1])
+    assert float(res[0][0].t["x"].abs().max()) > 0
