@@ -462,6 +462,10 @@ template <class R, int D> struct Sim {
   std::vector<BoundaryCondition<R, D>> bcs;
   bool musl = false;
   R alpha = R(0.99);
+  // true: P2G bound-checks only the linear node index, so a cell index
+  // outside the grid on some axis can wrap into range (usl_inline.h:122-126,
+  // the reference's behaviour); false: such cells are skipped on every axis
+  bool wrap_quirk = true;
 
   Sim(const V &origin, const V &end, R cell_size, ShapeFn sf, R dt_, int ppc)
       : dt(dt_), particles_per_cell(ppc), shape_fn(sf) {
@@ -651,6 +655,11 @@ template <class R, int D> struct Sim {
   }
 
   // ---------------- P2G (usl_inline.h:82-180) ----------------
+  bool in_grid(const VI &b) const {
+    for (int a = 0; a < D; ++a)
+      if (b[a] < 0 || b[a] >= grid.num_cells[a]) return false;
+    return true;
+  }
   void p2g() {
     const int nsn = win.nsn;
     for (int n = 0; n < grid.num_cells_total; ++n) {
@@ -660,6 +669,7 @@ template <class R, int D> struct Sim {
       for (int sid = 0; sid < nsn; ++sid) {
         VI sb;
         for (int a = 0; a < D; ++a) sb[a] = nb[a] + win.p2g[sid][a];
+        if (!wrap_quirk && !in_grid(sb)) continue;
         const unsigned h = (unsigned)grid.linear(sb);
         if (h >= (unsigned)grid.num_cells_total) continue;
         const int cend = cell_end[h];
@@ -747,6 +757,7 @@ template <class R, int D> struct Sim {
       for (int sid = 0; sid < nsn; ++sid) {
         VI sb;
         for (int a = 0; a < D; ++a) sb[a] = nb[a] + win.p2g[sid][a];
+        if (!wrap_quirk && !in_grid(sb)) continue;
         const unsigned h = (unsigned)grid.linear(sb);
         if (h >= (unsigned)grid.num_cells_total) continue;
         const int cend = cell_end[h];

@@ -291,6 +291,7 @@ template <class R, int D> void bind_sim(py::module &m, const std::string &name) 
       .def("step", &S::step)
       .def("run", &S::run)
       .def_readwrite("musl", &S::musl)
+      .def_readwrite("wrap_quirk", &S::wrap_quirk)
       .def_property("alpha", [](S &s) { return (double)s.alpha; }, [](S &s, double a) { s.alpha = (R)a; })
       .def_property("dt", [](S &s) { return (double)s.dt; }, [](S &s, double a) { s.dt = (R)a; })
       .def_readwrite("global_step", &S::global_step)

@@ -245,7 +245,7 @@ PMPM_HD void dp_vjp_rev(const float *p, float etab, bool increment, bool history
       f2 = f2 + str[i][j] * str[i][j];
     }
   const float J2 = 0.5f * f2;
-  const float eta = 0.57735026918962576f * M;
+  const float eta = (float)(0.57735026918962576 * (double)M);
   const float sqJ2 = sqrtf(J2);
   const float Phi = sqJ2 + eta * (ptr - Pt);
   // ---- reverse
@@ -263,7 +263,10 @@ PMPM_HD void dp_vjp_rev(const float *p, float etab, bool increment, bool history
     const float dg = Phi / den;
     float pn = ptr - K * etab * dg;
     const float f = 1.f - (G * dg) / sqJ2;
-    const bool apex = sqJ2 - G * dg < 0.f;
+    // the branch the forward map took: its apex test runs in double
+    // (rm_drucker_prager)
+    const double dgd = (double)Phi / ((double)K * (double)eta * (double)etab + (double)G);
+    const bool apex = (double)sqJ2 - (double)G * dgd < 0.0;
     float dga = 0.f;
     if (apex) {
       if (apex_guard && etab == 0.f) {

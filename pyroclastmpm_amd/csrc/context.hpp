@@ -22,6 +22,12 @@ struct pmpm_context {
   size_t partials_cap = 0;       // floats
   int32_t *tile_flags = nullptr;  // 1 = tile had particles this launch
   size_t tile_flags_cap = 0;
+  // wrap-around sums of typed cubic splines: per (wrap cell, lane) records
+  // ([fields][cells][64] floats) and a live flag per wrap cell
+  float *wrap_rec = nullptr;
+  size_t wrap_rec_cap = 0;
+  float *wrap_live = nullptr;  // int32 flags (held in a float-sized buffer)
+  size_t wrap_live_cap = 0;
   // adjoint step: per-particle scatter records (15 floats, cell order) and
   // node adjoints (one record of 8 floats per node, 4 in 1D)
   float *adj_rec = nullptr;

@@ -391,7 +391,7 @@ __device__ inline void rm_drucker_prager(const pmpm_material &m, float dt, const
       f2 += str[i][j] * str[i][j];
     }
   const float J2 = 0.5f * f2;
-  const float eta = 0.57735026918962576f * M;
+  const float eta = (float)(0.57735026918962576 * (double)M);
   const float sqJ2 = sqrtf(J2);
   const float Phi = sqJ2 + eta * (ptr - Pt);
   if (Phi <= 0.f) {
@@ -401,37 +401,43 @@ __device__ inline void rm_drucker_prager(const pmpm_material &m, float dt, const
                 D > 2 ? et[0][2] : 0.f, D > 2 ? et[1][2] : 0.f};
     return;
   }
-  // cone (dp_rheo_inline.h:50-65)
-  float dg = Phi / (K * eta * etab + G);
-  float pn = ptr - K * etab * dg;
-  float f = 1.f - (G * dg) / sqJ2;
+  // cone (dp_rheo_inline.h:50-65): the return runs in double, as the
+  // reference's return_mapping_cone takes its moduli, trial pressure, J2 and
+  // plastic multiplier as double; the apex test compares the float sqrt(J2)
+  // with the double G dgamma (dp_rheo_inline.h:237-238)
+  const double Kd = (double)K, Gd = (double)G;
+  const double dg = (double)Phi / (Kd * (double)eta * (double)etab + Gd);
+  double pn = (double)ptr - Kd * (double)etab * dg;
+  const float f = (float)(1.0 - (Gd * dg) / sqrt((double)J2));
   float sn[3][3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) sn[i][j] = f * str[i][j];
-  if (sqJ2 - G * dg < 0.f) {  // apex (dp_rheo_inline.h:67-78)
+  if ((double)sqJ2 - Gd * dg < 0.0) {  // apex (dp_rheo_inline.h:67-78)
     if ((m.flags & PMPM_MATF_APEX_GUARD) && etab == 0.f) {
-      pn = Pt;
+      pn = (double)Pt;
     } else {
-      const float dga = (ptr - Pt) / (etab * K);
-      pn = ptr - K * etab * dga;
+      const double dga = ((double)ptr - (double)Pt) / ((double)etab * Kd);
+      pn = (double)ptr - Kd * (double)etab * dga;
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j) sn[i][j] = 0.f;
   }
-  // update_plasticity (dp_rheo_inline.h:80-96)
-  s = Sym{sn[0][0] + pn, sn[1][1] + pn, sn[2][2] + pn, sn[0][1], sn[0][2], sn[1][2]};
+  // update_plasticity (dp_rheo_inline.h:80-96): p_next stays double until the
+  // stress is stored; the volumetric strain is formed in double
+  const float pnf = (float)pn;
+  s = Sym{sn[0][0] + pnf, sn[1][1] + pnf, sn[2][2] + pnf, sn[0][1], sn[0][2], sn[1][2]};
   if (m.flags & PMPM_MATF_UPDATE_HISTORY) {
-    const float i2G = 1.f / (2.f * G);
-    const float vol = (pn - pref) / (3.f * K);
+    const float ish = (float)(2.0 * Gd);
+    const float vol = (float)((pn - (double)pref) / (3.0 * Kd));
     float e[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) e[i][j] = (sn[i][j] - sr[i][j]) * i2G + (i == j ? vol : 0.f);
+      for (int j = 0; j < 3; ++j) e[i][j] = (sn[i][j] - sr[i][j]) / ish + (i == j ? vol : 0.f);
     eps = Sym{e[0][0], D > 1 ? e[1][1] : 0.f, D > 2 ? e[2][2] : 0.f, D > 1 ? e[0][1] : 0.f,
               D > 2 ? e[0][2] : 0.f, D > 2 ? e[1][2] : 0.f};
   }

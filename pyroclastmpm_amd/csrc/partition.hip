@@ -285,6 +285,8 @@ extern "C" int pmpm_context_destroy(pmpm_context *ctx) {
   (void)hipFree(ctx->adj_pb);
   (void)hipFree(ctx->partials);
   (void)hipFree(ctx->tile_flags);
+  (void)hipFree(ctx->wrap_rec);
+  (void)hipFree(ctx->wrap_live);
   (void)hipFree(ctx->keys_tmp);
   (void)hipFree(ctx->vals_tmp);
   (void)hipFree(ctx->hist);

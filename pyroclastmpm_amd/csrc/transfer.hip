@@ -468,28 +468,6 @@ __device__ void p2g_stray(const pmpm_params &prm, const pmpm_grid &g, const pmpm
     }
 }
 
-// Sums r (P2G field order) of node vector nb, which lies outside the grid,
-// into the node its linear index reaches, if that index is in range.
-template <int D, int MODE>
-__device__ inline void deposit_wrapped(const pmpm_grid &g, const pmpm_nodes &nodes, const int *nb, const float *r) {
-  const int nn = g.num_cells_total;
-  const int64_t lin = lin_index(g, nb, D);
-  if (lin < 0 || lin >= nn) return;
-  const int node = (int)lin;
-  if (MODE == 0) {
-    if (r[0] != 0.f) unsafeAtomicAdd(&nodes.mass[node], r[0]);
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      if (r[1 + a] != 0.f) unsafeAtomicAdd(&nodes.mom[a * nn + node], r[1 + a]);
-      if (r[1 + D + a] != 0.f) unsafeAtomicAdd(&nodes.mom_nt[a * nn + node], r[1 + D + a]);
-    }
-  } else {
-#pragma unroll
-    for (int a = 0; a < D; ++a)
-      if (r[a] != 0.f) unsafeAtomicAdd(&nodes.mom_nt[a * nn + node], r[a]);
-  }
-}
-
 // Per-particle P2G inputs staged in LDS, one float (or int bits) per field,
 // field-major within a wave's region:
 //   frac[D]  position in the cell (t - bin, as the weights use it); MODE 1
@@ -719,13 +697,11 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
   bool x_in[SPAN], z_in[SPAN];
   const int yn = c[1] + NLO + sy;
   const bool y_in = D < 2 || (yn >= 0 && yn < g.num_cells[1]);
-  bool all_in = y_in;
 #pragma unroll
   for (int s = 0; s < SPAN; ++s) {
     const int xn = c[0] + NLO + s, zn = c[2] + NLO + s;
     x_in[s] = xn >= 0 && xn < g.num_cells[0];
     z_in[s] = D < 3 || (zn >= 0 && zn < g.num_cells[2]);
-    all_in = all_in && x_in[s] && (s >= SZ || z_in[s]);
   }
 
   float acc[NE][NV];
@@ -1084,35 +1060,26 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
     }
   }
 
-  // in-grid sums into the LDS tile, one node offset (sx, sz) per phase; the
-  // waves holding that z offset write
+  // sums into the LDS tile, one node offset (sx, sz) per phase; the waves
+  // holding that z offset write. Node offsets outside the grid stay in the
+  // tile too when the wrap-around is reproduced with weights that do not
+  // depend on node types (linear, quadratic): the grid kernel adds such a
+  // virtual node's sum to the node its linear index reaches, in a fixed
+  // order. With typed cubic splines the wrapped node's types change the
+  // weights, and p2g_wrap_kernel forms those sums instead. (MODE 2 transposes
+  // G2P, which never reads wrapped nodes.)
+  const bool keep_out = MODE != 2 && prm.wrap_quirk && !(SF == PMPM_SF_CUBIC && prm.node_types_built);
 #pragma unroll
   for (int sx = 0; sx < SPAN; ++sx)
 #pragma unroll
     for (int sz = 0; sz < SZ; ++sz) {
       __syncthreads();
-      if (sz / SZH == zh && x_in[sx] && y_in && z_in[sz]) {
+      if (sz / SZH == zh && (keep_out || (x_in[sx] && y_in && z_in[sz]))) {
         const int li = (lx + sx) + NXP * (sy + NY * (lz + sz));
 #pragma unroll
         for (int k = 0; k < NV; ++k) tile[k * NT + li] += acc[sx * SZH + sz % SZH][k];
       }
     }
-
-  // wrap-around deposits of node offsets outside the grid; with typed cubic
-  // splines the weights differ (wrapped node's types) and p2g_wrap_kernel
-  // deposits them instead
-  // (MODE 2 transposes G2P, which never reads wrapped nodes)
-  if (MODE != 2 && !all_in && prm.wrap_quirk && !(SF == PMPM_SF_CUBIC && prm.node_types_built)) {
-#pragma unroll
-    for (int sx = 0; sx < SPAN; ++sx)
-#pragma unroll
-      for (int s = 0; s < SZH; ++s) {
-        const int sz = zh * SZH + s;
-        if (x_in[sx] && y_in && z_in[sz]) continue;
-        const int nb[3] = {c[0] + NLO + sx, c[1] + NLO + sy, c[2] + NLO + sz};
-        deposit_wrapped<D, MODE>(g, nodes, nb, acc[sx * SZH + s]);
-      }
-  }
 
   __syncthreads();
   constexpr int NX = G::NX, NTN = G::NTN;
@@ -1374,29 +1341,56 @@ template <int D> __host__ inline int wrap_cell_count(const pmpm_grid &g) {
   return 3 * ny * (D > 2 ? nz : 1) + (D > 2 && nx > 3 ? 3 * (nx - 3) * nz : 0);
 }
 
-// Wrap-around deposits of typed cubic splines: one wave per cell that can
-// wrap, lane = node offset (sx, sy, sz) of the cell's span. A lane whose node
-// lies outside the grid but whose linear index is in range re-sums the cell's
-// particles with the types of the node that index reaches
-// (shapefunction.cpp:96-112) and adds the sums there with global atomics.
-// The cell's particles are first loaded into LDS by the whole wave (lane =
+// Index of wrap cell c in the order of wrap_cell (its inverse), or -1 when c
+// is not one of the cells that can wrap.
+template <int D> __device__ inline int wrap_index(const pmpm_grid &g, const int *c) {
+  if (D < 2) return -1;
+  const int nx = g.num_cells[0], ny = g.num_cells[1], nz = D > 2 ? g.num_cells[2] : 1;
+  auto slot = [](int v, int len) { return v == 0 ? 0 : (v >= len - 2 ? v - (len - 3) : -1); };
+  const int i = slot(c[0], nx);
+  if (i >= 0) return i + 3 * (c[1] + ny * (D > 2 ? c[2] : 0));
+  if (D < 3) return -1;
+  const int j = slot(c[1], ny);
+  if (j < 0) return -1;
+  return 3 * ny * nz + j + 3 * ((c[0] - 1) + (nx - 3) * c[2]);
+}
+
+// Wrap-around sums of typed cubic splines: one wave per cell that can wrap
+// (four cells per workgroup), lane = node offset (sx, sy, sz) of the cell's
+// span. A lane whose node lies outside the grid but whose linear index is in
+// range re-sums the cell's particles with the types of the node that index
+// reaches (shapefunction.cpp:96-112). The sums are stored per (cell, lane) in
+// `rec` ([NV][wcells][64], zero for the other lanes of a live cell) and
+// `live[w]` says whether cell w wrote any; the grid kernel adds them to their
+// nodes in a fixed order, so the node sums do not depend on scheduling.
+// The cell's particles are first loaded into LDS by the wave (lane =
 // particle, up to 64 per round), so the per-node loops read LDS instead of
 // walking a chain of dependent global loads.
+constexpr int kWrapCellsPerBlock = 4;
 template <int D, int MODE>
-__global__ __launch_bounds__(64) void p2g_wrap_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles in,
-                                                      pmpm_particles out, pmpm_spatial part, pmpm_nodes nodes) {
+__global__ __launch_bounds__(64 * kWrapCellsPerBlock) void p2g_wrap_kernel(pmpm_params prm, pmpm_grid g,
+                                                                           pmpm_particles in, pmpm_particles out,
+                                                                           pmpm_spatial part, float *__restrict__ rec,
+                                                                           int32_t *__restrict__ live, int wcells) {
   constexpr int SPAN = 4, NLO = -1;
   constexpr int NV = MODE == 0 ? 1 + 2 * D : D;
   constexpr int NS = MODE == 0 ? 6 : 1;  // staged stress terms (-V sigma)
-  __shared__ float s_frac[3][64], s_v[3][64], s_m[64], s_A[NS][64];
-  __shared__ int s_own[64];
+  constexpr int WPB = kWrapCellsPerBlock;
+  __shared__ float s_frac_all[WPB][3][64], s_v_all[WPB][3][64], s_m_all[WPB][64], s_A_all[WPB][NS][64];
+  __shared__ int s_own_all[WPB][64];
+  // every wave works on its own cell; only wave-level synchronisation below
+  const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  const int w = (int)blockIdx.x * WPB + wv;
+  float(*s_frac)[64] = s_frac_all[wv];
+  float(*s_v)[64] = s_v_all[wv];
+  float(*s_A)[64] = s_A_all[wv];
+  float *s_m = s_m_all[wv];
+  int *s_own = s_own_all[wv];
   int c[3] = {0, 0, 0};
-  if (!wrap_cell<D>(g, blockIdx.x, c)) return;  // uniform over the block
+  if (w >= wcells || !wrap_cell<D>(g, w, c)) return;  // uniform over the wave
   const int cell = (int)lin_index(g, c, D);
   const int ce = part.cell_end[cell];
   const int cs = part.cell_start[cell];
-  if (ce <= cs) return;  // uniform
-  const int lane = threadIdx.x;
   const int s[3] = {lane % SPAN, D > 1 ? (lane / SPAN) % SPAN : 0, D > 2 ? lane / (SPAN * SPAN) : 0};
   const int nlanes = D == 3 ? 64 : (D == 2 ? 16 : 4);
   int nb[3] = {0, 0, 0};
@@ -1408,8 +1402,10 @@ __global__ __launch_bounds__(64) void p2g_wrap_kernel(pmpm_params prm, pmpm_grid
   }
   const int nn = g.num_cells_total;
   const int64_t lin = lin_index(g, nb, D);
-  const bool active = lane < nlanes && !inside && lin >= 0 && lin < nn;
-  if (!__syncthreads_or(active)) return;  // uniform
+  const bool active = ce > cs && lane < nlanes && !inside && lin >= 0 && lin < nn;
+  const bool any = __ballot(active) != 0;  // uniform over the wave
+  if (lane == 0) live[w] = any ? 1 : 0;
+  if (!any) return;
   // r = frac - (NLO + s) lies in piece 3 - s of the wrapped node's spline
   // (the same piece argument as span_eval): its coefficients are fixed per
   // lane
@@ -1458,7 +1454,7 @@ __global__ __launch_bounds__(64) void p2g_wrap_kernel(pmpm_params prm, pmpm_grid
         s_A[3][lane] = r.A.xy; s_A[4][lane] = r.A.xz; s_A[5][lane] = r.A.yz;
       }
     }
-    __syncthreads();
+    wave_lds_sync();
     if (active) {
       for (int k = 0; k < cnt; ++k) {
         if (!s_own[k]) continue;  // deposited whole by p2g_stray
@@ -1487,9 +1483,10 @@ __global__ __launch_bounds__(64) void p2g_wrap_kernel(pmpm_params prm, pmpm_grid
         }
       }
     }
-    __syncthreads();
+    wave_lds_sync();
   }
-  if (active) deposit_wrapped<D, MODE>(g, nodes, nb, acc);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) rec[((int64_t)k * wcells + w) * 64 + lane] = active ? acc[k] : 0.f;
 }
 
 // Geometry of the P2G tiles for the grid kernel's gather.
@@ -1500,6 +1497,14 @@ struct TileGeom {
   int nlo;       // first node offset of a tile block
   int nv;        // fields per node in the partials
   int gather;    // 0: none, 1: P2G (mass, momentum, force), 2: MUSL re-map (momentum)
+  // wrap-around sums (the reference bound-checks only the linear node index,
+  // usl_inline.h:122-126): 0 none; 1 the out-of-grid nodes of the tile
+  // partials (weights without node types); 2 the per-(cell, lane) records of
+  // p2g_wrap_kernel (typed cubic)
+  int wrap;
+  int wcells;
+  const float *wrec;
+  const int32_t *wlive;
 };
 
 // Sums the NV fields of the partial slices of the tiles whose node block
@@ -1559,6 +1564,67 @@ __device__ inline bool gather_partials(const TileGeom &tg, const float *partials
   return true;
 }
 
+// Adds to acc the wrap-around sums that land on node `node` (coordinates ids):
+// those of every node vector nb outside the grid whose linear index is
+// `node`, nb.x = ids.x + kx nx, nb.y ≡ (node - nb.x) / nx (mod ny), within the
+// nodes a cell's window reaches ([NLO, n - 1 + NLO + SPAN - 1] per axis), in
+// a fixed order (kx, then ky ascending). Each nb's sum is its virtual node in
+// the tile partials (wrap = 1) or the records of the wrap cells whose window
+// holds it (wrap = 2: cells c = nb - NLO - s, s in [0, 4)^D, lane s).
+template <int D, int NV>
+__device__ inline void gather_wrapped(const pmpm_grid &g, const TileGeom &tg, const float *partials,
+                                      const int32_t *flags, int node, float *acc) {
+  if (D < 2) return;  // the only axis is the slowest: nothing wraps
+  const int nx = g.num_cells[0], ny = g.num_cells[1], nz = D > 2 ? g.num_cells[2] : 1;
+  const int span = tg.N[0] - tg.T[0] + 1;
+  const int lo = tg.nlo, hi = tg.nlo + span - 1;
+  const int tx = node % nx;
+#pragma unroll 1
+  for (int kx = -2; kx <= 2; ++kx) {
+    const int bx = tx + kx * nx;
+    if (bx < lo || bx > nx - 1 + hi) continue;
+    const int64_t q = ((int64_t)node - bx) / nx;  // exact: node - bx is a multiple of nx
+    const int qy = (int)(((q % ny) + ny) % ny);
+#pragma unroll 1
+    for (int ky = -2; ky <= 2; ++ky) {
+      const int by = qy + ky * ny;
+      if (by < lo || by > ny - 1 + hi) continue;
+      const int64_t qz = (q - by) / ny;
+      if (D == 2 && qz != 0) continue;
+      if (D == 3 && (qz < lo || qz > nz - 1 + hi)) continue;
+      const int nb[3] = {bx, by, D > 2 ? (int)qz : 0};
+      if (bx >= 0 && bx < nx && by >= 0 && by < ny && (D < 3 || (qz >= 0 && qz < nz))) continue;  // the node itself
+      if (tg.wrap == 1) {
+        gather_partials<D, NV>(tg, partials, flags, nb, acc);
+        continue;
+      }
+#pragma unroll 1
+      for (int sz = 0; sz < (D > 2 ? 4 : 1); ++sz)
+#pragma unroll 1
+        for (int sy = 0; sy < 4; ++sy)
+#pragma unroll 1
+          for (int sx = 0; sx < 4; ++sx) {
+            const int c[3] = {nb[0] + 1 - sx, nb[1] + 1 - sy, D > 2 ? nb[2] + 1 - sz : 0};
+            if (c[0] < 0 || c[0] >= nx || c[1] < 0 || c[1] >= ny || (D > 2 && (c[2] < 0 || c[2] >= nz))) continue;
+            const int w = wrap_index<D>(g, c);
+            if (w < 0 || !tg.wlive[w]) continue;
+            const int lane = sx + 4 * (sy + 4 * sz);
+#pragma unroll
+            for (int k = 0; k < NV; ++k) acc[k] += tg.wrec[((int64_t)k * tg.wcells + w) * 64 + lane];
+          }
+    }
+  }
+}
+// Nodes the wrap-around can reach: x in {0, 1, nx - 1} or, in 3D, y in
+// {0, 1, ny - 1} (a node vector outside the grid moves by -+nx along x or
+// -+ny along y when its linear index is read back).
+template <int D> __device__ inline bool wrap_target(const pmpm_grid &g, const int *ids) {
+  if (D < 2) return false;
+  const bool xt = ids[0] <= 1 || ids[0] == g.num_cells[0] - 1;
+  if (D == 2) return xt;
+  return xt || ids[1] <= 1 || ids[1] == g.num_cells[1] - 1;
+}
+
 // Per node: cutoff + forward moment (finalize), boundary conditions
 // [bc_begin, bc_end) in list order, then the node velocities G2P reads.
 // remap_pass: mom_nt holds the MUSL re-mapped momentum; only kinematic
@@ -1582,6 +1648,10 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
     float acc[1 + 2 * 3] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (tg.gather == 1) gather_partials<D, 1 + 2 * D>(tg, partials, flags, ids0, acc);
     else gather_partials<D, D>(tg, partials, flags, ids0, acc);
+    if (tg.wrap && wrap_target<D>(g, ids0)) {
+      if (tg.gather == 1) gather_wrapped<D, 1 + 2 * D>(g, tg, partials, flags, i, acc);
+      else gather_wrapped<D, D>(g, tg, partials, flags, i, acc);
+    }
     if (tg.gather == 1) {
       m += acc[0];
 #pragma unroll
@@ -2026,6 +2096,16 @@ __global__ void shape_function_kernel(pmpm_grid g, int types_built, const float 
     default: return PMPM_ERR_INVALID;                                             \
   }
 
+// How the wrap-around deposits of a P2G pass are summed (TileGeom::wrap).
+static int wrap_mode(const pmpm_params &prm, const pmpm_grid &g) {
+  if (!prm.wrap_quirk || prm.dim < 2) return 0;
+  if (prm.shape_fn == PMPM_SF_CUBIC) {
+    if (!prm.node_types_built) return 0;  // untyped cubic weights are zero
+    return (prm.dim == 2 ? wrap_cell_count<2>(g) : wrap_cell_count<3>(g)) > 0 ? 2 : 0;
+  }
+  return 1;
+}
+
 template <int D, int SF> TileGeom tile_geom_of(const pmpm_grid &g, int gather) {
   using G = P2GGeom<D, SF>;
   TileGeom tg{};
@@ -2060,7 +2140,7 @@ static size_t tile_count(const TileGeom &tg) { return (size_t)tg.tiles[0] * tg.t
 template <int D, int SF>
 void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
                 const pmpm_spatial &part, const pmpm_nodes &nodes, int mode, float *partials, int32_t *flags,
-                hipStream_t s, const float *rec = nullptr) {
+                float *wrec, int32_t *wlive, hipStream_t s, const float *rec = nullptr) {
   const TileGeom tg = tile_geom_of<D, SF>(g, 1);
   const dim3 grid((unsigned)tile_count(tg)), block(P2GGeom<D, SF>::THREADS);
   // 3D typed cubic P2G and MUSL re-map on the matrix cores (the re-map after
@@ -2094,11 +2174,12 @@ void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles
                        tg.tiles[1], partials, flags, rec);
   }
   const int wcells = wrap_cell_count<D>(g);
-  if (mode != 2 && SF == PMPM_SF_CUBIC && prm.node_types_built && prm.wrap_quirk && wcells > 0) {
+  if (mode != 2 && wrap_mode(prm, g) == 2) {
+    const dim3 wgrid((unsigned)ceil_div(wcells, kWrapCellsPerBlock)), wblock(64 * kWrapCellsPerBlock);
     if (mode == 0)
-      hipLaunchKernelGGL((p2g_wrap_kernel<D, 0>), dim3(wcells), dim3(64), 0, s, prm, g, in, out, part, nodes);
+      hipLaunchKernelGGL((p2g_wrap_kernel<D, 0>), wgrid, wblock, 0, s, prm, g, in, out, part, wrec, wlive, wcells);
     else
-      hipLaunchKernelGGL((p2g_wrap_kernel<D, 1>), dim3(wcells), dim3(64), 0, s, prm, g, in, out, part, nodes);
+      hipLaunchKernelGGL((p2g_wrap_kernel<D, 1>), wgrid, wblock, 0, s, prm, g, in, out, part, wrec, wlive, wcells);
   }
 }
 template <int D, int SF>
@@ -2121,9 +2202,17 @@ void launch_shape(const pmpm_grid &g, int types_built, const float *x, int n, fl
 int launch_grid(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const pmpm_nodes &nodes, int finalize,
                 int b0, int b1, int remap, int gather, hipStream_t s, int clear_mom_nt = 0) {
   const unsigned blocks = (unsigned)ceil_div(g.num_cells_total, 256);
-  const TileGeom tg = tile_geom(prm.dim, prm.shape_fn, g, gather);
+  TileGeom tg = tile_geom(prm.dim, prm.shape_fn, g, gather);
   const float *partials = gather ? ctx->partials : nullptr;
   const int32_t *flags = gather ? ctx->tile_flags : nullptr;
+  if (gather == 1 || gather == 2) {
+    tg.wrap = wrap_mode(prm, g);
+    if (tg.wrap == 2) {
+      tg.wcells = prm.dim == 2 ? wrap_cell_count<2>(g) : wrap_cell_count<3>(g);
+      tg.wrec = ctx->wrap_rec;
+      tg.wlive = (const int32_t *)ctx->wrap_live;
+    }
+  }
   switch (prm.dim) {
     case 1: hipLaunchKernelGGL(grid_kernel<1>, dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt); break;
     case 2: hipLaunchKernelGGL(grid_kernel<2>, dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt); break;
@@ -2160,7 +2249,12 @@ static int reserve_tiles(pmpm_context *ctx, const pmpm_params &prm, const pmpm_g
   const TileGeom tg = tile_geom(prm.dim, prm.shape_fn, g, 1);
   const size_t tiles = tile_count(tg);
   const size_t ntn = (size_t)tg.N[0] * tg.N[1] * tg.N[2];
-  return ensure_partials(ctx, tiles * tg.nv * ntn, tiles);
+  int rc = ensure_partials(ctx, tiles * tg.nv * ntn, tiles);
+  if (rc != PMPM_OK || wrap_mode(prm, g) != 2) return rc;
+  const size_t wcells = (size_t)(prm.dim == 2 ? wrap_cell_count<2>(g) : wrap_cell_count<3>(g));
+  if ((rc = ensure_buffer(ctx->wrap_rec, ctx->wrap_rec_cap, (size_t)(1 + 2 * prm.dim) * wcells * 64)) != PMPM_OK)
+    return rc;
+  return ensure_buffer(ctx->wrap_live, ctx->wrap_live_cap, wcells);
 }
 
 // Scatter into the tile partials (+ wrap-around deposits into the cleared
@@ -2172,7 +2266,7 @@ int do_p2g(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const 
   if (rc != PMPM_OK) return rc;
   if (!zeroed && (rc = do_zero_nodes(prm, g, nodes, s)) != PMPM_OK) return rc;
   PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, p, p, part, nodes, 0, ctx->partials, ctx->tile_flags,
-                    s);
+                    ctx->wrap_rec, (int32_t *)ctx->wrap_live, s);
   PMPM_LAUNCH_CHECK();
   return PMPM_OK;
 }
@@ -2198,7 +2292,7 @@ int do_musl_map(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, c
   // for the re-map's wrap-around atomics first
   if (!zeroed && (rc = zero_fill(nodes.mom_nt, (int64_t)g.num_cells_total * prm.dim, s)) != PMPM_OK) return rc;
   PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, in, out, part, nodes, 1, ctx->partials,
-                    ctx->tile_flags, s);
+                    ctx->tile_flags, ctx->wrap_rec, (int32_t *)ctx->wrap_live, s);
   PMPM_LAUNCH_CHECK();
   return PMPM_OK;
 }
@@ -2952,7 +3046,7 @@ extern "C" int pmpm_adjoint_step(pmpm_context *ctx, const pmpm_params *prm, cons
     PMPM_LAUNCH_CHECK();
   }
   PMPM_DISPATCH_DSF(D, prm->shape_fn, launch_p2g, *prm, g, *state, *state, *part, *nodes, 2, ctx->partials,
-                    ctx->tile_flags, s, ctx->adj_rec);
+                    ctx->tile_flags, nullptr, nullptr, s, ctx->adj_rec);
   PMPM_LAUNCH_CHECK();
   // grid adjoint
   const TileGeom tg = tile_geom(D, prm->shape_fn, g, 3);

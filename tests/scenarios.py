@@ -185,6 +185,7 @@ def build_oracle(mod, spec: Spec, prec="f32"):
         s.add_bc(k, dict(p))
     s.musl = spec.musl
     s.alpha = spec.alpha
+    s.wrap_quirk = spec.wrap_quirk
     s.initialize()
     return s
 
