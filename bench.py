@@ -13,8 +13,10 @@ resident in HBM before the timed region. `value` times pmpm_step (the C ABI's
 whole-step entry point) on one stream; a second timed region issues the same
 step entry point by entry point (pmpm_partition, pmpm_p2g, pmpm_grid_bc,
 pmpm_musl_map, pmpm_grid_bc, pmpm_musl_g2p) with HIP events on the launch
-stream around each, which gives the per-entry split and the roofline
-kernel's own launch time (pmpm_musl_g2p launches only that kernel).
+stream around each (the per-entry split); a third runs pmpm_step with the
+library's kernel timing on (a HIP event pair around every kernel, on its
+launch stream), which gives each kernel's mean launch time. The roofline
+kernel is the one with the largest share of the step.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -44,14 +46,26 @@ BYTES_PER_PARTICLE = {
     "musl_g2p": 252,     # in: sorted 4 x 12 F 36 V0 4 stress 24 eps 24 m 4 colour 1 flags 1 id 4
                          # out: F 36 V 4 stress 24 eps 24 L 36 m 4 V0 4 colour 1 flags 1 id 4
 }
-# The roofline kernel: pmpm_musl_g2p launches exactly one kernel (the second
-# map with the return map), the one that moves the most bytes, so its HIP
-# event time is that kernel's duration.
-ROOFLINE_ENTRY = "musl_g2p"
-ROOFLINE_KERNEL = "g2p_kernel<3, 2, 2>"
+# Algorithmic HBM bytes of each kernel of the step, per particle and per grid
+# node of one launch (the compulsory reads and writes of a fused design).
+KERNEL_BYTES = {
+    "hash_kernel<3>": (16, 36),             # x 12 in, key 4 out; clears cell ranges 8 + node m, P, Q 28
+    "radix_hist_kernel<10>": (4, 0),        # key 4
+    "radix_scatter_kernel<10>": (16, 0),    # key + slot 8 in, 8 out
+    "cell_range_kernel": (4, 0),            # sorted key 4
+    "p2g_mfma_kernel<2, 0>": (56, 0),       # x 12, v 12, m 4, V 4, stress 24 (P2G of the particle state)
+    "p2g_mfma_kernel<2, 1>": (36, 0),       # re-map: sorted 4, hash 4, x 12, m 4, v' 12
+    "g2p_kernel<3, 2, 1>": (52, 0),         # first map: sorted 4, x 12, v 12 in; x 12, v 12 out
+    "g2p_kernel<3, 2, 2>": (252, 0),        # second map + return map (entry musl_g2p above)
+    "grid_kernel<3> gather 1": (0, 88),     # m, P, Q in 28; m, P, Q out 28, velocity record 32
+    "grid_kernel<3> gather 2": (0, 72),     # m, P, Q in 28; Q out 12, velocity record 32
+}
+# The step's algorithmic bytes per particle (SURVEY-style budget: sort 16,
+# P2G 56, MUSL map 52, second map + return map 188 + L 36, grid ~23).
+STEP_BYTES_PER_PARTICLE = 335
 # HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE)
 # and scaled by the same counters on a plain device copy of known size.
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "g2p_return_map_dp_column_1080k_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_kernels_dp_column_1080k_traffic.json")
 
 
 def parse():
@@ -405,17 +419,42 @@ def main():
                 for i, nm in enumerate(names)}
     print("bench: per-entry-point mean ms: " + json.dumps({k: round(v, 4) for k, v in per_call.items()}),
           file=sys.stderr, flush=True)
-    dom = ROOFLINE_ENTRY
-    achieved = BYTES_PER_PARTICLE[dom] * n / (per_call[dom] * 1e-3) / 1e9
+    # timed region 3: the fused step again, every kernel bracketed by HIP
+    # events on its launch stream (the library's kernel timing)
+    capi = wl.capi
+    capi.timing_read()
+    capi.timing_enable(True)
+    for k in range(args.steps):
+        wl.fused_step()
+    capi.timing_enable(False)
+    timed = capi.timing_read()
+    kernels = {}
+    for name, (cnt, ms) in timed.items():
+        avg_ms = ms / cnt
+        row = {"launches_per_step": round(cnt / args.steps, 3), "avg_us": round(avg_ms * 1e3, 2),
+               "share": round(ms / sum(v[1] for v in timed.values()), 4)}
+        bp = KERNEL_BYTES.get(name)
+        if bp is not None:
+            b = bp[0] * n + bp[1] * wl.nn
+            row["bytes_per_launch"] = b
+            row["gbs"] = round(b / (avg_ms * 1e-3) / 1e9, 1)
+            row["frac"] = round(row["gbs"] / HBM_PEAK_GBS, 4)
+        kernels[name] = row
+    print("bench: kernels " + json.dumps(kernels), file=sys.stderr, flush=True)
+    # the roofline kernel: the one with the largest share of the step time
+    dom = max(kernels, key=lambda k: kernels[k]["share"])
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            t = json.load(f).get(ROOFLINE_KERNEL)
+            t = json.load(f).get(dom)
         if t and t.get("particles") == n:
             traffic = t["bytes_per_launch"]
-    roof = {"bound": "hbm", "kernel": ROOFLINE_KERNEL, "entry_point": dom, "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "bytes_per_particle": BYTES_PER_PARTICLE[dom], "kernel_ms": round(per_call[dom], 4)}
+    d = kernels[dom]
+    roof = {"bound": "hbm", "kernel": dom, "achieved": d.get("gbs"), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": d.get("frac"), "traffic": traffic, "bytes_per_launch": d.get("bytes_per_launch"),
+            "kernel_us": d["avg_us"], "share_of_step": d["share"],
+            "bytes_per_particle": KERNEL_BYTES.get(dom, (None, None))[0],
+            "step_frac": round(STEP_BYTES_PER_PARTICLE * n / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -432,6 +471,7 @@ def main():
                    "material": "Drucker-Prager phi=30 psi=0 c=0", "entry_point": "pmpm_step",
                    "per_entry_point_ms": per_call, "entry_by_entry_ms_per_step": round(entries_ms_per_step, 4)},
         "roofline": roof,
+        "kernels": kernels,
         "cpu_baseline": cpu,
     }
     if not args.no_adjoint:

@@ -379,6 +379,20 @@ typedef struct pmpm_adjoint_scene {
 int pmpm_batch_adjoint_step(pmpm_batch *batch, const pmpm_adjoint_scene *scenes, int32_t count,
                             pmpm_stream_t stream);
 
+/* ---- kernel timing (instrumentation; the reference's only timing hook is
+ * the "Done Elapsed" stopwatch around Solver::run, solver.cpp:153-169).
+ * With timing enabled, every kernel that the calling thread's entry points
+ * launch is bracketed by a pair of HIP events recorded on its launch stream
+ * (thread-local state; leave it off while capturing a HIP graph).
+ * pmpm_timing_read waits for those events and returns, per kernel name in
+ * first-launch order, the launch count and the summed event time in
+ * milliseconds, then forgets them. `names` receives the names '\n'-separated
+ * (truncated to names_len - 1 bytes); the first max_kernels counts and times
+ * go to counts[] and ms[]; *num_kernels is the number of names. */
+int pmpm_timing_enable(int32_t enable);
+int pmpm_timing_read(char *names, int32_t names_len, int32_t *counts, float *ms, int32_t max_kernels,
+                     int32_t *num_kernels);
+
 /* The last HIP failure an entry point met on the calling thread, as
  * "call: hipGetErrorString (code)", or "" if none (thread-local storage). */
 const char *pmpm_last_error(void);

@@ -21,7 +21,7 @@ BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["partition.hip", "transfer.hip", "step.hip", "rigid.hip", "batch.hip"]
+HIP_SOURCES = ["partition.hip", "transfer.hip", "step.hip", "rigid.hip", "batch.hip", "timing.hip"]
 HOST_SOURCES = ["host.cpp", "bind.cpp"]
 
 DEVICE_FLAGS = [

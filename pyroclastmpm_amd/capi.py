@@ -122,6 +122,8 @@ PROTOTYPES = {
     "pmpm_batch_autotune": (C.c_int, [vp, P(Scene), C.c_int32, C.c_int32, C.c_int32, vp, P(BatchTuning)]),
     "pmpm_batch_get_tuning": (C.c_int, [vp, P(BatchTuning)]),
     "pmpm_batch_adjoint_step": (C.c_int, [vp, P(AdjointScene), C.c_int32, vp]),
+    "pmpm_timing_enable": (C.c_int, [C.c_int32]),
+    "pmpm_timing_read": (C.c_int, [C.c_char_p, C.c_int32, P(C.c_int32), P(C.c_float), C.c_int32, P(C.c_int32)]),
     "pmpm_last_error": (C.c_char_p, []),
     "pmpm_version": (C.c_char_p, []),
 }
@@ -191,3 +193,20 @@ class Context:
                 self.ptr = vp()
         except Exception:
             pass
+
+
+def timing_enable(on=True):
+    """Per-thread kernel timing with HIP events (pmpm_timing_enable)."""
+    check(lib().pmpm_timing_enable(1 if on else 0), "pmpm_timing_enable")
+
+
+def timing_read(max_kernels=64):
+    """{kernel name: (launches, total ms)} since the last read
+    (pmpm_timing_read; waits for the recorded events)."""
+    buf = C.create_string_buffer(1 << 16)
+    counts = (C.c_int32 * max_kernels)()
+    ms = (C.c_float * max_kernels)()
+    nk = C.c_int32(0)
+    check(lib().pmpm_timing_read(buf, len(buf), counts, ms, max_kernels, C.byref(nk)), "pmpm_timing_read")
+    names = buf.value.decode().split("\n") if nk.value else []
+    return {names[k]: (int(counts[k]), float(ms[k])) for k in range(min(nk.value, max_kernels))}

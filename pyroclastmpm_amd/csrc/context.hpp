@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <initializer_list>
+#include <string>
+
 #include "../../include/pmpm.h"
 
 struct pmpm_context {
@@ -87,6 +90,38 @@ inline bool hip_ok(hipError_t e, const char *what) {
       ::pmpm::note_hip_error(e_, "kernel launch");        \
       return PMPM_ERR_HIP;                                \
     }                                                     \
+  } while (0)
+
+// Kernel timing (pmpm_timing_enable): a TimedRegion around a launch records
+// a HIP event pair on the launch stream when the calling thread has timing on;
+// the name is only formed then.
+bool timing_enabled();
+std::string kname(const char *base, std::initializer_list<long> targs);
+class TimedRegion {
+ public:
+  template <class F> TimedRegion(hipStream_t s, F &&name) {
+    if (timing_enabled()) start(s, name());
+  }
+  ~TimedRegion() {
+    if (on_) finish();
+  }
+  TimedRegion(const TimedRegion &) = delete;
+  TimedRegion &operator=(const TimedRegion &) = delete;
+
+ private:
+  void start(hipStream_t s, std::string name);
+  void finish();
+  bool on_ = false;
+  hipStream_t s_ = nullptr;
+  hipEvent_t a_ = nullptr, b_ = nullptr;
+  std::string name_;
+};
+// hipLaunchKernelGGL with the launch timed under `name` (an expression giving
+// a std::string, in parentheses) when timing is on.
+#define PMPM_LAUNCH(kernel, name, grid, block, shm, stream, ...)                 \
+  do {                                                                           \
+    ::pmpm::TimedRegion pmpm_timed_((stream), [&]() { return std::string name; }); \
+    hipLaunchKernelGGL(kernel, grid, block, shm, stream, __VA_ARGS__);           \
   } while (0)
 
 // Fill n floats/ints with zero (a kernel, not hipMemsetAsync, so the step can

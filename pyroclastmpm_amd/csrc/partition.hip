@@ -25,7 +25,7 @@ int zero_fill(void *ptr, int64_t n_words, hipStream_t s) {
   if (n_words <= 0) return PMPM_OK;
   int64_t blocks = ceil_div(n_words, 256);
   if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(zero_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (uint32_t *)ptr, n_words);
+  PMPM_LAUNCH(zero_kernel, (::pmpm::kname("zero_kernel", {})), dim3((unsigned)blocks), dim3(256), 0, s, (uint32_t *)ptr, n_words);
   PMPM_LAUNCH_CHECK();
   return PMPM_OK;
 }
@@ -45,7 +45,7 @@ int zero_fill_lists(const ZeroLists &z, hipStream_t s) {
   if (most <= 0) return PMPM_OK;
   int64_t blocks = ceil_div(most, 256);
   if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(zero_lists_kernel, dim3((unsigned)blocks), dim3(256), 0, s, z);
+  PMPM_LAUNCH(zero_lists_kernel, (::pmpm::kname("zero_lists_kernel", {})), dim3((unsigned)blocks), dim3(256), 0, s, z);
   PMPM_LAUNCH_CHECK();
   return PMPM_OK;
 }
@@ -347,9 +347,9 @@ int pmpm::partition_clearing(pmpm_context *ctx, int dim, const pmpm_grid *grid, 
   nb = nb_clear > nb ? (nb_clear < 16384 ? nb_clear : (nb > 16384 ? nb : 16384)) : nb;
   const unsigned blocks = (unsigned)nb;
   switch (dim) {
-    case 1: hipLaunchKernelGGL(hash_kernel<1>, dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, part->cell_start, part->cell_end, extra); break;
-    case 2: hipLaunchKernelGGL(hash_kernel<2>, dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, part->cell_start, part->cell_end, extra); break;
-    default: hipLaunchKernelGGL(hash_kernel<3>, dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, part->cell_start, part->cell_end, extra); break;
+    case 1: PMPM_LAUNCH(hash_kernel<1>, (::pmpm::kname("hash_kernel", {(long)(1)})), dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, part->cell_start, part->cell_end, extra); break;
+    case 2: PMPM_LAUNCH(hash_kernel<2>, (::pmpm::kname("hash_kernel", {(long)(2)})), dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, part->cell_start, part->cell_end, extra); break;
+    default: PMPM_LAUNCH(hash_kernel<3>, (::pmpm::kname("hash_kernel", {(long)(3)})), dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, part->cell_start, part->cell_end, extra); break;
   }
   PMPM_LAUNCH_CHECK();
   // keys are in [0, ncells] (ncells = out-of-grid bucket)
@@ -370,26 +370,26 @@ int pmpm::partition_clearing(pmpm_context *ctx, int dim, const pmpm_grid *grid, 
     const int radix = 1 << rb;
     uint32_t *totals = ctx->hist + (size_t)radix * nblocks;
     if (rb == 8) {
-      hipLaunchKernelGGL(radix_hist_kernel<8>, dim3(nblocks), dim3(kSortBlock), 0, s, kin, n, shift, ctx->hist, nblocks);
+      PMPM_LAUNCH(radix_hist_kernel<8>, (::pmpm::kname("radix_hist_kernel", {(long)(8)})), dim3(nblocks), dim3(kSortBlock), 0, s, kin, n, shift, ctx->hist, nblocks);
       PMPM_LAUNCH_CHECK();
-      hipLaunchKernelGGL(scan_columns_kernel, dim3(radix), dim3(256), 0, s, ctx->hist, nblocks, totals);
+      PMPM_LAUNCH(scan_columns_kernel, (::pmpm::kname("scan_columns_kernel", {})), dim3(radix), dim3(256), 0, s, ctx->hist, nblocks, totals);
       PMPM_LAUNCH_CHECK();
-      hipLaunchKernelGGL(radix_scatter_kernel<8>, dim3(nblocks), dim3(kSortBlock), 0, s, kin, vin, kout, vout, n,
+      PMPM_LAUNCH(radix_scatter_kernel<8>, (::pmpm::kname("radix_scatter_kernel", {(long)(8)})), dim3(nblocks), dim3(kSortBlock), 0, s, kin, vin, kout, vout, n,
                          shift, ctx->hist, nblocks, totals);
     } else {
-      hipLaunchKernelGGL(radix_hist_kernel<10>, dim3(nblocks), dim3(kSortBlock), 0, s, kin, n, shift, ctx->hist,
+      PMPM_LAUNCH(radix_hist_kernel<10>, (::pmpm::kname("radix_hist_kernel", {(long)(10)})), dim3(nblocks), dim3(kSortBlock), 0, s, kin, n, shift, ctx->hist,
                          nblocks);
       PMPM_LAUNCH_CHECK();
-      hipLaunchKernelGGL(scan_columns_kernel, dim3(radix), dim3(256), 0, s, ctx->hist, nblocks, totals);
+      PMPM_LAUNCH(scan_columns_kernel, (::pmpm::kname("scan_columns_kernel", {})), dim3(radix), dim3(256), 0, s, ctx->hist, nblocks, totals);
       PMPM_LAUNCH_CHECK();
-      hipLaunchKernelGGL(radix_scatter_kernel<10>, dim3(nblocks), dim3(kSortBlock), 0, s, kin, vin, kout, vout, n,
+      PMPM_LAUNCH(radix_scatter_kernel<10>, (::pmpm::kname("radix_scatter_kernel", {(long)(10)})), dim3(nblocks), dim3(kSortBlock), 0, s, kin, vin, kout, vout, n,
                          shift, ctx->hist, nblocks, totals);
     }
     PMPM_LAUNCH_CHECK();
     kin = kout;
     vin = vout;
   }
-  hipLaunchKernelGGL(cell_range_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, part->hash_sorted, n, ncells,
+  PMPM_LAUNCH(cell_range_kernel, (::pmpm::kname("cell_range_kernel", {})), dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, part->hash_sorted, n, ncells,
                      part->cell_start, part->cell_end, ctx->counters);
   PMPM_LAUNCH_CHECK();
   return PMPM_OK;

@@ -167,13 +167,13 @@ extern "C" int pmpm_rigid_levelset(const pmpm_params *prm, const pmpm_grid *grid
   const unsigned pb = (unsigned)ceil_div(p->n > 0 ? p->n : 1, 256);
   const unsigned nb = (unsigned)ceil_div(grid->num_cells_total, 256);
   if (D == 2) {
-    hipLaunchKernelGGL(rigid_velocity_kernel<2>, dim3(pb), dim3(256), 0, s, *p, mo, prm->dt);
-    hipLaunchKernelGGL(rigid_normals_kernel<2>, dim3(nb), dim3(256), 0, s, *prm, *grid, *p, *part, *nodes, dpsi_orig);
-    hipLaunchKernelGGL(rigid_position_kernel<2>, dim3(pb), dim3(256), 0, s, *p, prm->dt, rigid_dx);
+    PMPM_LAUNCH(rigid_velocity_kernel<2>, (::pmpm::kname("rigid_velocity_kernel", {(long)(2)})), dim3(pb), dim3(256), 0, s, *p, mo, prm->dt);
+    PMPM_LAUNCH(rigid_normals_kernel<2>, (::pmpm::kname("rigid_normals_kernel", {(long)(2)})), dim3(nb), dim3(256), 0, s, *prm, *grid, *p, *part, *nodes, dpsi_orig);
+    PMPM_LAUNCH(rigid_position_kernel<2>, (::pmpm::kname("rigid_position_kernel", {(long)(2)})), dim3(pb), dim3(256), 0, s, *p, prm->dt, rigid_dx);
   } else {
-    hipLaunchKernelGGL(rigid_velocity_kernel<3>, dim3(pb), dim3(256), 0, s, *p, mo, prm->dt);
-    hipLaunchKernelGGL(rigid_normals_kernel<3>, dim3(nb), dim3(256), 0, s, *prm, *grid, *p, *part, *nodes, dpsi_orig);
-    hipLaunchKernelGGL(rigid_position_kernel<3>, dim3(pb), dim3(256), 0, s, *p, prm->dt, rigid_dx);
+    PMPM_LAUNCH(rigid_velocity_kernel<3>, (::pmpm::kname("rigid_velocity_kernel", {(long)(3)})), dim3(pb), dim3(256), 0, s, *p, mo, prm->dt);
+    PMPM_LAUNCH(rigid_normals_kernel<3>, (::pmpm::kname("rigid_normals_kernel", {(long)(3)})), dim3(nb), dim3(256), 0, s, *prm, *grid, *p, *part, *nodes, dpsi_orig);
+    PMPM_LAUNCH(rigid_position_kernel<3>, (::pmpm::kname("rigid_position_kernel", {(long)(3)})), dim3(pb), dim3(256), 0, s, *p, prm->dt, rigid_dx);
   }
   PMPM_LAUNCH_CHECK();
   return PMPM_OK;
@@ -187,8 +187,8 @@ extern "C" int pmpm_permute(int32_t n, int32_t ncomp, int32_t elem_bytes, const 
   hipStream_t s = (hipStream_t)stream;
   const unsigned b = (unsigned)ceil_div(n, 256);
   switch (elem_bytes) {
-    case 1: hipLaunchKernelGGL(permute_kernel<uint8_t>, dim3(b), dim3(256), 0, s, n, ncomp, (const uint8_t *)src, id, (uint8_t *)dst, inverse); break;
-    case 4: hipLaunchKernelGGL(permute_kernel<uint32_t>, dim3(b), dim3(256), 0, s, n, ncomp, (const uint32_t *)src, id, (uint32_t *)dst, inverse); break;
+    case 1: PMPM_LAUNCH(permute_kernel<uint8_t>, (::pmpm::kname("permute_kernel", {1})), dim3(b), dim3(256), 0, s, n, ncomp, (const uint8_t *)src, id, (uint8_t *)dst, inverse); break;
+    case 4: PMPM_LAUNCH(permute_kernel<uint32_t>, (::pmpm::kname("permute_kernel", {4})), dim3(b), dim3(256), 0, s, n, ncomp, (const uint32_t *)src, id, (uint32_t *)dst, inverse); break;
     default: return PMPM_ERR_INVALID;
   }
   PMPM_LAUNCH_CHECK();
@@ -200,7 +200,7 @@ extern "C" int pmpm_rigid_advance(int32_t dim, int32_t n, float *x, const float 
   if (dim < 1 || dim > 3 || n < 0) return PMPM_ERR_INVALID;
   if (n == 0) return PMPM_OK;
   if (!x || !rigid_dx) return PMPM_ERR_INVALID;
-  hipLaunchKernelGGL(rigid_advance_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, dim, n,
+  PMPM_LAUNCH(rigid_advance_kernel, (::pmpm::kname("rigid_advance_kernel", {})), dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, dim, n,
                      x, rigid_dx, sorted);
   PMPM_LAUNCH_CHECK();
   return PMPM_OK;

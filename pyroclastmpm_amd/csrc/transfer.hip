@@ -2149,13 +2149,13 @@ void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles
   if constexpr (D == 3 && SF == PMPM_SF_CUBIC) {
     mfma = (mode == 0 || mode == 1) && prm.node_types_built;
     if (mfma && mode == 0) {
-      hipLaunchKernelGGL((p2g_mfma_kernel<SF, false>), grid, dim3(256), 0, s, prm, g, in, out, part, nodes,
+      PMPM_LAUNCH((p2g_mfma_kernel<SF, false>), (::pmpm::kname("p2g_mfma_kernel", {(long)(SF), (long)(false)})), grid, dim3(256), 0, s, prm, g, in, out, part, nodes,
                          tg.tiles[0], tg.tiles[1], partials, flags);
     } else if (mfma && mode == 1) {
       if (in.n > 0)
-        hipLaunchKernelGGL((g2p_kernel<D, SF, 1>), dim3((unsigned)ceil_div(in.n, kG2PThreads)), dim3(kG2PThreads), 0,
+        PMPM_LAUNCH((g2p_kernel<D, SF, 1>), (::pmpm::kname("g2p_kernel", {(long)(D), (long)(SF), (long)(1)})), dim3((unsigned)ceil_div(in.n, kG2PThreads)), dim3(kG2PThreads), 0,
                            s, prm, g, in, out, part, nodes, 1);
-      hipLaunchKernelGGL((p2g_mfma_kernel<SF, true>), grid, dim3(256), 0, s, prm, g, in, out, part, nodes,
+      PMPM_LAUNCH((p2g_mfma_kernel<SF, true>), (::pmpm::kname("p2g_mfma_kernel", {(long)(SF), (long)(true)})), grid, dim3(256), 0, s, prm, g, in, out, part, nodes,
                          tg.tiles[0], tg.tiles[1], partials, flags);
     }
   }
@@ -2163,23 +2163,23 @@ void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles
   // `rec` and is never launched without it)
   if (mode == 0) {
     if (!mfma)
-      hipLaunchKernelGGL((p2g_kernel<D, SF, 0>), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
+      PMPM_LAUNCH((p2g_kernel<D, SF, 0>), (::pmpm::kname("p2g_kernel", {(long)(D), (long)(SF), (long)(0)})), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
                          tg.tiles[1], partials, flags, rec);
   } else if (mode == 1) {
     if (!mfma)
-      hipLaunchKernelGGL((p2g_kernel<D, SF, 1>), dim3((unsigned)tile_count(tg) + kMuslTailBlocks), block, 0, s, prm,
+      PMPM_LAUNCH((p2g_kernel<D, SF, 1>), (::pmpm::kname("p2g_kernel", {(long)(D), (long)(SF), (long)(1)})), dim3((unsigned)tile_count(tg) + kMuslTailBlocks), block, 0, s, prm,
                          g, in, out, part, nodes, tg.tiles[0], tg.tiles[1], partials, flags, rec);
   } else if (mode == 2 && rec) {
-    hipLaunchKernelGGL((p2g_kernel<D, SF, 2>), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
+    PMPM_LAUNCH((p2g_kernel<D, SF, 2>), (::pmpm::kname("p2g_kernel", {(long)(D), (long)(SF), (long)(2)})), grid, block, 0, s, prm, g, in, out, part, nodes, tg.tiles[0],
                        tg.tiles[1], partials, flags, rec);
   }
   const int wcells = wrap_cell_count<D>(g);
   if (mode != 2 && wrap_mode(prm, g) == 2) {
     const dim3 wgrid((unsigned)ceil_div(wcells, kWrapCellsPerBlock)), wblock(64 * kWrapCellsPerBlock);
     if (mode == 0)
-      hipLaunchKernelGGL((p2g_wrap_kernel<D, 0>), wgrid, wblock, 0, s, prm, g, in, out, part, wrec, wlive, wcells);
+      PMPM_LAUNCH((p2g_wrap_kernel<D, 0>), (::pmpm::kname("p2g_wrap_kernel", {(long)(D), (long)(0)})), wgrid, wblock, 0, s, prm, g, in, out, part, wrec, wlive, wcells);
     else
-      hipLaunchKernelGGL((p2g_wrap_kernel<D, 1>), wgrid, wblock, 0, s, prm, g, in, out, part, wrec, wlive, wcells);
+      PMPM_LAUNCH((p2g_wrap_kernel<D, 1>), (::pmpm::kname("p2g_wrap_kernel", {(long)(D), (long)(1)})), wgrid, wblock, 0, s, prm, g, in, out, part, wrec, wlive, wcells);
   }
 }
 template <int D, int SF>
@@ -2187,13 +2187,13 @@ void launch_g2p(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles
                 const pmpm_spatial &part, const pmpm_nodes &nodes, int mode, int reorder, hipStream_t s) {
   const unsigned blocks = (unsigned)ceil_div(in.n, 256);
   if (mode == 0)
-    hipLaunchKernelGGL((g2p_kernel<D, SF, 0>), dim3(blocks), dim3(256), 0, s, prm, g, in, out, part, nodes, reorder);
+    PMPM_LAUNCH((g2p_kernel<D, SF, 0>), (::pmpm::kname("g2p_kernel", {(long)(D), (long)(SF), (long)(0)})), dim3(blocks), dim3(256), 0, s, prm, g, in, out, part, nodes, reorder);
   else
-    hipLaunchKernelGGL((g2p_kernel<D, SF, 2>), dim3(blocks), dim3(256), 0, s, prm, g, in, out, part, nodes, reorder);
+    PMPM_LAUNCH((g2p_kernel<D, SF, 2>), (::pmpm::kname("g2p_kernel", {(long)(D), (long)(SF), (long)(2)})), dim3(blocks), dim3(256), 0, s, prm, g, in, out, part, nodes, reorder);
 }
 template <int D, int SF>
 void launch_shape(const pmpm_grid &g, int types_built, const float *x, int n, float *psi, float *dpsi, hipStream_t s) {
-  hipLaunchKernelGGL((shape_function_kernel<D, SF>), dim3((unsigned)ceil_div(n, 128)), dim3(128), 0, s, g, types_built,
+  PMPM_LAUNCH((shape_function_kernel<D, SF>), (::pmpm::kname("shape_function_kernel", {(long)(D), (long)(SF)})), dim3((unsigned)ceil_div(n, 128)), dim3(128), 0, s, g, types_built,
                      x, n, psi, dpsi);
 }
 
@@ -2214,9 +2214,9 @@ int launch_grid(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, c
     }
   }
   switch (prm.dim) {
-    case 1: hipLaunchKernelGGL(grid_kernel<1>, dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt); break;
-    case 2: hipLaunchKernelGGL(grid_kernel<2>, dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt); break;
-    case 3: hipLaunchKernelGGL(grid_kernel<3>, dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt); break;
+    case 1: PMPM_LAUNCH(grid_kernel<1>, (::pmpm::kname("grid_kernel", {(long)(1)}) + " gather " + std::to_string(gather)), dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt); break;
+    case 2: PMPM_LAUNCH(grid_kernel<2>, (::pmpm::kname("grid_kernel", {(long)(2)}) + " gather " + std::to_string(gather)), dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt); break;
+    case 3: PMPM_LAUNCH(grid_kernel<3>, (::pmpm::kname("grid_kernel", {(long)(3)}) + " gather " + std::to_string(gather)), dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt); break;
     default: return PMPM_ERR_INVALID;
   }
   PMPM_LAUNCH_CHECK();
@@ -2363,9 +2363,9 @@ extern "C" int pmpm_stress_update(const pmpm_params *prm, int32_t mat_id, pmpm_p
   hipStream_t s = (hipStream_t)stream;
   const unsigned blocks = (unsigned)ceil_div(p->n, 256);
   switch (prm->dim) {
-    case 1: hipLaunchKernelGGL(stress_update_kernel<1>, dim3(blocks), dim3(256), 0, s, *prm, mat_id, *p); break;
-    case 2: hipLaunchKernelGGL(stress_update_kernel<2>, dim3(blocks), dim3(256), 0, s, *prm, mat_id, *p); break;
-    default: hipLaunchKernelGGL(stress_update_kernel<3>, dim3(blocks), dim3(256), 0, s, *prm, mat_id, *p); break;
+    case 1: PMPM_LAUNCH(stress_update_kernel<1>, (::pmpm::kname("stress_update_kernel", {(long)(1)})), dim3(blocks), dim3(256), 0, s, *prm, mat_id, *p); break;
+    case 2: PMPM_LAUNCH(stress_update_kernel<2>, (::pmpm::kname("stress_update_kernel", {(long)(2)})), dim3(blocks), dim3(256), 0, s, *prm, mat_id, *p); break;
+    default: PMPM_LAUNCH(stress_update_kernel<3>, (::pmpm::kname("stress_update_kernel", {(long)(3)})), dim3(blocks), dim3(256), 0, s, *prm, mat_id, *p); break;
   }
   PMPM_LAUNCH_CHECK();
   return PMPM_OK;
@@ -2977,19 +2977,19 @@ void launch_adjoint_particles(const pmpm_params &prm, const pmpm_grid &g, const 
                               hipStream_t s) {
   const unsigned blocks = (unsigned)ceil_div(st.n, 256);
   if (which == 0)
-    hipLaunchKernelGGL((g2p_adjoint_kernel<D, SF>), dim3(blocks), dim3(256), 0, s, prm, g, st, ob, ib, part, nodes,
+    PMPM_LAUNCH((g2p_adjoint_kernel<D, SF>), (::pmpm::kname("g2p_adjoint_kernel", {(long)(D), (long)(SF)})), dim3(blocks), dim3(256), 0, s, prm, g, st, ob, ib, part, nodes,
                        rec, pbar);
   else {
     const int lanes = adjoint_p2g_lanes(st.n, D);
     const dim3 grid((unsigned)ceil_div(st.n, 256 / lanes));
     if (lanes == 1)
-      hipLaunchKernelGGL((p2g_adjoint_kernel<D, SF, 1>), grid, dim3(256), 0, s, prm, g, st, ib, part, nodes, nbar,
+      PMPM_LAUNCH((p2g_adjoint_kernel<D, SF, 1>), (::pmpm::kname("p2g_adjoint_kernel", {(long)(D), (long)(SF), (long)(1)})), grid, dim3(256), 0, s, prm, g, st, ib, part, nodes, nbar,
                          rec);
     else if (lanes == 4 || D != 3)
-      hipLaunchKernelGGL((p2g_adjoint_kernel<D, SF, 4>), grid, dim3(256), 0, s, prm, g, st, ib, part, nodes, nbar,
+      PMPM_LAUNCH((p2g_adjoint_kernel<D, SF, 4>), (::pmpm::kname("p2g_adjoint_kernel", {(long)(D), (long)(SF), (long)(4)})), grid, dim3(256), 0, s, prm, g, st, ib, part, nodes, nbar,
                          rec);
     else
-      hipLaunchKernelGGL((p2g_adjoint_kernel<D, SF, D == 3 ? 16 : 4>), grid, dim3(256), 0, s, prm, g, st, ib, part,
+      PMPM_LAUNCH((p2g_adjoint_kernel<D, SF, D == 3 ? 16 : 4>), (::pmpm::kname("p2g_adjoint_kernel", {(long)(D), (long)(SF), (long)(D == 3 ? 16 : 4)})), grid, dim3(256), 0, s, prm, g, st, ib, part,
                          nodes, nbar, rec);
   }
 }
@@ -3041,7 +3041,7 @@ extern "C" int pmpm_adjoint_step(pmpm_context *ctx, const pmpm_params *prm, cons
                     ctx->adj_rec, ctx->adj_pb, 0, nullptr, s);
   PMPM_LAUNCH_CHECK();
   if (entries > 0) {
-    hipLaunchKernelGGL(param_bar_reduce_kernel, dim3((unsigned)entries), dim3(256), 0, s, ctx->adj_pb, groups,
+    PMPM_LAUNCH(param_bar_reduce_kernel, (::pmpm::kname("param_bar_reduce_kernel", {})), dim3((unsigned)entries), dim3(256), 0, s, ctx->adj_pb, groups,
                        entries, param_bar);
     PMPM_LAUNCH_CHECK();
   }
@@ -3052,9 +3052,9 @@ extern "C" int pmpm_adjoint_step(pmpm_context *ctx, const pmpm_params *prm, cons
   const TileGeom tg = tile_geom(D, prm->shape_fn, g, 3);
   const unsigned nblocks = (unsigned)ceil_div(g.num_cells_total, 256);
   switch (D) {
-    case 1: hipLaunchKernelGGL(grid_adjoint_kernel<1>, dim3(nblocks), dim3(256), 0, s, *prm, g, *nodes, tg, ctx->partials, ctx->tile_flags, ctx->adj_nodes); break;
-    case 2: hipLaunchKernelGGL(grid_adjoint_kernel<2>, dim3(nblocks), dim3(256), 0, s, *prm, g, *nodes, tg, ctx->partials, ctx->tile_flags, ctx->adj_nodes); break;
-    default: hipLaunchKernelGGL(grid_adjoint_kernel<3>, dim3(nblocks), dim3(256), 0, s, *prm, g, *nodes, tg, ctx->partials, ctx->tile_flags, ctx->adj_nodes); break;
+    case 1: PMPM_LAUNCH(grid_adjoint_kernel<1>, (::pmpm::kname("grid_adjoint_kernel", {(long)(1)})), dim3(nblocks), dim3(256), 0, s, *prm, g, *nodes, tg, ctx->partials, ctx->tile_flags, ctx->adj_nodes); break;
+    case 2: PMPM_LAUNCH(grid_adjoint_kernel<2>, (::pmpm::kname("grid_adjoint_kernel", {(long)(2)})), dim3(nblocks), dim3(256), 0, s, *prm, g, *nodes, tg, ctx->partials, ctx->tile_flags, ctx->adj_nodes); break;
+    default: PMPM_LAUNCH(grid_adjoint_kernel<3>, (::pmpm::kname("grid_adjoint_kernel", {(long)(3)})), dim3(nblocks), dim3(256), 0, s, *prm, g, *nodes, tg, ctx->partials, ctx->tile_flags, ctx->adj_nodes); break;
   }
   PMPM_LAUNCH_CHECK();
   // P2G adjoint
