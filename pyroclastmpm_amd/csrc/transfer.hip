@@ -1694,7 +1694,7 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
       // gravity_inline.h:46-63
       if (remap_pass || m <= 0.000000001f) continue;
 #pragma unroll
-      for (int a = 0; a < D; ++a) Q[a] += bc.f[a] * m * prm.dt;
+      for (int a = 0; a < D; ++a) Q[a] = __fadd_rn(Q[a], __fmul_rn(__fmul_rn(bc.f[a], m), prm.dt));
     } else if (bc.kind == PMPM_BC_NODE_DOMAIN) {
       // nodedomain_inline.h:40-87: 0 fixed, 1 slip (one-sided clamp)
       if (m <= 0.00001f) continue;
@@ -2686,8 +2686,15 @@ __global__ void grid_adjoint_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nod
         }
       } else if (bc.kind == PMPM_BC_GRAVITY) {
         if (m <= 0.000000001f) continue;
+        // Q += g m dt: its adjoint, then its undo, so that a slip clamp
+        // earlier in the list is tested on the value it saw. The increment is
+        // rounded as grid_kernel rounds it, so a clamped component that
+        // gravity moved off zero returns to exactly zero.
 #pragma unroll
-        for (int a = 0; a < D; ++a) mb += bc.f[a] * prm.dt * Qb[a];
+        for (int a = 0; a < D; ++a) {
+          mb += bc.f[a] * prm.dt * Qb[a];
+          Q[a] = __fsub_rn(Q[a], __fmul_rn(__fmul_rn(bc.f[a], m), prm.dt));
+        }
       }
     }
     // Q = P + dt f

@@ -112,8 +112,12 @@ class DiffMPM:
     particles from `positions` at rest."""
 
     def __init__(self, dim, origin, end, cell_size, positions, ppc, *, density=1500.0, pois=0.3, dt=1e-4,
-                 alpha=0.99, shape_function="cubic", gravity=None, checkpoint_every=20, device="cuda"):
+                 alpha=0.99, shape_function="cubic", gravity=None, checkpoint_every=20, device="cuda",
+                 bc_order=("gravity", "walls")):
         self.dim, self.dt, self.alpha, self.pois = dim, dt, alpha, pois
+        if sorted(bc_order) != ["gravity", "walls"]:
+            raise ValueError("bc_order must order 'gravity' and 'walls'")
+        self.bc_order = tuple(bc_order)  # the node conditions' list order (usl.cpp:96-103)
         self.sf = capi.SF[shape_function]
         self.K = int(checkpoint_every)
         self.grid = capi.make_grid(origin, end, cell_size, dim)
@@ -151,13 +155,14 @@ class DiffMPM:
         for i, v in enumerate([K, G, M, Pt, etab]):
             m.p[i] = v
         prm.num_bcs = 2
-        prm.bcs[0].kind = capi.BC_GRAVITY
+        g, w = prm.bcs[self.bc_order.index("gravity")], prm.bcs[self.bc_order.index("walls")]
+        g.kind = capi.BC_GRAVITY
         for a in range(self.dim):
-            prm.bcs[0].f[a] = self.gravity[a]
-        prm.bcs[1].kind = capi.BC_NODE_DOMAIN
+            g.f[a] = self.gravity[a]
+        w.kind = capi.BC_NODE_DOMAIN
         for a in range(self.dim):
-            prm.bcs[1].i[a] = 1
-            prm.bcs[1].i[3 + a] = 1
+            w.i[a] = 1
+            w.i[3 + a] = 1
         return prm
 
     # -------------------------------------------------------------- stepping
