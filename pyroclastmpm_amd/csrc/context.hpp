@@ -26,11 +26,16 @@ struct pmpm_context {
   int32_t *tile_flags = nullptr;  // 1 = tile had particles this launch
   size_t tile_flags_cap = 0;
   // wrap-around sums of typed cubic splines: per (wrap cell, lane) records
-  // ([fields][cells][64] floats) and a live flag per wrap cell
+  // ([fields][cells][64] floats)
   float *wrap_rec = nullptr;
   size_t wrap_rec_cap = 0;
-  float *wrap_live = nullptr;  // int32 flags (held in a float-sized buffer)
-  size_t wrap_live_cap = 0;
+  // the non-empty wrap cells the last pmpm_partition found (counters[1] of
+  // them, any order), for the partition whose cell_start array and grid
+  // these are
+  int32_t *wrap_list = nullptr;
+  size_t wrap_list_cap = 0;
+  const int32_t *wrap_list_cs = nullptr;
+  pmpm_grid wrap_list_grid{};
   // adjoint step: per-particle scatter records (15 floats, cell order) and
   // node adjoints (one record of 8 floats per node, 4 in 1D)
   float *adj_rec = nullptr;
@@ -48,6 +53,7 @@ namespace pmpm {
 int ensure_partials(pmpm_context *ctx, size_t floats, size_t tiles);
 // Grows a context-owned float buffer to at least `floats`.
 int ensure_buffer(float *&buf, size_t &cap, size_t floats);
+int ensure_ints(int32_t *&buf, size_t &cap, size_t n);
 }  // namespace pmpm
 
 namespace pmpm {

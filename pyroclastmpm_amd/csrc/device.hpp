@@ -478,4 +478,79 @@ template <int D> __device__ inline float det_dd(const float *F) {
 
 __device__ inline int64_t ceil_div64(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// ---- the linear-index wrap-around (usl_inline.h:122-126: P2G bound-checks
+// only the linear node index, so a node vector outside the grid on x, or on y
+// in 3D, can land on an in-grid node).
+//
+// Cells whose window (node offsets -1 .. 2) can wrap into the grid: x in
+// {0, nx-2, nx-1} (any y, z) and, in 3D, y in {0, ny-2, ny-1} (other x).
+// Offsets outside the grid along the slowest axis always leave the index
+// range, so no other cell wraps. wrap_cell maps w in [0, wrap_cell_count) to
+// its cell (false for the duplicates of tiny grids), wrap_index inverts it.
+template <int D> __host__ __device__ inline bool wrap_cell(const pmpm_grid &g, int w, int *c) {
+  const int nx = g.num_cells[0], ny = D > 1 ? g.num_cells[1] : 1, nz = D > 2 ? g.num_cells[2] : 1;
+  if (D < 2) return false;
+  auto pick = [](int i, int len, int &v) {  // i-th of {0, len-2, len-1}, each cell once
+    v = i == 0 ? 0 : len - 3 + i;
+    return i == 0 || v > 0;
+  };
+  const int nxs = 3 * ny * (D > 2 ? nz : 1);
+  if (w < nxs) {
+    c[1] = (w / 3) % ny;
+    c[2] = D > 2 ? w / (3 * ny) : 0;
+    return pick(w % 3, nx, c[0]);
+  }
+  if (D < 3) return false;
+  w -= nxs;
+  const int mid = nx - 3;  // x in [1, nx-3]
+  if (mid <= 0 || w >= 3 * mid * nz) return false;
+  c[0] = 1 + (w / 3) % mid;
+  c[2] = w / (3 * mid);
+  return pick(w % 3, ny, c[1]);
+}
+template <int D> __host__ __device__ inline int wrap_cell_count(const pmpm_grid &g) {
+  const int nx = g.num_cells[0], ny = D > 1 ? g.num_cells[1] : 1, nz = D > 2 ? g.num_cells[2] : 1;
+  if (D < 2) return 0;
+  return 3 * ny * (D > 2 ? nz : 1) + (D > 2 && nx > 3 ? 3 * (nx - 3) * nz : 0);
+}
+template <int D> __host__ __device__ inline int wrap_index(const pmpm_grid &g, const int *c) {
+  if (D < 2) return -1;
+  const int nx = g.num_cells[0], ny = g.num_cells[1], nz = D > 2 ? g.num_cells[2] : 1;
+  auto slot = [](int v, int len) { return v == 0 ? 0 : (v >= len - 2 ? v - (len - 3) : -1); };
+  const int i = slot(c[0], nx);
+  if (i >= 0) return i + 3 * (c[1] + ny * (D > 2 ? c[2] : 0));
+  if (D < 3) return -1;
+  const int j = slot(c[1], ny);
+  if (j < 0) return -1;
+  return 3 * ny * nz + j + 3 * ((c[0] - 1) + (nx - 3) * c[2]);
+}
+// Nodes the wrap-around can reach: x in {0, 1, nx-1} (any y, z) and, in 3D,
+// y in {0, 1, ny-1} (x in [2, nx-2]). wrap_target maps t in
+// [0, wrap_target_count) to its node coordinates (false for the duplicates
+// of tiny grids).
+template <int D> __host__ __device__ inline bool wrap_target(const pmpm_grid &g, int t, int *ids) {
+  const int nx = g.num_cells[0], ny = D > 1 ? g.num_cells[1] : 1, nz = D > 2 ? g.num_cells[2] : 1;
+  if (D < 2) return false;
+  auto pick = [](int i, int len, int &v) {  // i-th of {0, 1, len-1}, each node once
+    v = i == 0 ? 0 : (i == 1 ? 1 : len - 1);
+    return i == 0 || (i == 1 ? len > 1 : len - 1 > 1);
+  };
+  const int nxs = 3 * ny * (D > 2 ? nz : 1);
+  if (t < nxs) {
+    ids[1] = (t / 3) % ny;
+    ids[2] = D > 2 ? t / (3 * ny) : 0;
+    return pick(t % 3, nx, ids[0]);
+  }
+  if (D < 3) return false;
+  t -= nxs;
+  const int mid = nx - 3;  // x in [2, nx-2]
+  if (mid <= 0 || t >= 3 * mid * nz) return false;
+  ids[0] = 2 + (t / 3) % mid;
+  ids[2] = t / (3 * mid);
+  return pick(t % 3, ny, ids[1]);
+}
+template <int D> __host__ __device__ inline int wrap_target_count(const pmpm_grid &g) {
+  return wrap_cell_count<D>(g);  // the same slab sizes
+}
+
 }  // namespace pmpm

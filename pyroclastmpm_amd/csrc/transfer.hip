@@ -18,6 +18,7 @@
 // particle x node is stored.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 
 #include "adjoint_math.hpp"
@@ -1310,87 +1311,14 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
   }
 }
 
-// Cells whose cubic window can wrap into the grid through the linear index:
-// x in {0, nx-2, nx-1} (any y, z) and, in 3D, y in {0, ny-2, ny-1} (other
-// x). Offsets outside the grid along the slowest axis always leave the index
-// range, so no other cell wraps. Returns false for w past the set.
-template <int D> __device__ inline bool wrap_cell(const pmpm_grid &g, int w, int *c) {
-  const int nx = g.num_cells[0], ny = D > 1 ? g.num_cells[1] : 1, nz = D > 2 ? g.num_cells[2] : 1;
-  if (D < 2) return false;
-  auto pick = [](int i, int len, int &v) {  // i-th of {0, len-2, len-1}, each cell once
-    v = i == 0 ? 0 : len - 3 + i;
-    return i == 0 || v > 0;
-  };
-  const int nxs = 3 * ny * (D > 2 ? nz : 1);
-  if (w < nxs) {
-    c[1] = (w / 3) % ny;
-    c[2] = D > 2 ? w / (3 * ny) : 0;
-    return pick(w % 3, nx, c[0]);
-  }
-  if (D < 3) return false;
-  w -= nxs;
-  const int mid = nx - 3;  // x in [1, nx-3]
-  if (mid <= 0 || w >= 3 * mid * nz) return false;
-  c[0] = 1 + (w / 3) % mid;
-  c[2] = w / (3 * mid);
-  return pick(w % 3, ny, c[1]);
-}
-template <int D> __host__ inline int wrap_cell_count(const pmpm_grid &g) {
-  const int nx = g.num_cells[0], ny = D > 1 ? g.num_cells[1] : 1, nz = D > 2 ? g.num_cells[2] : 1;
-  if (D < 2) return 0;
-  return 3 * ny * (D > 2 ? nz : 1) + (D > 2 && nx > 3 ? 3 * (nx - 3) * nz : 0);
-}
-
-// Index of wrap cell c in the order of wrap_cell (its inverse), or -1 when c
-// is not one of the cells that can wrap.
-template <int D> __device__ inline int wrap_index(const pmpm_grid &g, const int *c) {
-  if (D < 2) return -1;
-  const int nx = g.num_cells[0], ny = g.num_cells[1], nz = D > 2 ? g.num_cells[2] : 1;
-  auto slot = [](int v, int len) { return v == 0 ? 0 : (v >= len - 2 ? v - (len - 3) : -1); };
-  const int i = slot(c[0], nx);
-  if (i >= 0) return i + 3 * (c[1] + ny * (D > 2 ? c[2] : 0));
-  if (D < 3) return -1;
-  const int j = slot(c[1], ny);
-  if (j < 0) return -1;
-  return 3 * ny * nz + j + 3 * ((c[0] - 1) + (nx - 3) * c[2]);
-}
-
-// Wrap-around sums of typed cubic splines: one wave per cell that can wrap
-// (four cells per workgroup), lane = node offset (sx, sy, sz) of the cell's
-// span. A lane whose node lies outside the grid but whose linear index is in
-// range re-sums the cell's particles with the types of the node that index
-// reaches (shapefunction.cpp:96-112). The sums are stored per (cell, lane) in
-// `rec` ([NV][wcells][64], zero for the other lanes of a live cell) and
-// `live[w]` says whether cell w wrote any; the grid kernel adds them to their
-// nodes in a fixed order, so the node sums do not depend on scheduling.
-// The cell's particles are first loaded into LDS by the wave (lane =
-// particle, up to 64 per round), so the per-node loops read LDS instead of
-// walking a chain of dependent global loads.
-constexpr int kWrapCellsPerBlock = 4;
-template <int D, int MODE>
-__global__ __launch_bounds__(64 * kWrapCellsPerBlock) void p2g_wrap_kernel(pmpm_params prm, pmpm_grid g,
-                                                                           pmpm_particles in, pmpm_particles out,
-                                                                           pmpm_spatial part, float *__restrict__ rec,
-                                                                           int32_t *__restrict__ live, int wcells) {
+// The wrap-around sums of one non-empty wrap cell c (index w), one lane per
+// node offset; writes all 64 records of the cell.
+template <int D, int MODE, int NV, int NS>
+__device__ inline void wrap_cell_sums(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in,
+                                      const pmpm_particles &out, const pmpm_spatial &part, float *rec, int wcells,
+                                      int w, const int *c, int cs, int ce, int lane, float (*s_frac)[64],
+                                      float (*s_v)[64], float (*s_A)[64], float *s_m, int *s_own) {
   constexpr int SPAN = 4, NLO = -1;
-  constexpr int NV = MODE == 0 ? 1 + 2 * D : D;
-  constexpr int NS = MODE == 0 ? 6 : 1;  // staged stress terms (-V sigma)
-  constexpr int WPB = kWrapCellsPerBlock;
-  __shared__ float s_frac_all[WPB][3][64], s_v_all[WPB][3][64], s_m_all[WPB][64], s_A_all[WPB][NS][64];
-  __shared__ int s_own_all[WPB][64];
-  // every wave works on its own cell; only wave-level synchronisation below
-  const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
-  const int w = (int)blockIdx.x * WPB + wv;
-  float(*s_frac)[64] = s_frac_all[wv];
-  float(*s_v)[64] = s_v_all[wv];
-  float(*s_A)[64] = s_A_all[wv];
-  float *s_m = s_m_all[wv];
-  int *s_own = s_own_all[wv];
-  int c[3] = {0, 0, 0};
-  if (w >= wcells || !wrap_cell<D>(g, w, c)) return;  // uniform over the wave
-  const int cell = (int)lin_index(g, c, D);
-  const int ce = part.cell_end[cell];
-  const int cs = part.cell_start[cell];
   const int s[3] = {lane % SPAN, D > 1 ? (lane / SPAN) % SPAN : 0, D > 2 ? lane / (SPAN * SPAN) : 0};
   const int nlanes = D == 3 ? 64 : (D == 2 ? 16 : 4);
   int nb[3] = {0, 0, 0};
@@ -1402,10 +1330,7 @@ __global__ __launch_bounds__(64 * kWrapCellsPerBlock) void p2g_wrap_kernel(pmpm_
   }
   const int nn = g.num_cells_total;
   const int64_t lin = lin_index(g, nb, D);
-  const bool active = ce > cs && lane < nlanes && !inside && lin >= 0 && lin < nn;
-  const bool any = __ballot(active) != 0;  // uniform over the wave
-  if (lane == 0) live[w] = any ? 1 : 0;
-  if (!any) return;
+  const bool active = lane < nlanes && !inside && lin >= 0 && lin < nn;
   // r = frac - (NLO + s) lies in piece 3 - s of the wrapped node's spline
   // (the same piece argument as span_eval): its coefficients are fixed per
   // lane
@@ -1489,6 +1414,55 @@ __global__ __launch_bounds__(64 * kWrapCellsPerBlock) void p2g_wrap_kernel(pmpm_
   for (int k = 0; k < NV; ++k) rec[((int64_t)k * wcells + w) * 64 + lane] = active ? acc[k] : 0.f;
 }
 
+// Wrap-around sums of typed cubic splines: one wave per non-empty cell that
+// can wrap, lane = node offset (sx, sy, sz) of the cell's span. A lane whose
+// node lies outside the grid but whose linear index is in range re-sums the
+// cell's particles with the types of the node that index reaches
+// (shapefunction.cpp:96-112). The sums are stored per (cell, lane) in `rec`
+// ([NV][wcells][64], zero for the other lanes); wrap_gather_kernel adds them
+// to their nodes in a fixed order, so the node sums do not depend on
+// scheduling. The waves walk `list` (the non-empty wrap cells the partition
+// found, *count of them) with a stride of the launch's wave count, or every
+// wrap cell when list is null. The cell's particles are first loaded into LDS
+// by the wave (lane = particle, up to 64 per round), so the per-node loops
+// read LDS instead of walking a chain of dependent global loads.
+constexpr int kWrapCellsPerBlock = 4;
+constexpr int kWrapListBlocks = 512;  // 2048 waves walk the list
+template <int D, int MODE>
+__global__ __launch_bounds__(64 * kWrapCellsPerBlock) void p2g_wrap_kernel(pmpm_params prm, pmpm_grid g,
+                                                                           pmpm_particles in, pmpm_particles out,
+                                                                           pmpm_spatial part, float *__restrict__ rec,
+                                                                           const int32_t *__restrict__ list,
+                                                                           const uint32_t *__restrict__ count,
+                                                                           int wcells) {
+  constexpr int SPAN = 4, NLO = -1;
+  constexpr int NV = MODE == 0 ? 1 + 2 * D : D;
+  constexpr int NS = MODE == 0 ? 6 : 1;  // staged stress terms (-V sigma)
+  constexpr int WPB = kWrapCellsPerBlock;
+  __shared__ float s_frac_all[WPB][3][64], s_v_all[WPB][3][64], s_m_all[WPB][64], s_A_all[WPB][NS][64];
+  __shared__ int s_own_all[WPB][64];
+  // every wave works on its own cells; only wave-level synchronisation below
+  const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  float(*s_frac)[64] = s_frac_all[wv];
+  float(*s_v)[64] = s_v_all[wv];
+  float(*s_A)[64] = s_A_all[wv];
+  float *s_m = s_m_all[wv];
+  int *s_own = s_own_all[wv];
+  const int items = list ? (int)*count : wcells;
+  const int stride = (int)gridDim.x * WPB;
+  for (int item = (int)blockIdx.x * WPB + wv; item < items; item += stride) {  // uniform over the wave
+    const int w = list ? list[item] : item;
+    int c[3] = {0, 0, 0};
+    if (w < 0 || w >= wcells || !wrap_cell<D>(g, w, c)) continue;
+    const int cell = (int)lin_index(g, c, D);
+    const int ce = part.cell_end[cell];
+    const int cs = part.cell_start[cell];
+    if (ce <= cs) continue;
+    wrap_cell_sums<D, MODE, NV, NS>(prm, g, in, out, part, rec, wcells, w, c, cs, ce, lane, s_frac, s_v, s_A, s_m,
+                                    s_own);
+  }
+}
+
 // Geometry of the P2G tiles for the grid kernel's gather.
 struct TileGeom {
   int tiles[3];  // tiles per axis
@@ -1504,7 +1478,7 @@ struct TileGeom {
   int wrap;
   int wcells;
   const float *wrec;
-  const int32_t *wlive;
+  const int32_t *cstart, *cend;  // cell ranges: a non-empty wrap cell wrote its records
 };
 
 // Sums the NV fields of the partial slices of the tiles whose node block
@@ -1607,7 +1581,8 @@ __device__ inline void gather_wrapped(const pmpm_grid &g, const TileGeom &tg, co
             const int c[3] = {nb[0] + 1 - sx, nb[1] + 1 - sy, D > 2 ? nb[2] + 1 - sz : 0};
             if (c[0] < 0 || c[0] >= nx || c[1] < 0 || c[1] >= ny || (D > 2 && (c[2] < 0 || c[2] >= nz))) continue;
             const int w = wrap_index<D>(g, c);
-            if (w < 0 || !tg.wlive[w]) continue;
+            const int cl = c[0] + nx * (c[1] + ny * c[2]);
+            if (w < 0 || tg.cend[cl] <= tg.cstart[cl]) continue;
             const int lane = sx + 4 * (sy + 4 * sz);
 #pragma unroll
             for (int k = 0; k < NV; ++k) acc[k] += tg.wrec[((int64_t)k * tg.wcells + w) * 64 + lane];
@@ -1615,14 +1590,33 @@ __device__ inline void gather_wrapped(const pmpm_grid &g, const TileGeom &tg, co
     }
   }
 }
-// Nodes the wrap-around can reach: x in {0, 1, nx - 1} or, in 3D, y in
-// {0, 1, ny - 1} (a node vector outside the grid moves by -+nx along x or
-// -+ny along y when its linear index is read back).
-template <int D> __device__ inline bool wrap_target(const pmpm_grid &g, const int *ids) {
-  if (D < 2) return false;
-  const bool xt = ids[0] <= 1 || ids[0] == g.num_cells[0] - 1;
-  if (D == 2) return xt;
-  return xt || ids[1] <= 1 || ids[1] == g.num_cells[1] - 1;
+// One thread per node the wrap-around can reach (wrap_target): adds its
+// wrap-around sums to the node arrays, which hold nothing else but the
+// (rare) stray deposits at this point; the grid kernel adds the tile partials
+// next. Fixed order, no atomics.
+template <int D, int NV>
+__global__ __launch_bounds__(256) void wrap_gather_kernel(pmpm_grid g, TileGeom tg, const float *__restrict__ partials,
+                                                          const int32_t *__restrict__ flags, pmpm_nodes nodes) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int ids[3] = {0, 0, 0};
+  if (t >= wrap_target_count<D>(g) || !wrap_target<D>(g, t, ids)) return;
+  const int nn = g.num_cells_total;
+  const int i = ids[0] + g.num_cells[0] * (ids[1] + (D > 2 ? g.num_cells[1] * ids[2] : 0));
+  float acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.f;
+  gather_wrapped<D, NV>(g, tg, partials, flags, i, acc);
+  if constexpr (NV == 1 + 2 * D) {
+    nodes.mass[i] += acc[0];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      nodes.mom[a * nn + i] += acc[1 + a];
+      nodes.mom_nt[a * nn + i] += acc[1 + D + a];
+    }
+  } else {
+#pragma unroll
+    for (int a = 0; a < D; ++a) nodes.mom_nt[a * nn + i] += acc[a];
+  }
 }
 
 // Per node: cutoff + forward moment (finalize), boundary conditions
@@ -1648,10 +1642,6 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
     float acc[1 + 2 * 3] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (tg.gather == 1) gather_partials<D, 1 + 2 * D>(tg, partials, flags, ids0, acc);
     else gather_partials<D, D>(tg, partials, flags, ids0, acc);
-    if (tg.wrap && wrap_target<D>(g, ids0)) {
-      if (tg.gather == 1) gather_wrapped<D, 1 + 2 * D>(g, tg, partials, flags, i, acc);
-      else gather_wrapped<D, D>(g, tg, partials, flags, i, acc);
-    }
     if (tg.gather == 1) {
       m += acc[0];
 #pragma unroll
@@ -2140,7 +2130,8 @@ static size_t tile_count(const TileGeom &tg) { return (size_t)tg.tiles[0] * tg.t
 template <int D, int SF>
 void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
                 const pmpm_spatial &part, const pmpm_nodes &nodes, int mode, float *partials, int32_t *flags,
-                float *wrec, int32_t *wlive, hipStream_t s, const float *rec = nullptr) {
+                float *wrec, const int32_t *wlist, const uint32_t *wcount, hipStream_t s,
+                const float *rec = nullptr) {
   const TileGeom tg = tile_geom_of<D, SF>(g, 1);
   const dim3 grid((unsigned)tile_count(tg)), block(P2GGeom<D, SF>::THREADS);
   // 3D typed cubic P2G and MUSL re-map on the matrix cores (the re-map after
@@ -2175,11 +2166,17 @@ void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles
   }
   const int wcells = wrap_cell_count<D>(g);
   if (mode != 2 && wrap_mode(prm, g) == 2) {
-    const dim3 wgrid((unsigned)ceil_div(wcells, kWrapCellsPerBlock)), wblock(64 * kWrapCellsPerBlock);
+    // the partition's list of non-empty wrap cells when it is this
+    // partition's; otherwise one wave per wrap cell
+    const unsigned blocks = wlist ? (unsigned)std::min<int64_t>(kWrapListBlocks, ceil_div(wcells, kWrapCellsPerBlock))
+                                  : (unsigned)ceil_div(wcells, kWrapCellsPerBlock);
+    const dim3 wgrid(blocks), wblock(64 * kWrapCellsPerBlock);
     if (mode == 0)
-      PMPM_LAUNCH((p2g_wrap_kernel<D, 0>), (::pmpm::kname("p2g_wrap_kernel", {(long)(D), (long)(0)})), wgrid, wblock, 0, s, prm, g, in, out, part, wrec, wlive, wcells);
+      PMPM_LAUNCH((p2g_wrap_kernel<D, 0>), (::pmpm::kname("p2g_wrap_kernel", {(long)(D), (long)(0)})), wgrid, wblock,
+                  0, s, prm, g, in, out, part, wrec, wlist, wcount, wcells);
     else
-      PMPM_LAUNCH((p2g_wrap_kernel<D, 1>), (::pmpm::kname("p2g_wrap_kernel", {(long)(D), (long)(1)})), wgrid, wblock, 0, s, prm, g, in, out, part, wrec, wlive, wcells);
+      PMPM_LAUNCH((p2g_wrap_kernel<D, 1>), (::pmpm::kname("p2g_wrap_kernel", {(long)(D), (long)(1)})), wgrid, wblock,
+                  0, s, prm, g, in, out, part, wrec, wlist, wcount, wcells);
   }
 }
 template <int D, int SF>
@@ -2205,14 +2202,7 @@ int launch_grid(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, c
   TileGeom tg = tile_geom(prm.dim, prm.shape_fn, g, gather);
   const float *partials = gather ? ctx->partials : nullptr;
   const int32_t *flags = gather ? ctx->tile_flags : nullptr;
-  if (gather == 1 || gather == 2) {
-    tg.wrap = wrap_mode(prm, g);
-    if (tg.wrap == 2) {
-      tg.wcells = prm.dim == 2 ? wrap_cell_count<2>(g) : wrap_cell_count<3>(g);
-      tg.wrec = ctx->wrap_rec;
-      tg.wlive = (const int32_t *)ctx->wrap_live;
-    }
-  }
+
   switch (prm.dim) {
     case 1: PMPM_LAUNCH(grid_kernel<1>, (::pmpm::kname("grid_kernel", {(long)(1)}) + " gather " + std::to_string(gather)), dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt); break;
     case 2: PMPM_LAUNCH(grid_kernel<2>, (::pmpm::kname("grid_kernel", {(long)(2)}) + " gather " + std::to_string(gather)), dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt); break;
@@ -2252,9 +2242,49 @@ static int reserve_tiles(pmpm_context *ctx, const pmpm_params &prm, const pmpm_g
   int rc = ensure_partials(ctx, tiles * tg.nv * ntn, tiles);
   if (rc != PMPM_OK || wrap_mode(prm, g) != 2) return rc;
   const size_t wcells = (size_t)(prm.dim == 2 ? wrap_cell_count<2>(g) : wrap_cell_count<3>(g));
-  if ((rc = ensure_buffer(ctx->wrap_rec, ctx->wrap_rec_cap, (size_t)(1 + 2 * prm.dim) * wcells * 64)) != PMPM_OK)
-    return rc;
-  return ensure_buffer(ctx->wrap_live, ctx->wrap_live_cap, wcells);
+  return ensure_buffer(ctx->wrap_rec, ctx->wrap_rec_cap, (size_t)(1 + 2 * prm.dim) * wcells * 64);
+}
+
+// The wrap-around sums of a P2G pass (gather 1) or MUSL re-map (gather 2)
+// into the node arrays, after its tile kernel and wrap kernel.
+static int launch_wrap_gather(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const pmpm_spatial &part,
+                              const pmpm_nodes &nodes, int gather, hipStream_t s) {
+  const int mode = wrap_mode(prm, g);
+  if (mode == 0) return PMPM_OK;
+  TileGeom tg = tile_geom(prm.dim, prm.shape_fn, g, gather);
+  tg.wrap = mode;
+  tg.wcells = prm.dim == 2 ? wrap_cell_count<2>(g) : wrap_cell_count<3>(g);
+  tg.wrec = ctx->wrap_rec;
+  tg.cstart = part.cell_start;
+  tg.cend = part.cell_end;
+  const int targets = prm.dim == 2 ? wrap_target_count<2>(g) : wrap_target_count<3>(g);
+  const dim3 grid((unsigned)ceil_div(targets, 256)), block(256);
+  if (prm.dim == 2) {
+    if (gather == 1)
+      PMPM_LAUNCH((wrap_gather_kernel<2, 5>), (std::string("wrap_gather_kernel<2, 5>")), grid, block, 0, s, g, tg,
+                  ctx->partials, ctx->tile_flags, nodes);
+    else
+      PMPM_LAUNCH((wrap_gather_kernel<2, 2>), (std::string("wrap_gather_kernel<2, 2>")), grid, block, 0, s, g, tg,
+                  ctx->partials, ctx->tile_flags, nodes);
+  } else {
+    if (gather == 1)
+      PMPM_LAUNCH((wrap_gather_kernel<3, 7>), (std::string("wrap_gather_kernel<3, 7>")), grid, block, 0, s, g, tg,
+                  ctx->partials, ctx->tile_flags, nodes);
+    else
+      PMPM_LAUNCH((wrap_gather_kernel<3, 3>), (std::string("wrap_gather_kernel<3, 3>")), grid, block, 0, s, g, tg,
+                  ctx->partials, ctx->tile_flags, nodes);
+  }
+  PMPM_LAUNCH_CHECK();
+  return PMPM_OK;
+}
+
+// The context's wrap-cell list when it belongs to this partition and grid.
+static const int32_t *wrap_list_for(const pmpm_context *ctx, const pmpm_grid &g, const pmpm_spatial &part) {
+  if (!ctx->wrap_list || ctx->wrap_list_cs != part.cell_start) return nullptr;
+  const pmpm_grid &h = ctx->wrap_list_grid;
+  for (int a = 0; a < 3; ++a)
+    if (h.num_cells[a] != g.num_cells[a]) return nullptr;
+  return ctx->wrap_list;
 }
 
 // Scatter into the tile partials (+ wrap-around deposits into the cleared
@@ -2266,9 +2296,9 @@ int do_p2g(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const 
   if (rc != PMPM_OK) return rc;
   if (!zeroed && (rc = do_zero_nodes(prm, g, nodes, s)) != PMPM_OK) return rc;
   PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, p, p, part, nodes, 0, ctx->partials, ctx->tile_flags,
-                    ctx->wrap_rec, (int32_t *)ctx->wrap_live, s);
+                    ctx->wrap_rec, wrap_list_for(ctx, g, part), ctx->counters + 1, s);
   PMPM_LAUNCH_CHECK();
-  return PMPM_OK;
+  return launch_wrap_gather(ctx, prm, g, part, nodes, 1, s);
 }
 
 int do_g2p(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
@@ -2292,9 +2322,9 @@ int do_musl_map(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, c
   // for the re-map's wrap-around atomics first
   if (!zeroed && (rc = zero_fill(nodes.mom_nt, (int64_t)g.num_cells_total * prm.dim, s)) != PMPM_OK) return rc;
   PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, in, out, part, nodes, 1, ctx->partials,
-                    ctx->tile_flags, ctx->wrap_rec, (int32_t *)ctx->wrap_live, s);
+                    ctx->tile_flags, ctx->wrap_rec, wrap_list_for(ctx, g, part), ctx->counters + 1, s);
   PMPM_LAUNCH_CHECK();
-  return PMPM_OK;
+  return launch_wrap_gather(ctx, prm, g, part, nodes, 2, s);
 }
 
 }  // namespace pmpm
@@ -3053,7 +3083,7 @@ extern "C" int pmpm_adjoint_step(pmpm_context *ctx, const pmpm_params *prm, cons
     PMPM_LAUNCH_CHECK();
   }
   PMPM_DISPATCH_DSF(D, prm->shape_fn, launch_p2g, *prm, g, *state, *state, *part, *nodes, 2, ctx->partials,
-                    ctx->tile_flags, nullptr, nullptr, s, ctx->adj_rec);
+                    ctx->tile_flags, nullptr, nullptr, nullptr, s, ctx->adj_rec);
   PMPM_LAUNCH_CHECK();
   // grid adjoint
   const TileGeom tg = tile_geom(D, prm->shape_fn, g, 3);
