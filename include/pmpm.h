@@ -256,6 +256,20 @@ int pmpm_step(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid, 
               const pmpm_particles *a, pmpm_particles *b, pmpm_spatial *part,
               pmpm_nodes *nodes, pmpm_stream_t stream);
 
+/* ---- one full step that allocates nothing. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). It runs the
+ * step pmpm_step runs, with the same arguments minus the context: the
+ * workspace (sort keys and slots, digit histograms, P2G tile partials and
+ * flags, wrap-around records and cell list, counters) comes from a file-scope
+ * device array of PMPM_NOALLOC_SCRATCH_BYTES that each GPU holds until the
+ * process ends. A step that needs more returns PMPM_ERR_WORKSPACE (see
+ * pmpm_last_error); it calls no allocator. All calls on one GPU share that
+ * GPU's array: calls that overlap on one GPU race. */
+#define PMPM_NOALLOC_SCRATCH_BYTES (64u << 20)
+int pmpm_step_noalloc(const pmpm_params *prm, const pmpm_grid *grid, int32_t musl, const pmpm_particles *a,
+                      pmpm_particles *b, pmpm_spatial *part, pmpm_nodes *nodes, pmpm_stream_t stream);
+
 /* ---- adjoint of one USL step (differentiable form; the reference has no
  * counterpart: it is the reverse mode of the step pmpm_step(musl = 0) runs,
  * usl.cpp:69-115, for the hand-written adjoint kernels of P2G, the grid

@@ -21,8 +21,8 @@ BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["partition.hip", "transfer.hip", "step.hip", "rigid.hip", "batch.hip", "timing.hip"]
-HOST_SOURCES = ["host.cpp", "bind.cpp"]
+HIP_SOURCES = ["partition.hip", "transfer.hip", "step.hip", "rigid.hip", "batch.hip", "timing.hip", "noalloc.hip"]
+HOST_SOURCES = ["host.cpp", "bind.cpp", "output.cpp"]
 
 DEVICE_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",

@@ -15,6 +15,7 @@ PMPM_ERR_INVALID = -1
 PMPM_ERR_HIP = -2
 PMPM_ERR_WORKSPACE = -3
 PMPM_ERR_UNSUPPORTED = -4
+NOALLOC_SCRATCH_BYTES = 64 << 20  # PMPM_NOALLOC_SCRATCH_BYTES
 
 SF = {"linear": 0, "quadratic": 1, "cubic": 2}
 MAT_LINEAR_ELASTIC, MAT_NEWTON_FLUID, MAT_LOCAL_RHEO, MAT_DRUCKER_PRAGER = 1, 2, 3, 4
@@ -110,6 +111,8 @@ PROTOTYPES = {
     "pmpm_stress_update": (C.c_int, [P(Params), C.c_int32, P(Particles), vp]),
     "pmpm_step": (C.c_int, [vp, P(Params), P(Grid), C.c_int32, P(Particles), P(Particles), P(Spatial), P(Nodes),
                             vp]),
+    "pmpm_step_noalloc": (C.c_int, [P(Params), P(Grid), C.c_int32, P(Particles), P(Particles), P(Spatial), P(Nodes),
+                                    vp]),
     "pmpm_adjoint_step": (C.c_int, [vp, P(Params), P(Grid), P(Particles), P(Adjoint), P(Adjoint), vp, C.c_int32,
                                     P(Spatial), P(Nodes), vp]),
     "pmpm_batch_create": (C.c_int, [P(vp), C.c_int32]),

@@ -140,7 +140,7 @@ template <int D> void bind_dim(py::module &m, py::module &root) {
          py::arg("rigid_positions") = py::none(), py::arg("rigid_velocities") = py::none(),
          py::arg("rigid_colors") = py::none());
   pc.def("set_output_formats", &P::set_output_formats, py::arg("output_formats"));
-  pc.def("output_vtk", [](P &) {});
+  pc.def("output_vtk", &P::output_vtk);
   pc.def("set_spawner", &P::set_spawner, py::arg("rate"), py::arg("volume"));
   pc.def_readwrite("isRestart", &P::isRestart);
   pc.def_readonly("num_particles", &P::num_particles);
@@ -217,7 +217,7 @@ template <int D> void bind_dim(py::module &m, py::module &root) {
   nc.def("set_output_formats", [](N &s, const std::vector<std::string> &f) { s.output_formats = f; },
          py::arg("output_formats"));
   nc.def("give_coords", [](N &s) { return from_vecs<D>(s.give_node_coords()); });
-  nc.def("output_vtk", [](N &) {});
+  nc.def("output_vtk", &N::output_vtk);
   nc.def("reset", &N::reset);
   nc.def_property("moments", [](N &s) { return from_vecs<D>(s.moments()); },
                   [](N &s, py::object v) { s.set_moments(to_vecs<D>(v)); });

@@ -74,6 +74,8 @@ inline bool grid_ok(const pmpm_grid *g) {
 
 // Records the failing HIP call of the calling thread (pmpm_last_error).
 void note_hip_error(hipError_t e, const char *what);
+// Records a non-HIP failure of the calling thread (pmpm_last_error).
+void note_error(const char *what);
 inline bool hip_ok(hipError_t e, const char *what) {
   if (e == hipSuccess) return true;
   note_hip_error(e, what);
@@ -143,6 +145,16 @@ struct ZeroLists {  // buffers of 32-bit words to clear in one launch
   }
 };
 int zero_fill_lists(const ZeroLists &z, hipStream_t s);
+// One step on the workspace of ctx (arguments already checked by
+// step_args_ok); the body of pmpm_step.
+int step_on(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid, int32_t musl,
+            const pmpm_particles *a, pmpm_particles *b, pmpm_spatial *part, pmpm_nodes *nodes, hipStream_t s);
+bool step_args_ok(const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles *a, const pmpm_particles *b,
+                  const pmpm_spatial *part, const pmpm_nodes *nodes);
+// Workspace the P2G passes of one step use on grid g: tile partials (floats)
+// and tile flags, wrap-around records (floats) and wrap cells.
+void p2g_workspace(const pmpm_params &prm, const pmpm_grid &g, size_t *partial_floats, size_t *tiles,
+                   size_t *wrap_floats, size_t *wrap_cells);
 // pmpm_partition whose first kernel also clears the buffers of `extra`.
 int partition_clearing(pmpm_context *ctx, int dim, const pmpm_grid *grid, const float *x, int n, pmpm_spatial *part,
                        hipStream_t s, const ZeroLists &extra);

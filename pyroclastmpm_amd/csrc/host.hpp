@@ -113,6 +113,8 @@ template <int D> class NodesContainer {
   std::vector<VecI<D>> node_ids() const;
   std::vector<VecI<D>> node_types() const;
   pmpm_nodes view() const;
+  // nodes.cpp:214-244: writes nodes<step>.vtp / .csv for each format
+  void output_vtk() const;
 
   pmpm_grid grid{};
   VecD<D> origin{}, end{};
@@ -172,6 +174,8 @@ template <int D> class ParticlesContainer {
   void set_eps_e(const std::vector<Mat3> &v);
 
   void set_output_formats(const std::vector<std::string> &f) { output_formats = f; }
+  // particles.cpp:209-320: writes particles<step>.vtp / .csv for each format
+  void output_vtk() const;
   void set_spawner(int rate, int volume);
   void spawn_particles(int global_step);
   void set_spatialpartition(const pmpm_grid &g);
@@ -302,7 +306,7 @@ template <int D> class Solver {
   void solve_nsteps(int n);
   void run(int total_steps, int output_frequency);
   void stress_update();
-  void output() {}
+  void output();
   // stepwise pieces of the reference's API (usl.cpp:118-199), on the current
   // partition of the current positions, in place
   void partition();

@@ -13,6 +13,7 @@
 namespace pmpm {
 
 thread_local char g_last_error[256] = "";
+void note_error(const char *what) { snprintf(g_last_error, sizeof(g_last_error), "%s", what); }
 void note_hip_error(hipError_t e, const char *what) {
   snprintf(g_last_error, sizeof(g_last_error), "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
 }

@@ -20,14 +20,10 @@ int launch_grid(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, c
 
 using namespace pmpm;
 
-extern "C" int pmpm_step(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid, int32_t musl,
-                         const pmpm_particles *a, pmpm_particles *b, pmpm_spatial *part, pmpm_nodes *nodes,
-                         pmpm_stream_t stream) {
-  if (!ctx || !prm || !grid_ok(grid) || !a || !b || !part || !nodes) return PMPM_ERR_INVALID;
-  if (prm->dim < 1 || prm->dim > 3 || prm->shape_fn < 0 || prm->shape_fn > 2) return PMPM_ERR_INVALID;
-  if (prm->num_bcs < 0 || prm->num_bcs > PMPM_MAX_BCS) return PMPM_ERR_INVALID;
-  if (a->n != b->n || a->x == b->x) return PMPM_ERR_INVALID;
-  hipStream_t s = (hipStream_t)stream;
+// The step on the workspace of `ctx` (pmpm_step, pmpm_step_noalloc).
+int pmpm::step_on(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid, int32_t musl,
+                  const pmpm_particles *a, pmpm_particles *b, pmpm_spatial *part, pmpm_nodes *nodes,
+                  hipStream_t s) {
   // the partition's first kernel also clears the node arrays P2G sums into,
   // and in MUSL the first grid pass leaves mom_nt cleared for the re-map
   const int64_t nn = grid->num_cells_total;
@@ -43,6 +39,22 @@ extern "C" int pmpm_step(pmpm_context *ctx, const pmpm_params *prm, const pmpm_g
   if ((rc = do_musl_map(ctx, *prm, *grid, *a, *b, *part, *nodes, s, true)) != PMPM_OK) return rc;
   if ((rc = launch_grid(ctx, *prm, *grid, *nodes, 0, 0, prm->num_bcs, 1, 2, s, 0)) != PMPM_OK) return rc;
   return do_g2p(*prm, *grid, *a, *b, *part, *nodes, 2, s);
+}
+
+bool pmpm::step_args_ok(const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles *a,
+                        const pmpm_particles *b, const pmpm_spatial *part, const pmpm_nodes *nodes) {
+  if (!prm || !grid_ok(grid) || !a || !b || !part || !nodes) return false;
+  if (prm->dim < 1 || prm->dim > 3 || prm->shape_fn < 0 || prm->shape_fn > 2) return false;
+  if (prm->num_bcs < 0 || prm->num_bcs > PMPM_MAX_BCS) return false;
+  if (prm->num_materials < 0 || prm->num_materials > PMPM_MAX_MATERIALS) return false;
+  return a->n >= 0 && a->n == b->n && a->x != b->x;
+}
+
+extern "C" int pmpm_step(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid, int32_t musl,
+                         const pmpm_particles *a, pmpm_particles *b, pmpm_spatial *part, pmpm_nodes *nodes,
+                         pmpm_stream_t stream) {
+  if (!ctx || !step_args_ok(prm, grid, a, b, part, nodes)) return PMPM_ERR_INVALID;
+  return step_on(ctx, prm, grid, musl, a, b, part, nodes, (hipStream_t)stream);
 }
 
 extern "C" const char *pmpm_version(void) { return "pyroclastmpm_amd 0.1 (gfx950)"; }

@@ -2245,6 +2245,15 @@ static int reserve_tiles(pmpm_context *ctx, const pmpm_params &prm, const pmpm_g
   return ensure_buffer(ctx->wrap_rec, ctx->wrap_rec_cap, (size_t)(1 + 2 * prm.dim) * wcells * 64);
 }
 
+void p2g_workspace(const pmpm_params &prm, const pmpm_grid &g, size_t *partial_floats, size_t *tiles,
+                   size_t *wrap_floats, size_t *wrap_cells) {
+  const TileGeom tg = tile_geom(prm.dim, prm.shape_fn, g, 1);
+  *tiles = tile_count(tg);
+  *partial_floats = *tiles * tg.nv * (size_t)tg.N[0] * tg.N[1] * tg.N[2];
+  *wrap_cells = (size_t)(prm.dim == 3 ? wrap_cell_count<3>(g) : (prm.dim == 2 ? wrap_cell_count<2>(g) : 0));
+  *wrap_floats = wrap_mode(prm, g) == 2 ? (size_t)(1 + 2 * prm.dim) * *wrap_cells * 64 : 0;
+}
+
 // The wrap-around sums of a P2G pass (gather 1) or MUSL re-map (gather 2)
 // into the node arrays, after its tile kernel and wrap kernel.
 static int launch_wrap_gather(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const pmpm_spatial &part,
