@@ -42,7 +42,8 @@ enum {
 enum { PMPM_SF_LINEAR = 0, PMPM_SF_QUADRATIC = 1, PMPM_SF_CUBIC = 2 };
 enum { PMPM_MAT_NONE = 0, PMPM_MAT_LINEAR_ELASTIC = 1, PMPM_MAT_NEWTON_FLUID = 2,
        PMPM_MAT_LOCAL_RHEO = 3, PMPM_MAT_DRUCKER_PRAGER = 4 };
-enum { PMPM_BC_NONE = 0, PMPM_BC_GRAVITY = 1, PMPM_BC_NODE_DOMAIN = 2, PMPM_BC_RIGID_SDF = 4 };
+enum { PMPM_BC_NONE = 0, PMPM_BC_GRAVITY = 1, PMPM_BC_NODE_DOMAIN = 2, PMPM_BC_RIGID_SDF = 4,
+       PMPM_BC_BODY_FORCE = 5, PMPM_BC_PLANAR_DOMAIN = 6 };
 
 /* material flags (DP) */
 enum { PMPM_MATF_UPDATE_HISTORY = 1, PMPM_MATF_VELGRAD_IS_INCREMENT = 2,
@@ -122,12 +123,23 @@ typedef struct pmpm_material {
 
 typedef struct pmpm_bc {
   int32_t kind; /* PMPM_BC_* */
-  /* NODE_DOMAIN: face0_mode[3], face1_mode[3] (0 fixed, 1 slip, other none) */
+  /* NODE_DOMAIN: face0_mode[3], face1_mode[3] (0 fixed, 1 slip, other none)
+   * BODY_FORCE:  mode (0 "forces", 1 "moments": moments += values on masked
+   *              nodes; 2 "fixed": moments_nt = values on masked nodes;
+   *              bodyforce.cpp:28-87, bodyforce_inline.h:41-59) */
   int32_t i[6];
   /* GRAVITY: gravity[3] applied this step
    * RIGID_SDF: half[3], center[3], velocity[3], rotation[9] (row-major,
-   *            body->world), band, advance (1 = move body by velocity*dt) */
+   *            body->world), band, advance (1 = move body by velocity*dt)
+   * PLANAR_DOMAIN: face0 friction[3], face1 friction[3] in radians, used as
+   *            the friction coefficients of the walls at the grid's origin
+   *            and at the domain end (planardomain.cpp:27-33), end[3] = the
+   *            domain end the walls sit at (the reference's Grid::end) */
   float f[24];
+  /* BODY_FORCE: values [dim][num_cells_total] (device), mask [num_cells_total]
+   * (device, nonzero = apply); NULL for the other kinds */
+  const float *values;
+  const uint8_t *mask;
 } pmpm_bc;
 
 /* Everything one step needs besides the buffers. */

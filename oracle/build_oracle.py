@@ -18,7 +18,7 @@ def build(verbose: bool = False) -> str:
     ext = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
     out = os.path.join(HERE, "_oracle" + ext)
     cmd = [
-        "g++", "-O2", "-std=c++17", "-shared", "-fPIC",
+        "g++", "-O3", "-std=c++17", "-shared", "-fPIC",
         "-ffp-contract=off",  # no FMA contraction: the reference CPU build has none
         "-I" + pybind11.get_include(),
         "-I" + sysconfig.get_paths()["include"],

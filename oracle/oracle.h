@@ -358,7 +358,8 @@ template <class R> struct Material {
 // ---------------------------------------------------------------------------
 // Boundary conditions.
 // ---------------------------------------------------------------------------
-enum BCKind { BC_GRAVITY = 1, BC_NODE_DOMAIN = 2, BC_RIGID_LEVELSET = 3, BC_RIGID_SDF = 4 };
+enum BCKind { BC_GRAVITY = 1, BC_NODE_DOMAIN = 2, BC_RIGID_LEVELSET = 3, BC_RIGID_SDF = 4, BC_BODY_FORCE = 5,
+              BC_PLANAR_DOMAIN = 6 };
 
 template <class R, int D> struct BoundaryCondition {
   BCKind kind = BC_GRAVITY;
@@ -379,6 +380,14 @@ template <class R, int D> struct BoundaryCondition {
   Vec<R, 3> sdf_half{}, sdf_center{}, sdf_velocity{};
   Mat<R, 3> sdf_rot = Mat<R, 3>::eye();
   R sdf_band = R(0);
+  // BodyForce (bodyforce.cpp:28-50): mode 0 "forces", 1 "moments", 2 "fixed";
+  // one value and mask entry per node
+  int body_mode = 0;
+  std::vector<Vec<R, D>> body_values;
+  std::vector<uint8_t> body_mask;
+  // PlanarDomain (planardomain.cpp:27-33): friction per face, stored in
+  // radians as the reference converts them
+  Vec<R, D> face0_friction = Vec<R, D>::zero(), face1_friction = Vec<R, D>::zero();
 };
 
 // Box signed distance in the body frame and its world-frame gradient.
@@ -1117,6 +1126,53 @@ template <class R, int D> struct Sim {
     if (advance)
       for (int a = 0; a < D; ++a) bc.sdf_center[a] += bc.sdf_velocity[a] * dt;
   }
+  // planardomain_inline.h:56-138 (one particle; apply_on_particles,
+  // planardomain.cpp:40-66). The radius is half the D-th root of the volume
+  // (pow of float arguments), the scalar m / dt^2 * overlap is a double that
+  // scales the float vector n - mu (v - (n.v) n), as Eigen rounds a double
+  // scalar to the vector's float before the product.
+  void apply_planar_domain(const BoundaryCondition<R, D> &bc) {
+    for (int p = 0; p < num_particles; ++p) {
+      const V pos = positions[p], vel = velocities[p];
+      const R vol = volumes[p];
+      const R inv_dim = (R)1. / (R)D;
+      const double pow_vol = std::pow(vol, inv_dim);
+      const R radius = (R)0.5 * (R)pow_vol;
+      const R mass = masses[p];
+      for (int face = 0; face < 2; ++face) {
+        for (int i = 0; i < D; ++i) {
+          const R gap = face == 0 ? pos[i] - grid.origin[i] : grid.end[i] - pos[i];
+          const R overlap = radius - gap;
+          if (!(overlap > 0)) continue;
+          V normal = V::zero();
+          normal[i] = face == 0 ? R(1) : R(-1);
+          R ndotv = R(0);
+          for (int a = 0; a < D; ++a) ndotv += normal[a] * vel[a];
+          const R fric = face == 0 ? bc.face0_friction[i] : bc.face1_friction[i];
+          V fric_term;
+          for (int a = 0; a < D; ++a) fric_term[a] = normal[a] - fric * (vel[a] - ndotv * normal[a]);
+          const R scale = (R)(((double)mass / std::pow((double)dt, 2.)) * (double)overlap);
+          for (int a = 0; a < D; ++a) forces_external[p][a] += scale * fric_term[a];
+        }
+      }
+    }
+  }
+  // apply_on_particles hooks in list order (usl.cpp:79-84, 109-114)
+  void apply_particle_bcs() {
+    for (auto &bc : bcs)
+      if (bc.kind == BC_PLANAR_DOMAIN) apply_planar_domain(bc);
+  }
+  // bodyforce_inline.h:41-59 (bodyforce.cpp:56-87): no mass test. In the
+  // MUSL re-map pass only the kinematic "fixed" mode applies again.
+  void apply_body_force(const BoundaryCondition<R, D> &bc, bool second) {
+    for (int n = 0; n < grid.num_cells_total; ++n) {
+      if (!bc.body_mask[n]) continue;
+      if (bc.body_mode == 2) node_moments_nt[n] = bc.body_values[n];
+      else if (!second)
+        for (int a = 0; a < D; ++a) node_moments[n][a] += bc.body_values[n][a];
+    }
+  }
+
   // apply_on_nodes_moments hooks in list order (usl.cpp:96-103). The MUSL
   // re-map pass (second=true) applies only the kinematic constraints: gravity
   // is already in the particle velocities and the SDF body does not advance.
@@ -1127,6 +1183,8 @@ template <class R, int D> struct Sim {
         case BC_NODE_DOMAIN: apply_node_domain(bc); break;
         case BC_RIGID_LEVELSET: if (!second) apply_rigid_levelset(bc); break;
         case BC_RIGID_SDF: apply_rigid_sdf(bc, !second); break;
+        case BC_BODY_FORCE: apply_body_force(bc, second); break;
+        case BC_PLANAR_DOMAIN: break;  // a particle condition
       }
     }
   }
@@ -1144,6 +1202,7 @@ template <class R, int D> struct Sim {
     reset();
     partition();
     calculate_shape_function();
+    apply_particle_bcs();
     p2g();
     apply_node_bcs(false);
     if (!musl) {
@@ -1155,6 +1214,7 @@ template <class R, int D> struct Sim {
       g2p(2);
     }
     stress_update();
+    apply_particle_bcs();  // (forces the next reset clears)
   }
   void step() { solve(); global_step += 1; }
   // Solver::run executes total_steps + 1 solves (solver.cpp:140-174)

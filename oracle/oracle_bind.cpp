@@ -184,6 +184,22 @@ template <class R, int D> void bind_sim(py::module &m, const std::string &name) 
                    for (int j = 0; j < 3; ++j) bc.sdf_rot(i, j) = (R)r[i * 3 + j];
                }
                bc.sdf_band = (R)getp(p, "band", 0);
+             } else if (kind == "BodyForce") {
+               bc.kind = BC_BODY_FORCE;
+               const std::string mode = p.contains("mode") ? py::cast<std::string>(p["mode"]) : "forces";
+               bc.body_mode = mode == "fixed" ? 2 : (mode == "moments" ? 1 : 0);
+               bc.body_values = to_vecs<R, D>(p["values"]);
+               for (auto x : to_scalars<double>(p["mask"])) bc.body_mask.push_back(x != 0 ? 1 : 0);
+               if ((int)bc.body_values.size() != s.grid.num_cells_total ||
+                   (int)bc.body_mask.size() != s.grid.num_cells_total)
+                 throw std::invalid_argument("BodyForce: one value and mask entry per node");
+             } else if (kind == "PlanarDomain") {
+               bc.kind = BC_PLANAR_DOMAIN;
+               const V f0 = vec("face0_friction"), f1 = vec("face1_friction");
+               for (int i = 0; i < D; ++i) {  // planardomain.cpp:27-33
+                 bc.face0_friction[i] = f0[i] * (R)(3.14159265358979323846 / 180.);
+                 bc.face1_friction[i] = f1[i] * (R)(3.14159265358979323846 / 180.);
+               }
              } else
                throw std::invalid_argument("unknown bc " + kind);
              s.bcs.push_back(bc);
@@ -209,6 +225,7 @@ template <class R, int D> void bind_sim(py::module &m, const std::string &name) 
            [](S &s, const std::string &k) -> py::object {
              if (k == "positions") return from_vecs<R, D>(s.positions);
              if (k == "velocities") return from_vecs<R, D>(s.velocities);
+             if (k == "forces_external") return from_vecs<R, D>(s.forces_external);
              if (k == "masses") return from_scalars(s.masses);
              if (k == "volumes") return from_scalars(s.volumes);
              if (k == "volumes_original") return from_scalars(s.volumes_original);
@@ -284,6 +301,7 @@ template <class R, int D> void bind_sim(py::module &m, const std::string &name) 
       .def("g2p", &S::g2p, py::arg("mode") = 0)
       .def("p2g_double_mapping", &S::p2g_double_mapping)
       .def("apply_node_bcs", &S::apply_node_bcs, py::arg("second") = false)
+      .def("apply_particle_bcs", &S::apply_particle_bcs)
       .def("stress_update", &S::stress_update)
       .def("stress_update_material", &S::stress_update_material)
       .def("initialize", &S::initialize)

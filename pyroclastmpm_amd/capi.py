@@ -19,7 +19,7 @@ NOALLOC_SCRATCH_BYTES = 64 << 20  # PMPM_NOALLOC_SCRATCH_BYTES
 
 SF = {"linear": 0, "quadratic": 1, "cubic": 2}
 MAT_LINEAR_ELASTIC, MAT_NEWTON_FLUID, MAT_LOCAL_RHEO, MAT_DRUCKER_PRAGER = 1, 2, 3, 4
-BC_GRAVITY, BC_NODE_DOMAIN, BC_RIGID_SDF = 1, 2, 4
+BC_GRAVITY, BC_NODE_DOMAIN, BC_RIGID_SDF, BC_BODY_FORCE, BC_PLANAR_DOMAIN = 1, 2, 4, 5, 6
 MATF_UPDATE_HISTORY, MATF_VELGRAD_IS_INCREMENT, MATF_APEX_GUARD, MATF_HAS_STRESS_REF = 1, 2, 4, 8
 PF_RIGID, PF_ACTIVE = 1, 2
 MAX_MATERIALS = 8
@@ -62,7 +62,7 @@ class Material(C.Structure):
 
 
 class BC(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("i", C.c_int32 * 6), ("f", C.c_float * 24)]
+    _fields_ = [("kind", C.c_int32), ("i", C.c_int32 * 6), ("f", C.c_float * 24), ("values", vp), ("mask", vp)]
 
 
 class Params(C.Structure):

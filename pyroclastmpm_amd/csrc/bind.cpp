@@ -273,6 +273,36 @@ template <int D> void bind_dim(py::module &m, py::module &root) {
              return new NodeDomain<D>(a, b);
            }),
            py::arg("face0_mode") = py::none(), py::arg("face1_mode") = py::none());
+  // bodyforce_bind.cpp:40-77
+  py::class_<BodyForce<D>>(m, "BodyForce", py::dynamic_attr())
+      .def(py::init([](const std::string &mode, py::object values, py::object mask) {
+             auto v = to_vecs<D>(values);
+             std::vector<bool> mk;
+             if (!mask.is_none()) {
+               py::array_t<bool, py::array::c_style | py::array::forcecast> a(mask);
+               mk.assign(a.data(), a.data() + a.size());
+             }
+             return new BodyForce<D>(mode, v, mk);
+           }),
+           py::arg("mode"), py::arg("values"), py::arg("mask"))
+      .def_readonly("mode_id", &BodyForce<D>::mode_id)
+      .def_readwrite("isActive", &BodyForce<D>::isActive);
+  // planardomain_bind.cpp:41-73 (friction angles in degrees)
+  py::class_<PlanarDomain<D>>(m, "PlanarDomain", py::dynamic_attr())
+      .def(py::init([](py::object f0, py::object f1) {
+             VecD<D> a{}, b{};
+             if (!f0.is_none()) a = to_vec<D>(f0);
+             if (!f1.is_none()) b = to_vec<D>(f1);
+             return new PlanarDomain<D>(a, b);
+           }),
+           py::arg("face0_friction") = py::none(), py::arg("face1_friction") = py::none())
+      .def_property_readonly("face0_friction", [](PlanarDomain<D> &s) {
+        return arr(std::vector<float>(s.face0_friction.begin(), s.face0_friction.end()));
+      })
+      .def_property_readonly("face1_friction", [](PlanarDomain<D> &s) {
+        return arr(std::vector<float>(s.face1_friction.begin(), s.face1_friction.end()));
+      })
+      .def_readwrite("isActive", &PlanarDomain<D>::isActive);
   if (D > 1) {
     py::class_<RigidBodyLevelSet<D>>(m, "RigidBodyLevelSet", py::dynamic_attr())
         .def(py::init<>())

@@ -29,13 +29,6 @@ struct pmpm_context {
   // ([fields][cells][64] floats)
   float *wrap_rec = nullptr;
   size_t wrap_rec_cap = 0;
-  // the non-empty wrap cells the last pmpm_partition found (counters[1] of
-  // them, any order), for the partition whose cell_start array and grid
-  // these are
-  int32_t *wrap_list = nullptr;
-  size_t wrap_list_cap = 0;
-  const int32_t *wrap_list_cs = nullptr;
-  pmpm_grid wrap_list_grid{};
   // adjoint step: per-particle scatter records (15 floats, cell order) and
   // node adjoints (one record of 8 floats per node, 4 in 1D)
   float *adj_rec = nullptr;
@@ -53,7 +46,6 @@ namespace pmpm {
 int ensure_partials(pmpm_context *ctx, size_t floats, size_t tiles);
 // Grows a context-owned float buffer to at least `floats`.
 int ensure_buffer(float *&buf, size_t &cap, size_t floats);
-int ensure_ints(int32_t *&buf, size_t &cap, size_t n);
 }  // namespace pmpm
 
 namespace pmpm {
@@ -149,6 +141,8 @@ int zero_fill_lists(const ZeroLists &z, hipStream_t s);
 // step_args_ok); the body of pmpm_step.
 int step_on(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *grid, int32_t musl,
             const pmpm_particles *a, pmpm_particles *b, pmpm_spatial *part, pmpm_nodes *nodes, hipStream_t s);
+// Dimension, shape function, list sizes, and the node arrays of body forces.
+bool params_ok(const pmpm_params *prm);
 bool step_args_ok(const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles *a, const pmpm_particles *b,
                   const pmpm_spatial *part, const pmpm_nodes *nodes);
 // Workspace the P2G passes of one step use on grid g: tile partials (floats)

@@ -469,6 +469,50 @@ __device__ inline void return_map(const pmpm_params &prm, int color, uint8_t pfl
   }
 }
 
+// True when the step has a PLANAR_DOMAIN condition (a particle force).
+__host__ __device__ inline bool has_planar(const pmpm_params &prm) {
+  for (int k = 0; k < prm.num_bcs; ++k)
+    if (prm.bcs[k].kind == PMPM_BC_PLANAR_DOMAIN) return true;
+  return false;
+}
+
+// External force of one particle from the PLANAR_DOMAIN conditions
+// (planardomain_inline.h:56-138, applied before P2G: usl.cpp:79-84): a
+// particle of radius 0.5 V^(1/D) that overlaps a wall of the box
+// [origin, end] by o > 0 gets (m / dt^2) o (n - mu (v - (n.v) n)), n the
+// wall's inward normal and mu the wall's friction parameter. The scalar
+// (m / dt^2) o is formed in double and rounded to float before it scales the
+// vector, as the reference's double * float-vector product does. f is added
+// to (the caller zeroes it).
+template <int D>
+__device__ inline void planar_force(const pmpm_params &prm, const pmpm_grid &g, const float *x, const float *v,
+                                    float vol, float m, float *f) {
+  for (int k = 0; k < prm.num_bcs; ++k) {
+    const pmpm_bc &bc = prm.bcs[k];
+    if (bc.kind != PMPM_BC_PLANAR_DOMAIN) continue;
+    const float inv_dim = 1.f / (float)D;
+    const float radius = 0.5f * powf(vol, inv_dim);
+    const double s0 = (double)m / ((double)prm.dt * (double)prm.dt);
+#pragma unroll
+    for (int face = 0; face < 2; ++face) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        const float gap = face == 0 ? __fsub_rn(x[i], g.origin[i]) : __fsub_rn(bc.f[6 + i], x[i]);
+        const float ov = __fsub_rn(radius, gap);
+        if (!(ov > 0.f)) continue;
+        const float mu = bc.f[3 * face + i];
+        const float sc = (float)(s0 * (double)ov);
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          // n - mu (v - (n.v) n): the normal component of the bracket is 0
+          const float t = a == i ? (face == 0 ? 1.f : -1.f) : __fsub_rn(0.f, __fmul_rn(mu, v[a]));
+          f[a] = __fadd_rn(f[a], __fmul_rn(sc, t));
+        }
+      }
+    }
+  }
+}
+
 template <int D> __device__ inline float det_dd(const float *F) {
   if (D == 1) return F[0];
   if (D == 2) return F[0] * F[3] - F[1] * F[2];

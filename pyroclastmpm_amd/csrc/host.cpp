@@ -574,6 +574,57 @@ template <int D> pmpm_bc RigidBodySDF<D>::device() const {
   return b;
 }
 
+template <int D>
+BodyForce<D>::BodyForce(const std::string &mode, const std::vector<VecD<D>> &v, const std::vector<bool> &m)
+    : values(v), mask(m) {
+  if (mode == "forces") mode_id = 0;
+  else if (mode == "moments") mode_id = 1;
+  else if (mode == "fixed") mode_id = 2;
+  else throw std::invalid_argument("BodyForce mode must be \"forces\", \"moments\" or \"fixed\"");
+  if (values.size() != mask.size()) throw std::invalid_argument("BodyForce: values and mask differ in length");
+}
+template <int D> pmpm_bc BodyForce<D>::device(int num_nodes) {
+  pmpm_bc b{};
+  if (!isActive) return b;  // bodyforce.cpp:62-65
+  if ((int)values.size() != num_nodes)
+    throw std::invalid_argument("BodyForce: values and mask need one entry per grid node");
+  if (!values_dev) {
+    std::vector<float> vh((size_t)D * num_nodes);
+    std::vector<uint8_t> mh((size_t)num_nodes);
+    for (int i = 0; i < num_nodes; ++i) {
+      for (int a = 0; a < D; ++a) vh[(size_t)a * num_nodes + i] = values[i][a];
+      mh[i] = mask[i] ? 1 : 0;
+    }
+    values_dev = std::make_shared<DevBuf<float>>();
+    mask_dev = std::make_shared<DevBuf<uint8_t>>();
+    values_dev->upload(vh);
+    mask_dev->upload(mh);
+  }
+  b.kind = PMPM_BC_BODY_FORCE;
+  b.i[0] = mode_id;
+  b.values = values_dev->get();
+  b.mask = mask_dev->get();
+  return b;
+}
+template <int D>
+PlanarDomain<D>::PlanarDomain(const VecD<D> &f0, const VecD<D> &f1) {
+  for (int a = 0; a < D; ++a) {
+    face0_friction[a] = f0[a] * (float)(3.14159265358979323846 / 180.);
+    face1_friction[a] = f1[a] * (float)(3.14159265358979323846 / 180.);
+  }
+}
+template <int D> pmpm_bc PlanarDomain<D>::device(const VecD<D> &domain_end) const {
+  pmpm_bc b{};
+  if (!isActive) return b;
+  b.kind = PMPM_BC_PLANAR_DOMAIN;
+  for (int a = 0; a < D; ++a) {
+    b.f[a] = face0_friction[a];
+    b.f[3 + a] = face1_friction[a];
+    b.f[6 + a] = domain_end[a];
+  }
+  return b;
+}
+
 // ================================ Solver ===================================
 template <int D>
 Solver<D>::Solver(const ParticlesContainer<D> &p, const NodesContainer<D> &n, const std::vector<MaterialType> &m,
@@ -642,13 +693,17 @@ template <int D> void Solver<D>::synchronize() const { hip_check(hipStreamSynchr
 
 // Node BCs for this step in list order; gravity advances its ramp here, once
 // per step, like the reference's apply_on_nodes_moments (gravity.cpp:65-76).
-template <int D> static void fill_bcs(pmpm_params &p, std::vector<BoundaryConditionType<D>> &bcs, int step) {
+template <int D>
+static void fill_bcs(pmpm_params &p, std::vector<BoundaryConditionType<D>> &bcs, int step,
+                     const NodesContainer<D> &nodes) {
   p.num_bcs = (int)bcs.size();
   for (size_t k = 0; k < bcs.size(); ++k) {
     pmpm_bc b{};
     if (auto *g = std::get_if<Gravity<D>>(&bcs[k])) b = g->device(step);
     else if (auto *nd = std::get_if<NodeDomain<D>>(&bcs[k])) b = nd->device();
     else if (auto *sd = std::get_if<RigidBodySDF<D>>(&bcs[k])) b = sd->device();
+    else if (auto *bf = std::get_if<BodyForce<D>>(&bcs[k])) b = bf->device(nodes.grid.num_cells_total);
+    else if (auto *pd = std::get_if<PlanarDomain<D>>(&bcs[k])) b = pd->device(nodes.end);
     else b.kind = PMPM_BC_NONE;  // rigid-particle level set: own pass
     p.bcs[k] = b;
   }
@@ -663,7 +718,7 @@ template <int D> static void advance_sdf(std::vector<BoundaryConditionType<D>> &
 template <int D> void Solver<D>::solve() {
   particles.spawn_particles(globals().step);
   pmpm_params prm = params();
-  fill_bcs<D>(prm, boundaryconditions, globals().step);
+  fill_bcs<D>(prm, boundaryconditions, globals().step, nodes);
   last_prm_ = prm;
   if (has_levelset()) step_composed();
   else step_fused();
@@ -681,7 +736,8 @@ template <int D> void Solver<D>::step_fused() {
   pmpm_nodes nv = nodes.view();
   check(pmpm_step(ctx, &last_prm_, &nodes.grid, musl ? 1 : 0, &a, &b, &sp, &nv, stream), "pmpm_step");
   particles.swap_buffers();
-  synchronize();
+  // no wait here: the steps queue up on the stream, and every host read
+  // (getters, output) copies on the same stream after them
 }
 
 // General sequence for lists with a rigid-particle level set: partition, P2G,
@@ -698,7 +754,7 @@ template <int D> void Solver<D>::step_composed() {
   pmpm_nodes nv = nodes.view();
   check(pmpm_partition(ctx, D, &nodes.grid, a.x, N, &sp, stream), "pmpm_partition");
   check(pmpm_p2g(ctx, &prm, &nodes.grid, &a, &sp, &nv, stream), "pmpm_p2g");
-  DevBuf<float> rigid_dx(N * D);
+  rigid_dx_.resize((size_t)N * D);  // kept between steps: no allocation per step
   int begin = 0;
   for (int k = 0; k < prm.num_bcs; ++k) {
     auto *ls = std::get_if<RigidBodyLevelSet<D>>(&boundaryconditions[k]);
@@ -716,7 +772,7 @@ template <int D> void Solver<D>::step_composed() {
                               dpsi_orig_.get(), stream),
           "pmpm_shape_function");
     check(pmpm_rigid_levelset(&prm, &nodes.grid, &a, &sp, &nv, ls->rotation_matrix.data(), ls->COM.data(),
-                              dpsi_orig_.get(), rigid_dx.get(), stream),
+                              dpsi_orig_.get(), rigid_dx_.get(), stream),
           "pmpm_rigid_levelset");
   }
   check(pmpm_grid_bc(&prm, &nodes.grid, &nv, begin, prm.num_bcs, 0, stream), "pmpm_grid_bc");
@@ -727,9 +783,8 @@ template <int D> void Solver<D>::step_composed() {
     check(pmpm_grid_bc(&prm, &nodes.grid, &nv, 0, prm.num_bcs, 1, stream), "pmpm_grid_bc");
     check(pmpm_musl_g2p(&prm, &nodes.grid, &a, &b, &sp, &nv, stream), "pmpm_musl_g2p");
   }
-  check(pmpm_rigid_advance(D, N, b.x, rigid_dx.get(), sp.sorted, stream), "pmpm_rigid_advance");
+  check(pmpm_rigid_advance(D, N, b.x, rigid_dx_.get(), sp.sorted, stream), "pmpm_rigid_advance");
   particles.swap_buffers();
-  synchronize();
 }
 
 template <int D> void Solver<D>::solve_nsteps(int n) {  // solver.cpp:125-135
@@ -778,7 +833,7 @@ template <int D> void Solver<D>::P2G() {
 }
 template <int D> void Solver<D>::apply_boundary_conditions(bool remap_pass) {
   pmpm_params prm = params();
-  fill_bcs<D>(prm, boundaryconditions, globals().step);
+  fill_bcs<D>(prm, boundaryconditions, globals().step, nodes);
   pmpm_nodes nv = nodes.view();
   check(pmpm_grid_bc(&prm, &nodes.grid, &nv, 0, prm.num_bcs, remap_pass ? 1 : 0, stream), "pmpm_grid_bc");
   synchronize();
@@ -841,6 +896,12 @@ template class ParticlesContainer<3>;
 template struct Gravity<1>;
 template struct Gravity<2>;
 template struct Gravity<3>;
+template struct BodyForce<1>;
+template struct BodyForce<2>;
+template struct BodyForce<3>;
+template struct PlanarDomain<1>;
+template struct PlanarDomain<2>;
+template struct PlanarDomain<3>;
 template struct NodeDomain<1>;
 template struct NodeDomain<2>;
 template struct NodeDomain<3>;

@@ -290,8 +290,32 @@ template <int D> struct RigidBodySDF {
   float band;
   pmpm_bc device() const;
 };
+// bodyforce.cpp:28-87: per-node values added to the moments ("forces",
+// "moments") or imposed on the forward moments ("fixed") where mask is set.
+template <int D> struct BodyForce {
+  BodyForce(const std::string &mode, const std::vector<VecD<D>> &values, const std::vector<bool> &mask);
+  int mode_id;
+  bool isActive = true;
+  std::vector<VecD<D>> values;
+  std::vector<bool> mask;
+  // device copies ([D][nn] values, [nn] mask), made on first use and shared
+  // by copies of this object
+  std::shared_ptr<DevBuf<float>> values_dev;
+  std::shared_ptr<DevBuf<uint8_t>> mask_dev;
+  pmpm_bc device(int num_nodes);
+};
+// planardomain.cpp:27-33: frictional walls on the faces of the domain box;
+// the angles are given in degrees and kept in radians, which the contact uses
+// as friction coefficients (planardomain_inline.h:108-136).
+template <int D> struct PlanarDomain {
+  PlanarDomain(const VecD<D> &face0_friction_deg, const VecD<D> &face1_friction_deg);
+  VecD<D> face0_friction, face1_friction;
+  bool isActive = true;
+  pmpm_bc device(const VecD<D> &domain_end) const;
+};
 template <int D>
-using BoundaryConditionType = std::variant<Gravity<D>, NodeDomain<D>, RigidBodyLevelSet<D>, RigidBodySDF<D>>;
+using BoundaryConditionType = std::variant<Gravity<D>, NodeDomain<D>, RigidBodyLevelSet<D>, RigidBodySDF<D>,
+                                           BodyForce<D>, PlanarDomain<D>>;
 
 // ---------------------------------------------------------------------------
 // Solvers (solver.cpp:69-199, usl.cpp:48-199, MUSL from test_musl.h).
@@ -334,7 +358,7 @@ template <int D> class Solver {
   void initialize_materials();
   bool has_levelset() const;
   pmpm_params last_prm_{};
-  DevBuf<float> dpsi_orig_, x_orig_, psi_tmp_;
+  DevBuf<float> dpsi_orig_, x_orig_, psi_tmp_, rigid_dx_;
 };
 
 }  // namespace pmpm_host

@@ -1,7 +1,7 @@
 // noalloc.hip -- pmpm_step_noalloc: the whole step without a context and
 // without allocating. Its workspace (the radix sort's ping-pong keys, slots and
-// digit histograms, the P2G tile partials and flags, the wrap-around records
-// and cell list, two counters) is carved out of one file-scope device array of
+// digit histograms, the P2G tile partials and flags, the wrap-around records,
+// the counters) is carved out of one file-scope device array of
 // PMPM_NOALLOC_SCRATCH_BYTES, which each GPU holds for the life of the process.
 // Every call on a GPU uses that GPU's copy: calls that overlap on one GPU race.
 #include <hip/hip_runtime.h>
@@ -67,7 +67,6 @@ extern "C" int pmpm_step_noalloc(const pmpm_params *prm, const pmpm_grid *grid, 
   const size_t o_partials = c.take(partial_floats * sizeof(float));
   const size_t o_flags = c.take(tiles * sizeof(int32_t));
   const size_t o_wrec = c.take(wrap_floats * sizeof(float));
-  const size_t o_wlist = c.take(wrap_cells * sizeof(int32_t));
   if (c.used > (size_t)PMPM_NOALLOC_SCRATCH_BYTES) {
     note_error("pmpm_step_noalloc: the step needs more workspace than PMPM_NOALLOC_SCRATCH_BYTES "
                "(fewer particles or a smaller grid, or pmpm_step with a context)");
@@ -93,8 +92,6 @@ extern "C" int pmpm_step_noalloc(const pmpm_params *prm, const pmpm_grid *grid, 
   ctx.tile_flags_cap = tiles;
   ctx.wrap_rec = wrap_floats ? reinterpret_cast<float *>(base + o_wrec) : nullptr;
   ctx.wrap_rec_cap = wrap_floats;
-  ctx.wrap_list = wrap_cells ? reinterpret_cast<int32_t *>(base + o_wlist) : nullptr;
-  ctx.wrap_list_cap = wrap_cells;
   // (the context is a plain struct on this stack frame: nothing frees the
   // scratch pieces it points at)
   return step_on(&ctx, prm, grid, musl, a, b, part, nodes, (hipStream_t)stream);

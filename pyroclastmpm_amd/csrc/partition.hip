@@ -60,10 +60,9 @@ int zero_fill_lists(const ZeroLists &z, hipStream_t s) {
 template <int D>
 __global__ void hash_kernel(const float *__restrict__ x, int n, pmpm_grid g, uint32_t *__restrict__ hash,
                             int32_t *__restrict__ bins, int32_t *__restrict__ cstart, int32_t *__restrict__ cend,
-                            ZeroLists clear, uint32_t *__restrict__ list_count) {
+                            ZeroLists clear) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t == 0 && list_count) *list_count = 0u;
   for (int64_t k = t; k < g.num_cells_total; k += stride) {
     cstart[k] = 0;
     cend[k] = 0;
@@ -215,42 +214,20 @@ __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
 // Cell ranges over the sorted hashes (spatialpartition_inline.h:85-115);
 // empty cells keep the 0 of the preceding clear (spatialpartition.cpp:81-82).
 // The out-of-grid particles sort last: the first of them (or the last
-// particle, when there are none) writes their count. With `list`, the start
-// of every non-empty wrap cell (wrap_cell) also appends its index there (one
-// atomic per wave on *list_count, which the hash kernel cleared).
-__global__ void cell_range_kernel(const uint32_t *__restrict__ hs, int n, pmpm_grid g, int dim,
+// particle, when there are none) writes their count.
+__global__ void cell_range_kernel(const uint32_t *__restrict__ hs, int n, uint32_t ncells,
                                   int32_t *__restrict__ cstart, int32_t *__restrict__ cend,
-                                  uint32_t *__restrict__ out_of_grid, int32_t *__restrict__ list,
-                                  uint32_t *__restrict__ list_count) {
+                                  uint32_t *__restrict__ out_of_grid) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  const uint32_t ncells = (uint32_t)g.num_cells_total;
   const uint32_t h = hs[j];
-  int w = -1;
   if (h >= ncells) {
     if (j == 0 || hs[j - 1] < ncells) *out_of_grid = (uint32_t)(n - j);
-  } else {
-    if (j == n - 1) *out_of_grid = 0u;
-    if (j == 0 || hs[j - 1] != h) {
-      cstart[h] = j;
-      if (list) {
-        const int nx = g.num_cells[0], ny = g.num_cells[1];
-        const int c[3] = {(int)(h % (uint32_t)nx), (int)((h / (uint32_t)nx) % (uint32_t)ny),
-                          (int)(h / ((uint32_t)nx * (uint32_t)ny))};
-        w = dim == 3 ? wrap_index<3>(g, c) : (dim == 2 ? wrap_index<2>(g, c) : -1);
-      }
-    }
-    if (j == n - 1 || hs[j + 1] != h) cend[h] = j + 1;
+    return;
   }
-  const uint64_t want = __ballot(w >= 0);
-  if (want) {  // uniform over the active lanes
-    const int lane = (int)(threadIdx.x & 63);
-    const int leader = __ffsll((unsigned long long)want) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(list_count, (uint32_t)__popcll(want));
-    base = __shfl(base, leader);
-    if (w >= 0) list[base + __popcll(want & ((1ull << lane) - 1ull))] = w;
-  }
+  if (j == n - 1) *out_of_grid = 0u;
+  if (j == 0 || hs[j - 1] != h) cstart[h] = j;
+  if (j == n - 1 || hs[j + 1] != h) cend[h] = j + 1;
 }
 
 }  // namespace pmpm
@@ -303,17 +280,6 @@ int pmpm::ensure_buffer(float *&buf, size_t &cap, size_t floats) {
   return PMPM_OK;
 }
 
-int pmpm::ensure_ints(int32_t *&buf, size_t &cap, size_t n) {
-  if (n > cap) {
-    (void)hipFree(buf);
-    buf = nullptr;
-    cap = 0;
-    PMPM_HIP_CHECK(hipMalloc(&buf, n * sizeof(int32_t)));
-    cap = n;
-  }
-  return PMPM_OK;
-}
-
 extern "C" int pmpm_context_destroy(pmpm_context *ctx) {
   if (!ctx) return PMPM_OK;
   (void)hipFree(ctx->adj_rec);
@@ -322,7 +288,6 @@ extern "C" int pmpm_context_destroy(pmpm_context *ctx) {
   (void)hipFree(ctx->partials);
   (void)hipFree(ctx->tile_flags);
   (void)hipFree(ctx->wrap_rec);
-  (void)hipFree(ctx->wrap_list);
   (void)hipFree(ctx->keys_tmp);
   (void)hipFree(ctx->vals_tmp);
   (void)hipFree(ctx->hist);
@@ -368,21 +333,9 @@ int pmpm::partition_clearing(pmpm_context *ctx, int dim, const pmpm_grid *grid, 
   const pmpm_grid g = *grid;
   const uint32_t ncells = (uint32_t)g.num_cells_total;
   int rc;
-  // the non-empty wrap cells, listed by the range pass for the P2G passes
-  const int wcells = dim == 3 ? wrap_cell_count<3>(g) : (dim == 2 ? wrap_cell_count<2>(g) : 0);
-  int32_t *wlist = nullptr;
-  uint32_t *wcount = nullptr;
-  ctx->wrap_list_cs = nullptr;
-  if (wcells > 0) {
-    if ((rc = ensure_ints(ctx->wrap_list, ctx->wrap_list_cap, (size_t)wcells)) != PMPM_OK) return rc;
-    wlist = ctx->wrap_list;
-    wcount = ctx->counters + 1;
-    ctx->wrap_list_cs = part->cell_start;
-    ctx->wrap_list_grid = g;
-  }
   if (n == 0) {
     ZeroLists z = extra;
-    z.add(ctx->counters, 2);
+    z.add(ctx->counters, 1);
     z.add(part->cell_start, ncells);
     z.add(part->cell_end, ncells);
     return zero_fill_lists(z, s);
@@ -395,9 +348,9 @@ int pmpm::partition_clearing(pmpm_context *ctx, int dim, const pmpm_grid *grid, 
   nb = nb_clear > nb ? (nb_clear < 16384 ? nb_clear : (nb > 16384 ? nb : 16384)) : nb;
   const unsigned blocks = (unsigned)nb;
   switch (dim) {
-    case 1: PMPM_LAUNCH(hash_kernel<1>, (::pmpm::kname("hash_kernel", {(long)(1)})), dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, part->cell_start, part->cell_end, extra, wcount); break;
-    case 2: PMPM_LAUNCH(hash_kernel<2>, (::pmpm::kname("hash_kernel", {(long)(2)})), dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, part->cell_start, part->cell_end, extra, wcount); break;
-    default: PMPM_LAUNCH(hash_kernel<3>, (::pmpm::kname("hash_kernel", {(long)(3)})), dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, part->cell_start, part->cell_end, extra, wcount); break;
+    case 1: PMPM_LAUNCH(hash_kernel<1>, (::pmpm::kname("hash_kernel", {(long)(1)})), dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, part->cell_start, part->cell_end, extra); break;
+    case 2: PMPM_LAUNCH(hash_kernel<2>, (::pmpm::kname("hash_kernel", {(long)(2)})), dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, part->cell_start, part->cell_end, extra); break;
+    default: PMPM_LAUNCH(hash_kernel<3>, (::pmpm::kname("hash_kernel", {(long)(3)})), dim3(blocks), dim3(256), 0, s, x, n, g, part->hash, part->bins, part->cell_start, part->cell_end, extra); break;
   }
   PMPM_LAUNCH_CHECK();
   // keys are in [0, ncells] (ncells = out-of-grid bucket)
@@ -437,8 +390,8 @@ int pmpm::partition_clearing(pmpm_context *ctx, int dim, const pmpm_grid *grid, 
     kin = kout;
     vin = vout;
   }
-  PMPM_LAUNCH(cell_range_kernel, (::pmpm::kname("cell_range_kernel", {})), dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, part->hash_sorted, n, g,
-                     dim, part->cell_start, part->cell_end, ctx->counters, wlist, wcount);
+  PMPM_LAUNCH(cell_range_kernel, (::pmpm::kname("cell_range_kernel", {})), dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, part->hash_sorted, n, ncells,
+                     part->cell_start, part->cell_end, ctx->counters);
   PMPM_LAUNCH_CHECK();
   return PMPM_OK;
 }

@@ -43,10 +43,7 @@ int pmpm::step_on(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *gr
 
 bool pmpm::step_args_ok(const pmpm_params *prm, const pmpm_grid *grid, const pmpm_particles *a,
                         const pmpm_particles *b, const pmpm_spatial *part, const pmpm_nodes *nodes) {
-  if (!prm || !grid_ok(grid) || !a || !b || !part || !nodes) return false;
-  if (prm->dim < 1 || prm->dim > 3 || prm->shape_fn < 0 || prm->shape_fn > 2) return false;
-  if (prm->num_bcs < 0 || prm->num_bcs > PMPM_MAX_BCS) return false;
-  if (prm->num_materials < 0 || prm->num_materials > PMPM_MAX_MATERIALS) return false;
+  if (!params_ok(prm) || !grid_ok(grid) || !a || !b || !part || !nodes) return false;
   return a->n >= 0 && a->n == b->n && a->x != b->x;
 }
 

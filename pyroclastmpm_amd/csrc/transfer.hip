@@ -333,7 +333,7 @@ __device__ inline void node_ids(const pmpm_grid &g, int64_t lin, int *ids) {
 template <int D, int SF, int MODE>
 __device__ inline void scatter_entry_global(const pmpm_params &prm, const pmpm_grid &g, const pmpm_nodes &nodes,
                                             int nn, const PW<D, SF> &w, const int *e, float m, const float *mv,
-                                            const float A[3][3]) {
+                                            const float A[3][3], const float *fext) {
   int nb[3] = {0, 0, 0}, off[3] = {0, 0, 0};
   bool inside = true;
 #pragma unroll
@@ -359,7 +359,7 @@ __device__ inline void scatter_entry_global(const pmpm_params &prm, const pmpm_g
     for (int a = 0; a < D; ++a) {
       const float pm = wgt * mv[a];
       if (pm != 0.f) unsafeAtomicAdd(&nodes.mom[a * nn + node], pm);
-      float f = 0.f;
+      float f = wgt * fext[a];
 #pragma unroll
       for (int b = 0; b < D; ++b) f += A[a][b] * gr[b];
       if (f != 0.f) unsafeAtomicAdd(&nodes.mom_nt[a * nn + node], f);
@@ -410,11 +410,13 @@ __device__ inline void span_evals(float frac, const int *type, float inv_h, floa
 struct P2GRec {
   float x[3], v[3], m;
   Sym A;  // -V sigma (MODE 0)
+  float f[3];  // external force (MODE 0: the planar-domain walls, else 0)
   float dv[3], Lb[9];
 };
 template <int D, int MODE>
-__device__ inline void p2g_load(const pmpm_particles &in, const pmpm_particles &out, const float *rec, int n,
-                                int slot, int j, P2GRec &r) {
+__device__ inline void p2g_load(const pmpm_params &prm, const pmpm_grid &g, bool planar, const pmpm_particles &in,
+                                const pmpm_particles &out, const float *rec, int n, int slot, int j, P2GRec &r) {
+  r.f[0] = r.f[1] = r.f[2] = 0.f;
 #pragma unroll
   for (int a = 0; a < D; ++a) {
     r.x[a] = in.x[a * n + slot];
@@ -434,8 +436,10 @@ __device__ inline void p2g_load(const pmpm_particles &in, const pmpm_particles &
   r.m = in.mass[slot];
   if (MODE == 0) {
     const Sym s = load_sym(in.stress, n, slot);
-    const float mvol = -in.volume[slot];
+    const float vol = in.volume[slot];
+    const float mvol = -vol;
     r.A = Sym{mvol * s.xx, mvol * s.yy, mvol * s.zz, mvol * s.xy, mvol * s.xz, mvol * s.yz};
+    if (planar) planar_force<D>(prm, g, r.x, r.v, vol, r.m, r.f);
   } else {
     r.A = sym_zero();
   }
@@ -446,7 +450,7 @@ __device__ inline void p2g_load(const pmpm_particles &in, const pmpm_particles &
 // lane's y slice of its window goes straight to the node arrays.
 template <int D, int SF, int MODE>
 __device__ void p2g_stray(const pmpm_params &prm, const pmpm_grid &g, const pmpm_nodes &nodes, const float *x, int n,
-                          int slot, int sy, float m, const float *v, const Sym &A) {
+                          int slot, int sy, float m, const float *v, const Sym &A, const float *fext) {
   constexpr int E = Spline<SF>::ENTRIES;
   constexpr int EZ = D > 2 ? E : 1;
   if (sy >= (D > 1 ? E : 1)) return;
@@ -465,7 +469,7 @@ __device__ void p2g_stray(const pmpm_params &prm, const pmpm_grid &g, const pmpm
 #pragma unroll 1
     for (int ez = 0; ez < EZ; ++ez) {
       const int e3[3] = {ex, sy, ez};
-      scatter_entry_global<D, SF, MODE>(prm, g, nodes, g.num_cells_total, w, e3, m, mv, A3);
+      scatter_entry_global<D, SF, MODE>(prm, g, nodes, g.num_cells_total, w, e3, m, mv, A3, fext);
     }
 }
 
@@ -689,6 +693,7 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
   // to deposit
   // (MODE 1 still moves the particles: its first map gathers zeros)
   const bool none = SF == PMPM_SF_CUBIC && !prm.node_types_built;
+  const bool planar = MODE == 0 && has_planar(prm);  // wall forces on the particles
   if (!__syncthreads_or(ce > cs) || (none && MODE != 1)) {
     if (t == 0) flags[blockIdx.x] = 0;
     return;
@@ -812,6 +817,7 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
           p[a] = sym_at(cur.A, a, 0) * dxy;
           if (D > 1) p[a] += sym_at(cur.A, a, 1) * dyx;
           q[a] = D > 2 ? sym_at(cur.A, a, 2) * nxy : 0.f;
+          if (MODE == 0 && planar) p[a] += cur.f[a] * nxy;  // w f_ext (usl_inline.h:152-155)
         }
       }
 #pragma unroll
@@ -846,11 +852,11 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
     int slot = j < ce ? part.sorted[j] : 0;
     int slot_next = j + 1 < ce ? part.sorted[j + 1] : 0;
     P2GRec cur;
-    if (j < ce) p2g_load<D, MODE>(in, out, rec, n, slot, j, cur);
+    if (j < ce) p2g_load<D, MODE>(prm, g, planar, in, out, rec, n, slot, j, cur);
     for (; j < ce; ++j) {
       const int jn = j + 1;
       P2GRec nxt = cur;
-      if (jn < ce) p2g_load<D, MODE>(in, out, rec, n, slot_next, jn, nxt);
+      if (jn < ce) p2g_load<D, MODE>(prm, g, planar, in, out, rec, n, slot_next, jn, nxt);
       const int slot_cur = slot;
       slot = slot_next;
       slot_next = jn + 1 < ce ? part.sorted[jn + 1] : 0;
@@ -868,7 +874,7 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
         weights(frac, W);
         deposit(cur, W);
       } else if (MODE != 2 && zh == 0)  // (MODE 2 runs on a partition of the same positions)
-        p2g_stray<D, SF, MODE>(prm, g, nodes, in.x, n, slot_cur, sy, cur.m, cur.v, cur.A);
+        p2g_stray<D, SF, MODE>(prm, g, nodes, in.x, n, slot_cur, sy, cur.m, cur.v, cur.A, cur.f);
       cur = nxt;
     }
   } else {
@@ -934,6 +940,7 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
         for (int a = 0; a < D; ++a) fr[a] = st[a * CW + q];
         lin = __float_as_int(st[ST::F_LIN * CW + q]);
         r.m = MODE == 2 ? 1.f : st[ST::F_M * CW + q];
+        r.f[0] = r.f[1] = r.f[2] = 0.f;  // (staged passes carry no force)
 #pragma unroll
         for (int a = 0; a < D; ++a) r.v[a] = st[(ST::F_V + a) * CW + q];
         r.A = sym_zero();
@@ -1026,14 +1033,14 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
           if (own) {
             if (!none) deposit(rv, W);
           } else if (zh == 0) {
-            p2g_stray<D, SF, 1>(prm, g, nodes, in.x, n, part.sorted[j], sy, rv.m, rv.v, rv.A);
+            p2g_stray<D, SF, 1>(prm, g, nodes, in.x, n, part.sorted[j], sy, rv.m, rv.v, rv.A, rv.f);
           }
         } else if (lin != cell) {
           // a particle listed in this cell whose bin is another (only when the
           // partition was built from other positions); MODE 2 is only ever
           // called on a partition of the same positions
           if (MODE != 2 && zh == 0)
-            p2g_stray<D, SF, MODE>(prm, g, nodes, in.x, n, part.sorted[j], sy, cur.m, cur.v, cur.A);
+            p2g_stray<D, SF, MODE>(prm, g, nodes, in.x, n, part.sorted[j], sy, cur.m, cur.v, cur.A, cur.f);
         } else {
           Wts W;
           weights(fr, W);
@@ -1128,7 +1135,10 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 // REMAP: the MUSL re-map instead (p2g_kernel MODE 1's deposit): m v' with v'
 // from slot j of `out` (written by the first map, g2p_kernel MODE 1) and the
 // weights of the old positions; the slice holds the 3 momentum fields.
-template <int SF, bool REMAP = false>
+// FEXT: the step has planar-domain walls; each particle's wall force f
+// (planar_force) is staged too and w f joins the force product through its
+// A operand (Nx f_a next to dNx A_a.x, both against Ny Nz).
+template <int SF, bool REMAP = false, bool FEXT = false>
 __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles in,
                                                        pmpm_particles out, pmpm_spatial part, pmpm_nodes nodes,
                                                        int tiles_x, int tiles_y, float *__restrict__ partials,
@@ -1139,8 +1149,10 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
                 "the tile of p2g_kernel<3, SF, 0>");
   constexpr int NV = REMAP ? 3 : 7, NTN = G::NTN, NLO = G::NLO;
   static_assert(NV == p2g_nv<3, REMAP ? 1 : 0>(), "slice fields of the matching p2g_kernel mode");
+  static_assert(!(REMAP && FEXT), "the re-map carries no force");
+  constexpr int NFX = M::NF + (FEXT ? 3 : 0);  // staged rows (13..15: wall force)
   __shared__ float block[4][NV][M::BN];
-  __shared__ float stage[4][M::NF][64];
+  __shared__ float stage[4][NFX][64];
 
   const int n = in.n;
   const float inv_h = g.inv_cell_size;
@@ -1225,16 +1237,20 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
 #pragma unroll
         for (int a = 0; a < 3; ++a) v[a] = REMAP ? out.v[a * n + j] : in.v[a * n + slot];
         Sym A = sym_zero();
+        float fe[3] = {0.f, 0.f, 0.f};
         if (!REMAP) {
           const Sym sg = load_sym(in.stress, n, slot);
-          const float mvol = -in.volume[slot];
+          const float vol = in.volume[slot];
+          const float mvol = -vol;
           A = Sym{mvol * sg.xx, mvol * sg.yy, mvol * sg.zz, mvol * sg.xy, mvol * sg.xz, mvol * sg.yz};
+          if (FEXT) planar_force<3>(prm, g, x, v, vol, m, fe);
         }
         if (!own) {
           // listed in this cell but binned elsewhere (a partition of other
           // positions): deposited over its own window, no product terms here
 #pragma unroll 1
-          for (int sy = 0; sy < 4; ++sy) p2g_stray<3, SF, REMAP ? 1 : 0>(prm, g, nodes, in.x, n, slot, sy, m, v, A);
+          for (int sy = 0; sy < 4; ++sy)
+            p2g_stray<3, SF, REMAP ? 1 : 0>(prm, g, nodes, in.x, n, slot, sy, m, v, A, fe);
         }
         const float keep = own ? 1.f : 0.f;
         st[0 * 64 + lane] = fr[0];
@@ -1250,6 +1266,10 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
           st[10 * 64 + lane] = keep * A.xy;
           st[11 * 64 + lane] = keep * A.xz;
           st[12 * 64 + lane] = keep * A.yz;
+        }
+        if (FEXT) {
+#pragma unroll
+          for (int a = 0; a < 3; ++a) st[(M::NF + a) * 64 + lane] = keep * fe[a];
         }
       }
       wave_lds_sync();
@@ -1268,7 +1288,9 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
         piece(zc, __fsub_rn(fz, (float)(NLO + s2)), s2 == 0, Nz, dNz);
         const float pad = (valid && s2 < 3) ? 1.f : 0.f;
         const float a1 = valid ? Nx * Ff : 0.f;
-        const float a2x = pad * dNx * Aax, a2y = pad * Nx * Aay, a2z = pad * Nx * Aaz;
+        float a2x = pad * dNx * Aax;
+        const float a2y = pad * Nx * Aay, a2z = pad * Nx * Aaz;
+        if (FEXT) a2x += pad * Nx * st[(M::NF + (s2 < 3 ? s2 : 0)) * 64 + qq];  // + w f_a
         const float b1 = valid ? Ny * Nz : 0.f;
         const float b2y = valid ? dNy * Nz : 0.f, b2z = valid ? Ny * dNz : 0.f;
         C1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, C1, 0, 0, 0);
@@ -1370,13 +1392,14 @@ __device__ inline void wrap_cell_sums(const pmpm_params &prm, const pmpm_grid &g
       }
       s_own[lane] = own;
       P2GRec r;
-      p2g_load<D, MODE>(in, out, nullptr, n, slot, j, r);
+      p2g_load<D, MODE>(prm, g, MODE == 0 && has_planar(prm), in, out, nullptr, n, slot, j, r);
       s_m[lane] = r.m;
 #pragma unroll
       for (int a = 0; a < D; ++a) s_v[a][lane] = r.v[a];
       if constexpr (MODE == 0) {
         s_A[0][lane] = r.A.xx; s_A[1][lane] = r.A.yy; s_A[2][lane] = r.A.zz;
         s_A[3][lane] = r.A.xy; s_A[4][lane] = r.A.xz; s_A[5][lane] = r.A.yz;
+        s_A[6][lane] = r.f[0]; s_A[7][lane] = r.f[1]; s_A[8][lane] = r.f[2];  // wall force
       }
     }
     wave_lds_sync();
@@ -1391,7 +1414,8 @@ __device__ inline void wrap_cell_sums(const pmpm_params &prm, const pmpm_grid &g
           dN[a] = inv_h * ((3.f * cf[a][3] * r + 2.f * cf[a][2]) * r + cf[a][1]);
           if (s[a] == 0 && !(r < 2.f)) N[a] = dN[a] = 0.f;  // frac = 1 - 2^-24
         }
-        const float wm = ((N[0] * N[1]) * N[2]) * s_m[k];
+        const float wk = (N[0] * N[1]) * N[2];
+        const float wm = wk * s_m[k];
         const float gr[3] = {(dN[0] * N[1]) * N[2], (dN[1] * N[0]) * N[2], (dN[2] * N[0]) * N[1]};
         if constexpr (MODE == 0) {
           const Sym A{s_A[0][k], s_A[1][k], s_A[2][k], s_A[3][k], s_A[4][k], s_A[5][k]};
@@ -1399,6 +1423,7 @@ __device__ inline void wrap_cell_sums(const pmpm_params &prm, const pmpm_grid &g
 #pragma unroll
           for (int a = 0; a < D; ++a) {
             acc[1 + a] += wm * s_v[a][k];
+            acc[1 + D + a] += wk * s_A[6 + a][k];
 #pragma unroll
             for (int b = 0; b < D; ++b) acc[1 + D + a] += sym_at(A, a, b) * gr[b];
           }
@@ -1414,53 +1439,39 @@ __device__ inline void wrap_cell_sums(const pmpm_params &prm, const pmpm_grid &g
   for (int k = 0; k < NV; ++k) rec[((int64_t)k * wcells + w) * 64 + lane] = active ? acc[k] : 0.f;
 }
 
-// Wrap-around sums of typed cubic splines: one wave per non-empty cell that
-// can wrap, lane = node offset (sx, sy, sz) of the cell's span. A lane whose
-// node lies outside the grid but whose linear index is in range re-sums the
-// cell's particles with the types of the node that index reaches
-// (shapefunction.cpp:96-112). The sums are stored per (cell, lane) in `rec`
-// ([NV][wcells][64], zero for the other lanes); wrap_gather_kernel adds them
-// to their nodes in a fixed order, so the node sums do not depend on
-// scheduling. The waves walk `list` (the non-empty wrap cells the partition
-// found, *count of them) with a stride of the launch's wave count, or every
-// wrap cell when list is null. The cell's particles are first loaded into LDS
-// by the wave (lane = particle, up to 64 per round), so the per-node loops
-// read LDS instead of walking a chain of dependent global loads.
+// Wrap-around sums of typed cubic splines: one wave per cell that can wrap
+// (four per workgroup), lane = node offset (sx, sy, sz) of the cell's span. A
+// lane whose node lies outside the grid but whose linear index is in range
+// re-sums the cell's particles with the types of the node that index reaches
+// (shapefunction.cpp:96-112). The sums of a non-empty cell are stored per
+// (cell, lane) in `rec` ([NV][wcells][64], zero for the other lanes);
+// wrap_gather_kernel adds them to their nodes in a fixed order, so the node
+// sums do not depend on scheduling. Empty cells write nothing (the gather
+// skips them by their cell ranges). The cell's particles are first loaded
+// into LDS by the wave (lane = particle, up to 64 per round), so the per-node
+// loops read LDS instead of walking a chain of dependent global loads.
 constexpr int kWrapCellsPerBlock = 4;
-constexpr int kWrapListBlocks = 512;  // 2048 waves walk the list
 template <int D, int MODE>
 __global__ __launch_bounds__(64 * kWrapCellsPerBlock) void p2g_wrap_kernel(pmpm_params prm, pmpm_grid g,
                                                                            pmpm_particles in, pmpm_particles out,
                                                                            pmpm_spatial part, float *__restrict__ rec,
-                                                                           const int32_t *__restrict__ list,
-                                                                           const uint32_t *__restrict__ count,
                                                                            int wcells) {
-  constexpr int SPAN = 4, NLO = -1;
   constexpr int NV = MODE == 0 ? 1 + 2 * D : D;
-  constexpr int NS = MODE == 0 ? 6 : 1;  // staged stress terms (-V sigma)
+  constexpr int NS = MODE == 0 ? 9 : 1;  // staged -V sigma (6) and wall force (3)
   constexpr int WPB = kWrapCellsPerBlock;
   __shared__ float s_frac_all[WPB][3][64], s_v_all[WPB][3][64], s_m_all[WPB][64], s_A_all[WPB][NS][64];
   __shared__ int s_own_all[WPB][64];
-  // every wave works on its own cells; only wave-level synchronisation below
+  // every wave works on its own cell; only wave-level synchronisation below
   const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
-  float(*s_frac)[64] = s_frac_all[wv];
-  float(*s_v)[64] = s_v_all[wv];
-  float(*s_A)[64] = s_A_all[wv];
-  float *s_m = s_m_all[wv];
-  int *s_own = s_own_all[wv];
-  const int items = list ? (int)*count : wcells;
-  const int stride = (int)gridDim.x * WPB;
-  for (int item = (int)blockIdx.x * WPB + wv; item < items; item += stride) {  // uniform over the wave
-    const int w = list ? list[item] : item;
-    int c[3] = {0, 0, 0};
-    if (w < 0 || w >= wcells || !wrap_cell<D>(g, w, c)) continue;
-    const int cell = (int)lin_index(g, c, D);
-    const int ce = part.cell_end[cell];
-    const int cs = part.cell_start[cell];
-    if (ce <= cs) continue;
-    wrap_cell_sums<D, MODE, NV, NS>(prm, g, in, out, part, rec, wcells, w, c, cs, ce, lane, s_frac, s_v, s_A, s_m,
-                                    s_own);
-  }
+  const int w = (int)blockIdx.x * WPB + wv;
+  int c[3] = {0, 0, 0};
+  if (w >= wcells || !wrap_cell<D>(g, w, c)) return;  // uniform over the wave
+  const int cell = (int)lin_index(g, c, D);
+  const int ce = part.cell_end[cell];
+  const int cs = part.cell_start[cell];
+  if (ce <= cs) return;
+  wrap_cell_sums<D, MODE, NV, NS>(prm, g, in, out, part, rec, wcells, w, c, cs, ce, lane, s_frac_all[wv],
+                                  s_v_all[wv], s_A_all[wv], s_m_all[wv], s_own_all[wv]);
 }
 
 // Geometry of the P2G tiles for the grid kernel's gather.
@@ -1538,23 +1549,26 @@ __device__ inline bool gather_partials(const TileGeom &tg, const float *partials
   return true;
 }
 
-// Adds to acc the wrap-around sums that land on node `node` (coordinates ids):
-// those of every node vector nb outside the grid whose linear index is
-// `node`, nb.x = ids.x + kx nx, nb.y ≡ (node - nb.x) / nx (mod ny), within the
-// nodes a cell's window reaches ([NLO, n - 1 + NLO + SPAN - 1] per axis), in
-// a fixed order (kx, then ky ascending). Each nb's sum is its virtual node in
-// the tile partials (wrap = 1) or the records of the wrap cells whose window
-// holds it (wrap = 2: cells c = nb - NLO - s, s in [0, 4)^D, lane s).
+// This lane's share of the wrap-around sums that land on node `node`: the
+// node vectors outside the grid whose linear index is `node` (x out of range,
+// or y in 3D), and for each the cells whose window holds it. Typed cubic
+// (tg.wrap == 2): lane = the window offset (sx, sy, sz) of the source cell,
+// whose record the wrap kernel left (empty cells wrote none and are skipped);
+// other splines (tg.wrap == 1): lane 0 sums the tile partials at the outside
+// node. Returns whether this lane added anything.
 template <int D, int NV>
-__device__ inline void gather_wrapped(const pmpm_grid &g, const TileGeom &tg, const float *partials,
-                                      const int32_t *flags, int node, float *acc) {
-  if (D < 2) return;  // the only axis is the slowest: nothing wraps
+__device__ inline bool gather_wrapped_lane(const pmpm_grid &g, const TileGeom &tg, const float *partials,
+                                           const int32_t *flags, int node, int lane, float *acc) {
+  if (D < 2) return false;  // the only axis is the slowest: nothing wraps
   const int nx = g.num_cells[0], ny = g.num_cells[1], nz = D > 2 ? g.num_cells[2] : 1;
   const int span = tg.N[0] - tg.T[0] + 1;
   const int lo = tg.nlo, hi = tg.nlo + span - 1;
   const int tx = node % nx;
+  const int sx = lane & 3, sy = (lane >> 2) & 3, sz = lane >> 4;
+  const bool my_lane = D > 2 || lane < 16;
+  bool found = false;
 #pragma unroll 1
-  for (int kx = -2; kx <= 2; ++kx) {
+  for (int kx = -2; kx <= 2; ++kx) {  // wave-uniform loops over the outside nodes
     const int bx = tx + kx * nx;
     if (bx < lo || bx > nx - 1 + hi) continue;
     const int64_t q = ((int64_t)node - bx) / nx;  // exact: node - bx is a multiple of nx
@@ -1569,35 +1583,35 @@ __device__ inline void gather_wrapped(const pmpm_grid &g, const TileGeom &tg, co
       const int nb[3] = {bx, by, D > 2 ? (int)qz : 0};
       if (bx >= 0 && bx < nx && by >= 0 && by < ny && (D < 3 || (qz >= 0 && qz < nz))) continue;  // the node itself
       if (tg.wrap == 1) {
-        gather_partials<D, NV>(tg, partials, flags, nb, acc);
+        if (lane == 0) found = gather_partials<D, NV>(tg, partials, flags, nb, acc) || found;
         continue;
       }
-#pragma unroll 1
-      for (int sz = 0; sz < (D > 2 ? 4 : 1); ++sz)
-#pragma unroll 1
-        for (int sy = 0; sy < 4; ++sy)
-#pragma unroll 1
-          for (int sx = 0; sx < 4; ++sx) {
-            const int c[3] = {nb[0] + 1 - sx, nb[1] + 1 - sy, D > 2 ? nb[2] + 1 - sz : 0};
-            if (c[0] < 0 || c[0] >= nx || c[1] < 0 || c[1] >= ny || (D > 2 && (c[2] < 0 || c[2] >= nz))) continue;
-            const int w = wrap_index<D>(g, c);
-            const int cl = c[0] + nx * (c[1] + ny * c[2]);
-            if (w < 0 || tg.cend[cl] <= tg.cstart[cl]) continue;
-            const int lane = sx + 4 * (sy + 4 * sz);
+      if (!my_lane) continue;
+      const int c[3] = {nb[0] + 1 - sx, nb[1] + 1 - sy, D > 2 ? nb[2] + 1 - sz : 0};
+      if (c[0] < 0 || c[0] >= nx || c[1] < 0 || c[1] >= ny || (D > 2 && (c[2] < 0 || c[2] >= nz))) continue;
+      const int w = wrap_index<D>(g, c);
+      if (w < 0) continue;
+      const int cl = c[0] + nx * (c[1] + ny * c[2]);
+      if (tg.cend[cl] <= tg.cstart[cl]) continue;
+      const int rl = sx + 4 * (sy + 4 * sz);
 #pragma unroll
-            for (int k = 0; k < NV; ++k) acc[k] += tg.wrec[((int64_t)k * tg.wcells + w) * 64 + lane];
-          }
+      for (int k = 0; k < NV; ++k) acc[k] += tg.wrec[((int64_t)k * tg.wcells + w) * 64 + rl];
+      found = true;
     }
   }
+  return found;
 }
-// One thread per node the wrap-around can reach (wrap_target): adds its
-// wrap-around sums to the node arrays, which hold nothing else but the
-// (rare) stray deposits at this point; the grid kernel adds the tile partials
-// next. Fixed order, no atomics.
+// One wave per node the wrap-around can reach (wrap_target), four per
+// workgroup: the lanes gather their candidates in parallel, a butterfly over
+// the wave sums them (the same lanes in the same order every run) and lane 0
+// adds the result to the node arrays, which hold nothing else but the (rare)
+// stray deposits at this point; the grid kernel adds the tile partials next.
+// Fixed order, no atomics.
 template <int D, int NV>
 __global__ __launch_bounds__(256) void wrap_gather_kernel(pmpm_grid g, TileGeom tg, const float *__restrict__ partials,
                                                           const int32_t *__restrict__ flags, pmpm_nodes nodes) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = (int)threadIdx.x & 63;
+  const int t = (int)blockIdx.x * 4 + ((int)threadIdx.x >> 6);  // wave-uniform
   int ids[3] = {0, 0, 0};
   if (t >= wrap_target_count<D>(g) || !wrap_target<D>(g, t, ids)) return;
   const int nn = g.num_cells_total;
@@ -1605,7 +1619,13 @@ __global__ __launch_bounds__(256) void wrap_gather_kernel(pmpm_grid g, TileGeom 
   float acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.f;
-  gather_wrapped<D, NV>(g, tg, partials, flags, i, acc);
+  const bool found = gather_wrapped_lane<D, NV>(g, tg, partials, flags, i, lane, acc);
+  if (!__ballot(found)) return;  // uniform: no wrap-around deposit reaches this node
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o);
+  if (lane != 0) return;
   if constexpr (NV == 1 + 2 * D) {
     nodes.mass[i] += acc[0];
 #pragma unroll
@@ -1663,6 +1683,19 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
     }
 #pragma unroll
     for (int a = 0; a < D; ++a) nodes.mom_nt[a * nn + i] = 0.f;
+    // a body force acts whatever the node's mass (bodyforce_inline.h:41-59);
+    // the gathers skip the node all the same
+    for (int k = bc_begin; k < bc_end; ++k) {
+      const pmpm_bc &bc = prm.bcs[k];
+      if (bc.kind != PMPM_BC_BODY_FORCE || !bc.mask[i]) continue;
+      if (bc.i[0] == 2) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) nodes.mom_nt[a * nn + i] = bc.values[a * nn + i];
+      } else if (!remap_pass) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) nodes.mom[a * nn + i] += bc.values[a * nn + i];
+      }
+    }
     const float z[3] = {0.f, 0.f, 0.f};
     store_node_vel<D>(nodes.vel, i, z, z);
     return;
@@ -1707,6 +1740,18 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
             Q[a] = fminf(0.f, Q[a]);
           }
         }
+      }
+    } else if (bc.kind == PMPM_BC_BODY_FORCE) {
+      // bodyforce_inline.h:41-59: "fixed" sets the forward moments, the
+      // other modes add to the moments; no mass test. The MUSL re-map pass
+      // re-applies only the kinematic "fixed" mode.
+      if (!bc.mask[i]) continue;
+      if (bc.i[0] == 2) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) Q[a] = bc.values[a * nn + i];
+      } else if (!remap_pass) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) P[a] = __fadd_rn(P[a], bc.values[a * nn + i]);
       }
     } else if (bc.kind == PMPM_BC_RIGID_SDF) {
       if (m <= 0.000000001f) continue;
@@ -2130,7 +2175,7 @@ static size_t tile_count(const TileGeom &tg) { return (size_t)tg.tiles[0] * tg.t
 template <int D, int SF>
 void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
                 const pmpm_spatial &part, const pmpm_nodes &nodes, int mode, float *partials, int32_t *flags,
-                float *wrec, const int32_t *wlist, const uint32_t *wcount, hipStream_t s,
+                float *wrec, hipStream_t s,
                 const float *rec = nullptr) {
   const TileGeom tg = tile_geom_of<D, SF>(g, 1);
   const dim3 grid((unsigned)tile_count(tg)), block(P2GGeom<D, SF>::THREADS);
@@ -2139,7 +2184,11 @@ void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles
   bool mfma = false;
   if constexpr (D == 3 && SF == PMPM_SF_CUBIC) {
     mfma = (mode == 0 || mode == 1) && prm.node_types_built;
-    if (mfma && mode == 0) {
+    if (mfma && mode == 0 && has_planar(prm)) {
+      PMPM_LAUNCH((p2g_mfma_kernel<SF, false, true>),
+                  (::pmpm::kname("p2g_mfma_kernel", {(long)(SF), (long)(false), (long)(true)})), grid, dim3(256), 0, s,
+                  prm, g, in, out, part, nodes, tg.tiles[0], tg.tiles[1], partials, flags);
+    } else if (mfma && mode == 0) {
       PMPM_LAUNCH((p2g_mfma_kernel<SF, false>), (::pmpm::kname("p2g_mfma_kernel", {(long)(SF), (long)(false)})), grid, dim3(256), 0, s, prm, g, in, out, part, nodes,
                          tg.tiles[0], tg.tiles[1], partials, flags);
     } else if (mfma && mode == 1) {
@@ -2165,18 +2214,14 @@ void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles
                        tg.tiles[1], partials, flags, rec);
   }
   const int wcells = wrap_cell_count<D>(g);
-  if (mode != 2 && wrap_mode(prm, g) == 2) {
-    // the partition's list of non-empty wrap cells when it is this
-    // partition's; otherwise one wave per wrap cell
-    const unsigned blocks = wlist ? (unsigned)std::min<int64_t>(kWrapListBlocks, ceil_div(wcells, kWrapCellsPerBlock))
-                                  : (unsigned)ceil_div(wcells, kWrapCellsPerBlock);
-    const dim3 wgrid(blocks), wblock(64 * kWrapCellsPerBlock);
+  if (mode != 2 && wrap_mode(prm, g) == 2 && wcells > 0) {
+    const dim3 wgrid((unsigned)ceil_div(wcells, kWrapCellsPerBlock)), wblock(64 * kWrapCellsPerBlock);
     if (mode == 0)
       PMPM_LAUNCH((p2g_wrap_kernel<D, 0>), (::pmpm::kname("p2g_wrap_kernel", {(long)(D), (long)(0)})), wgrid, wblock,
-                  0, s, prm, g, in, out, part, wrec, wlist, wcount, wcells);
+                  0, s, prm, g, in, out, part, wrec, wcells);
     else
       PMPM_LAUNCH((p2g_wrap_kernel<D, 1>), (::pmpm::kname("p2g_wrap_kernel", {(long)(D), (long)(1)})), wgrid, wblock,
-                  0, s, prm, g, in, out, part, wrec, wlist, wcount, wcells);
+                  0, s, prm, g, in, out, part, wrec, wcells);
   }
 }
 template <int D, int SF>
@@ -2213,10 +2258,16 @@ int launch_grid(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, c
   return PMPM_OK;
 }
 
-static bool params_ok(const pmpm_params *prm) {
-  return prm && prm->dim >= 1 && prm->dim <= 3 && prm->shape_fn >= 0 && prm->shape_fn <= 2 &&
-         prm->num_materials >= 0 && prm->num_materials <= PMPM_MAX_MATERIALS && prm->num_bcs >= 0 &&
-         prm->num_bcs <= PMPM_MAX_BCS;
+bool params_ok(const pmpm_params *prm) {
+  if (!(prm && prm->dim >= 1 && prm->dim <= 3 && prm->shape_fn >= 0 && prm->shape_fn <= 2 &&
+        prm->num_materials >= 0 && prm->num_materials <= PMPM_MAX_MATERIALS && prm->num_bcs >= 0 &&
+        prm->num_bcs <= PMPM_MAX_BCS))
+    return false;
+  for (int k = 0; k < prm->num_bcs; ++k) {  // a body force reads its node arrays
+    const pmpm_bc &bc = prm->bcs[k];
+    if (bc.kind == PMPM_BC_BODY_FORCE && (!bc.values || !bc.mask || bc.i[0] < 0 || bc.i[0] > 2)) return false;
+  }
+  return true;
 }
 static bool particles_ok(const pmpm_particles *p) {
   return p && p->n >= 0 &&
@@ -2267,7 +2318,8 @@ static int launch_wrap_gather(pmpm_context *ctx, const pmpm_params &prm, const p
   tg.cstart = part.cell_start;
   tg.cend = part.cell_end;
   const int targets = prm.dim == 2 ? wrap_target_count<2>(g) : wrap_target_count<3>(g);
-  const dim3 grid((unsigned)ceil_div(targets, 256)), block(256);
+  if (targets <= 0) return PMPM_OK;
+  const dim3 grid((unsigned)ceil_div(targets, 4)), block(256);  // one wave per target node
   if (prm.dim == 2) {
     if (gather == 1)
       PMPM_LAUNCH((wrap_gather_kernel<2, 5>), (std::string("wrap_gather_kernel<2, 5>")), grid, block, 0, s, g, tg,
@@ -2287,14 +2339,6 @@ static int launch_wrap_gather(pmpm_context *ctx, const pmpm_params &prm, const p
   return PMPM_OK;
 }
 
-// The context's wrap-cell list when it belongs to this partition and grid.
-static const int32_t *wrap_list_for(const pmpm_context *ctx, const pmpm_grid &g, const pmpm_spatial &part) {
-  if (!ctx->wrap_list || ctx->wrap_list_cs != part.cell_start) return nullptr;
-  const pmpm_grid &h = ctx->wrap_list_grid;
-  for (int a = 0; a < 3; ++a)
-    if (h.num_cells[a] != g.num_cells[a]) return nullptr;
-  return ctx->wrap_list;
-}
 
 // Scatter into the tile partials (+ wrap-around deposits into the cleared
 // node arrays); a grid_kernel pass with gather = 1 completes the P2G.
@@ -2305,7 +2349,7 @@ int do_p2g(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const 
   if (rc != PMPM_OK) return rc;
   if (!zeroed && (rc = do_zero_nodes(prm, g, nodes, s)) != PMPM_OK) return rc;
   PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, p, p, part, nodes, 0, ctx->partials, ctx->tile_flags,
-                    ctx->wrap_rec, wrap_list_for(ctx, g, part), ctx->counters + 1, s);
+                    ctx->wrap_rec, s);
   PMPM_LAUNCH_CHECK();
   return launch_wrap_gather(ctx, prm, g, part, nodes, 1, s);
 }
@@ -2331,7 +2375,7 @@ int do_musl_map(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, c
   // for the re-map's wrap-around atomics first
   if (!zeroed && (rc = zero_fill(nodes.mom_nt, (int64_t)g.num_cells_total * prm.dim, s)) != PMPM_OK) return rc;
   PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, in, out, part, nodes, 1, ctx->partials,
-                    ctx->tile_flags, ctx->wrap_rec, wrap_list_for(ctx, g, part), ctx->counters + 1, s);
+                    ctx->tile_flags, ctx->wrap_rec, s);
   PMPM_LAUNCH_CHECK();
   return launch_wrap_gather(ctx, prm, g, part, nodes, 2, s);
 }
@@ -3092,7 +3136,7 @@ extern "C" int pmpm_adjoint_step(pmpm_context *ctx, const pmpm_params *prm, cons
     PMPM_LAUNCH_CHECK();
   }
   PMPM_DISPATCH_DSF(D, prm->shape_fn, launch_p2g, *prm, g, *state, *state, *part, *nodes, 2, ctx->partials,
-                    ctx->tile_flags, nullptr, nullptr, nullptr, s, ctx->adj_rec);
+                    ctx->tile_flags, nullptr, s, ctx->adj_rec);
   PMPM_LAUNCH_CHECK();
   // grid adjoint
   const TileGeom tg = tile_geom(D, prm->shape_fn, g, 3);

@@ -105,6 +105,28 @@ def column(h=0.01, per_axis=3, phi=30.0, cohesion=0.0, E=1e6, dt=1e-5, jitter=0.
                 musl=musl)
 
 
+def vibrating_bar(musl=False, alpha=0.99):
+    """1D vibrating bar of tests/examples/vibrating_bar/test_vibrating_bar.py:
+    L = 25, E = 100, rho = 1, 49 cells, one particle per cell, mode-1
+    velocity v0 sin(beta0 x), node 0 held by a "fixed" BodyForce of zeros,
+    linear spline, dt = 0.1 h / c."""
+    L, E, rho = 25.0, 100.0, 1.0
+    h = L / 49
+    c = math.sqrt(E / rho)
+    dt = 0.1 * h / c
+    v0, beta0 = 0.1, 0.5 * math.pi / L
+    x = (np.arange(0, L, h) + 0.5 * h)[:, None]
+    nn = int(np.float32(L) / np.float32(h)) + 1  # the grid's node count (float arithmetic, as the solver)
+    mask = np.zeros(nn)
+    mask[0] = 1
+    spec = Spec(dim=1, origin=[0.0], end=[L], cell_size=h, shape_function="linear", dt=dt, ppc=1,
+                positions=x, velocities=v0 * np.sin(beta0 * x),
+                materials=[("LinearElastic", {"density": rho, "E": E, "pois": 0.0})],
+                bcs=[("BodyForce", {"mode": "fixed", "values": np.zeros((nn, 1)), "mask": mask})],
+                musl=musl, alpha=alpha)
+    return spec, dict(L=L, E=E, rho=rho, v0=v0, beta0=beta0, dt=dt)
+
+
 def scoop(h=0.0125, per_axis=2, domain=(2.0, 1.0, 1.0), bed=0.4, dt=5e-6, seed=0):
     """3D rigid box scoop into a mu(I) bed, quadratic B-spline, USL."""
     spacing = h / per_axis
@@ -152,6 +174,10 @@ def _mk_bc(pm, kind, p, dim):
         return b
     if kind == "RigidBodySDF":
         return pm.RigidBodySDF(p["half_extents"], p["center"], p["velocity"], p.get("rotation"), p.get("band", 0.0))
+    if kind == "BodyForce":
+        return pm.BodyForce(p["mode"], p["values"], np.asarray(p["mask"], dtype=bool))
+    if kind == "PlanarDomain":
+        return pm.PlanarDomain(p.get("face0_friction", [0.0] * dim), p.get("face1_friction", [0.0] * dim))
     raise ValueError(kind)
 
 
