@@ -41,7 +41,8 @@ enum {
 
 enum { PMPM_SF_LINEAR = 0, PMPM_SF_QUADRATIC = 1, PMPM_SF_CUBIC = 2 };
 enum { PMPM_MAT_NONE = 0, PMPM_MAT_LINEAR_ELASTIC = 1, PMPM_MAT_NEWTON_FLUID = 2,
-       PMPM_MAT_LOCAL_RHEO = 3, PMPM_MAT_DRUCKER_PRAGER = 4 };
+       PMPM_MAT_LOCAL_RHEO = 3, PMPM_MAT_DRUCKER_PRAGER = 4, PMPM_MAT_VON_MISES = 5,
+       PMPM_MAT_MOHR_COULOMB = 6, PMPM_MAT_MODIFIED_CAM_CLAY = 7 };
 enum { PMPM_BC_NONE = 0, PMPM_BC_GRAVITY = 1, PMPM_BC_NODE_DOMAIN = 2, PMPM_BC_RIGID_SDF = 4,
        PMPM_BC_BODY_FORCE = 5, PMPM_BC_PLANAR_DOMAIN = 6 };
 
@@ -82,8 +83,14 @@ typedef struct pmpm_particles {
   uint8_t *color;    /* [n] material id */
   uint8_t *flags;    /* [n] PMPM_PF_* */
   int32_t *id;       /* [n] original particle index of each slot */
-  float *eps_e;      /* [6][n] elastic strain history (Drucker-Prager) or NULL */
-  float *stress_ref; /* [6][n] reference stress (Drucker-Prager) or NULL */
+  float *eps_e;      /* [6][n] elastic strain history (Drucker-Prager, von Mises,
+                        Mohr-Coulomb, modified Cam-Clay) or NULL */
+  float *stress_ref; /* [6][n] reference stress (Drucker-Prager, modified
+                        Cam-Clay) or NULL */
+  float *hist;       /* [2][n] plastic history scalars or NULL: row 0 the
+                        accumulated plastic strain (von Mises, Mohr-Coulomb) or
+                        the volumetric plastic strain (modified Cam-Clay), row 1
+                        the preconsolidation pressure (modified Cam-Clay) */
 } pmpm_particles;
 
 /* Floats per node of the derived velocity records pmpm_nodes.vel. */
@@ -117,7 +124,13 @@ typedef struct pmpm_material {
   /* LINEAR_ELASTIC: lame, shear, bulk
    * NEWTON_FLUID:   viscosity, bulk_modulus, gamma
    * LOCAL_RHEO:     shear, lame, bulk, rho_c, mu_s, mu_2, I0, EPS
-   * DRUCKER_PRAGER: bulk, shear, M, Pt, eta_overline, Vs */
+   * DRUCKER_PRAGER: bulk, shear, M, Pt, eta_overline, Vs
+   * VON_MISES:      bulk, shear, yield_stress, H  (3D only)
+   * MOHR_COULOMB:   bulk, shear, cohesion, friction angle, dilatancy angle
+   *                 (radians), H
+   * MODIFIED_CAM_CLAY: bulk, shear, M, lam, kap, Pt, Vs
+   * The last three need L, eps_e and hist stored; their return maps run in a
+   * pass after G2P (the reference's stress_update, solver.cpp:110-121). */
   float p[16];
 } pmpm_material;
 

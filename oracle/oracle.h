@@ -289,7 +289,8 @@ template <class R> R shape_dN(ShapeFn sf, R r, R inv_h, int type) {
 // ---------------------------------------------------------------------------
 // Materials. Parameter sets mirror the reference constructors.
 // ---------------------------------------------------------------------------
-enum MatKind { MAT_LINEAR_ELASTIC = 1, MAT_NEWTON_FLUID = 2, MAT_LOCAL_RHEO = 3, MAT_DP = 4 };
+enum MatKind { MAT_LINEAR_ELASTIC = 1, MAT_NEWTON_FLUID = 2, MAT_LOCAL_RHEO = 3, MAT_DP = 4, MAT_VON_MISES = 5,
+               MAT_MOHR_COULOMB = 6, MAT_MCC = 7 };
 
 template <class R> struct Material {
   MatKind kind = MAT_LINEAR_ELASTIC;
@@ -304,13 +305,20 @@ template <class R> struct Material {
   // Drucker-Prager (DP_rheo)
   R M = R(0), lam = R(0), kap = R(0), Vs = R(0), Rratio = R(0), Pt = R(0), beta = R(0),
     dilatancy_angle = R(0), eta_overline = R(0), solid_volume_fraction = R(1);
+  // von Mises / Mohr-Coulomb (hardening H; yield stress or cohesion,
+  // friction angle in radians); modified Cam-Clay uses M, lam, kap, Vs, Rratio,
+  // Pt above
+  R yield_stress = R(0), H = R(0), cohesion = R(0), friction_angle = R(0);
   bool do_update_history = true;
   bool is_velgrad_strain_increment = false;
   // when the apex return divides by eta_overline == 0, compute its analytic
   // limit p = Pt instead of the reference's 0*inf = NaN
   bool apex_guard = true;
-  // DP per-particle history (allocated by initialize)
-  std::vector<Mat<R, 3>> eps_e;       // D x D block used
+  // per-particle history (allocated by initialize): elastic strain (D x D
+  // block used), reference stress, preconsolidation pressure, alpha = the
+  // accumulated (von Mises, Mohr-Coulomb) or volumetric (Cam-Clay) plastic
+  // strain, solid volume fraction (DP)
+  std::vector<Mat<R, 3>> eps_e;
   std::vector<Mat<R, 3>> stress_ref;
   std::vector<R> pc, alpha, svf;
 
@@ -338,6 +346,33 @@ template <class R> struct Material {
     m.shear_modulus = ((R)1. / (R)2.) * E / ((R)1 + pois);
     m.lame_modulus = (pois * E) / (((R)1.0 + pois) * ((R)1 - (R)2.0 * pois));
     m.EPS = I0 / (R)std::sqrt(std::pow((double)particle_diameter, 2) * (double)particle_density);
+    return m;
+  }
+  // vonmises.cpp:34-50
+  static Material von_mises(R density, R E, R pois, R yield_stress, R H) {
+    Material m; m.kind = MAT_VON_MISES; m.density = density; m.E = E; m.pois = pois;
+    m.yield_stress = yield_stress; m.H = H;
+    m.bulk_modulus = E / ((R)3.0 * ((R)1.0 - (R)2.0 * pois));
+    m.shear_modulus = E / ((R)2.0 * ((R)1 + pois));
+    return m;
+  }
+  // mohrcoulomb.cpp:33-52
+  static Material mohr_coulomb(R density, R E, R pois, R cohesion, R friction_deg, R dilatancy_deg, R H) {
+    Material m; m.kind = MAT_MOHR_COULOMB; m.density = density; m.E = E; m.pois = pois;
+    m.cohesion = cohesion; m.H = H;
+    m.bulk_modulus = ((R)1. / (R)3.) * (E / ((R)1. - (R)2. * pois));
+    m.shear_modulus = ((R)1. / (R)2.) * E / ((R)1 + pois);
+    m.lame_modulus = (pois * E) / (((R)1.0 + pois) * ((R)1 - (R)2.0 * pois));
+    m.friction_angle = friction_deg * (R)(3.14159265358979323846 / 180.);
+    m.dilatancy_angle = dilatancy_deg * (R)(3.14159265358979323846 / 180.);
+    return m;
+  }
+  // modifiedcamclay.cpp:47-63
+  static Material modified_cam_clay(R density, R E, R pois, R M, R lam, R kap, R Vs, R Rratio, R Pt, R beta) {
+    Material m; m.kind = MAT_MCC; m.density = density; m.E = E; m.pois = pois; m.M = M; m.lam = lam;
+    m.kap = kap; m.Vs = Vs; m.Rratio = Rratio; m.Pt = Pt; m.beta = beta;
+    m.bulk_modulus = E / ((R)3.0 * ((R)1.0 - (R)2.0 * pois));
+    m.shear_modulus = E / ((R)2.0 * ((R)1 + pois));
     return m;
   }
   // dp_rheo.cpp:45-70
@@ -959,6 +994,382 @@ template <class R, int D> struct Sim {
       }
     }
   }
+  // vonmises.cpp:55-59 and mohrcoulomb.cpp:55-59: zero history
+  void plastic_initialize(Material<R> &mt) {
+    mt.alpha.assign(num_particles, R(0));
+    mt.eps_e.assign(num_particles, M3::zero());
+  }
+  // modifiedcamclay.cpp:68-104: stress_ref = the stresses, pc = R p0
+  void mcc_initialize(Material<R> &mt) {
+    const int N = num_particles;
+    mt.alpha.assign(N, R(0));
+    mt.pc.assign(N, R(0));
+    mt.eps_e.assign(N, M3::zero());
+    mt.stress_ref = stresses;
+    for (int p = 0; p < N; ++p) {
+      const R p0 = -trace(stresses[p]) / (R)3.;
+      mt.pc[p] = p0 * mt.Rratio;
+    }
+  }
+  // (M M^T).trace(), each diagonal entry a row dot product
+  static R trace_mmt(const M3 &m) {
+    R t = R(0);
+    for (int i = 0; i < 3; ++i) {
+      const R r = m(i, 0) * m(i, 0) + m(i, 1) * m(i, 1) + m(i, 2) * m(i, 2);
+      t = i == 0 ? r : t + r;
+    }
+    return t;
+  }
+  // Newton iterations stop here at the latest (the reference's loops have no
+  // cap; see each)
+  static constexpr int kMaxIters = 100;
+
+  // vonmises_inline.h:41-140 (3D; 2D prints a message there and leaves the
+  // stress alone)
+  void update_von_mises(Material<R> &mt, int mat_id) {
+    if (D != 3) return;
+    const R K = mt.bulk_modulus, G = mt.shear_modulus;
+    for (int p = 0; p < num_particles; ++p) {
+      if (colors[p] != mat_id) continue;
+      const M3 L3 = pad3<R, D>(velocity_gradient[p]);
+      M3 deps;
+      for (int k = 0; k < 9; ++k)
+        deps.m[k] = mt.is_velgrad_strain_increment ? L3.m[k] : (R)0.5 * (L3.m[k] + transpose(L3).m[k]) * dt;
+      M3 etr;
+      for (int k = 0; k < 9; ++k) etr.m[k] = mt.eps_e[p].m[k] + deps.m[k];
+      const R ev = trace(etr);
+      const R ptr = K * ev;
+      M3 dev = etr;
+      for (int i = 0; i < 3; ++i) dev(i, i) -= (R)((1 / 3.) * (double)ev);
+      M3 st;
+      for (int k = 0; k < 9; ++k) st.m[k] = (R)(2. * (double)G) * dev.m[k];
+      const R acc_tr = mt.alpha[p];
+      const R sy_tr = mt.yield_stress + mt.H * acc_tr;
+      const R q = (R)std::sqrt(3 * 0.5 * (double)trace_mmt(st));
+      const R phi = q - sy_tr;
+      if (phi <= 0) {
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) stresses[p](i, j) = st(i, j) + (i == j ? ptr : R(0));
+        if (mt.do_update_history) mt.eps_e[p] = etr;
+        continue;
+      }
+      double dg = 0.0, acc = 0.0, ph = 0.0;
+      for (int it = 0; it < kMaxIters; ++it) {
+        acc = (double)acc_tr + dg;
+        const double sy = (double)mt.yield_stress + (double)mt.H * acc;
+        ph = (double)q - 3.0 * (double)G * dg - sy;
+        const double d = -3.0 * (double)G - (double)mt.H;
+        dg = dg - ph / d;
+        if (!(std::fabs(ph) > 1e-7)) break;
+      }
+      const R f = (R)(1.0 - (3.0 * (double)G * dg) / (double)q);
+      M3 sc;
+      for (int k = 0; k < 9; ++k) sc.m[k] = f * st.m[k];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) stresses[p](i, j) = sc(i, j) + (i == j ? ptr : R(0));
+      if (mt.do_update_history) {
+        const R g2 = (R)(2.0 * (double)G), v3 = (R)((1. / 3.) * (double)ev);
+        M3 e;
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) e(i, j) = sc(i, j) / g2 + (i == j ? v3 : R(0));
+        mt.eps_e[p] = e;
+        mt.alpha[p] = (R)acc;
+      }
+    }
+  }
+
+  // Symmetric 3 x 3 eigen-decomposition by cyclic Jacobi rotations in double
+  // (the reference uses Eigen's closed form, SelfAdjointEigenSolver::
+  // computeDirect); w descending, column k of V the eigenvector of w[k].
+  static void sym_eig3(const double A[3][3], double w[3], double V[3][3]) {
+    double a[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) { a[i][j] = A[i][j]; V[i][j] = i == j ? 1.0 : 0.0; }
+    for (int sweep = 0; sweep < 12; ++sweep) {
+      const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+      const double dia = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+      if (off <= 1e-32 * dia || off == 0.0) break;
+      for (int p = 0; p < 2; ++p)
+        for (int q = p + 1; q < 3; ++q) {
+          if (a[p][q] == 0.0) continue;
+          const double th = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+          const double t = (th >= 0.0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+          const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+          for (int k = 0; k < 3; ++k) {
+            const double akp = a[k][p], akq = a[k][q];
+            a[k][p] = c * akp - sn * akq;
+            a[k][q] = sn * akp + c * akq;
+          }
+          for (int k = 0; k < 3; ++k) {
+            const double apk = a[p][k], aqk = a[q][k];
+            a[p][k] = c * apk - sn * aqk;
+            a[q][k] = sn * apk + c * aqk;
+          }
+          for (int k = 0; k < 3; ++k) {
+            const double vkp = V[k][p], vkq = V[k][q];
+            V[k][p] = c * vkp - sn * vkq;
+            V[k][q] = sn * vkp + c * vkq;
+          }
+        }
+    }
+    int o[3] = {0, 1, 2};
+    std::sort(o, o + 3, [&](int x, int y) { return a[x][x] > a[y][y]; });
+    double Vs[3][3];
+    for (int k = 0; k < 3; ++k) {
+      w[k] = a[o[k]][o[k]];
+      for (int i = 0; i < 3; ++i) Vs[i][k] = V[i][o[k]];
+    }
+    for (int i = 0; i < 3; ++i)
+      for (int k = 0; k < 3; ++k) V[i][k] = Vs[i][k];
+  }
+
+  // mohrcoulomb_inline.h:301-457 with the returns of lines 34-289: main plane,
+  // right edge, left edge (each kept when the truncated principal stresses
+  // stay ordered), apex
+  void update_mohr_coulomb(Material<R> &mt, int mat_id) {
+    const R K = mt.bulk_modulus, G = mt.shear_modulus, H = mt.H;
+    for (int p = 0; p < num_particles; ++p) {
+      if (colors[p] != mat_id) continue;
+      const M Lp = velocity_gradient[p];
+      M deps;
+      for (int k = 0; k < D * D; ++k)
+        deps.m[k] = mt.is_velgrad_strain_increment ? Lp.m[k] : (R)0.5 * (Lp.m[k] + transpose(Lp).m[k]) * dt;
+      const M eps_prev = block<R, D>(mt.eps_e[p]);
+      M etr;
+      for (int k = 0; k < D * D; ++k) etr.m[k] = eps_prev.m[k] + deps.m[k];
+      const R acc_tr = mt.alpha[p];
+      const R ev = trace(etr);
+      const R ptr = K * ev;
+      M sdev = etr;
+      for (int i = 0; i < D; ++i) sdev(i, i) -= (R)((1 / 3.) * (double)ev);
+      for (int k = 0; k < D * D; ++k) sdev.m[k] = (R)(2. * (double)G) * sdev.m[k];
+      const R coh_tr = mt.cohesion + H * acc_tr;
+      M3 sig = M3::zero();
+      for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) sig(i, j) = sdev(i, j);
+      for (int i = 0; i < 3; ++i) sig(i, i) += ptr;
+      double A[3][3], w[3], V[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) A[i][j] = sig(i, j);
+      sym_eig3(A, w, V);
+      const R ps[3] = {(R)w[0], (R)w[1], (R)w[2]};
+      const R sfa = std::sin(mt.friction_angle), cfa = std::cos(mt.friction_angle);
+      const R phi_tr = (R)((double)(ps[0] - ps[2] + (ps[0] + ps[2]) * sfa) - 2. * (double)coh_tr * (double)cfa);
+      if (phi_tr <= 0) {
+        stresses[p] = sig;
+        if (mt.do_update_history) {
+          M3 e = M3::zero();
+          for (int i = 0; i < D; ++i)
+            for (int j = 0; j < D; ++j) e(i, j) = etr(i, j);
+          mt.eps_e[p] = e;
+          mt.alpha[p] = acc_tr;
+        }
+        continue;
+      }
+      const R sda = std::sin(mt.dilatancy_angle);
+      const double Kd = K, Gd = G, Hd = H, sf = sfa, cf = cfa, sd = sda;
+      const R a = (R)(4. * Gd * (1. + (1. / 3.) * sf * sd) + (double)(4 * K * sfa * sda));
+      auto finish = [&](const R pc[3], double acc_next) {
+        M3 out;
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) {
+            double t = 0.0;
+            for (int k = 0; k < 3; ++k) t += V[i][k] * (double)pc[k] * V[j][k];
+            out(i, j) = (R)t;
+          }
+        stresses[p] = out;
+        if (mt.do_update_history) {
+          const R pn = (R)((double)trace(out) / 3.0);
+          const R i2g = (R)(1. / (2. * Gd)), i3k = (R)(1. / (3. * Kd));
+          M3 e = M3::zero();
+          for (int i = 0; i < D; ++i)
+            for (int j = 0; j < D; ++j) e(i, j) = (out(i, j) - (i == j ? pn : R(0))) * i2g + (i == j ? pn * i3k : R(0));
+          mt.eps_e[p] = e;
+          mt.alpha[p] = (R)acc_next;
+        }
+      };
+      auto ordered = [](const R pc[3]) {
+        return std::trunc(pc[0]) >= std::trunc(pc[1]) && std::trunc(pc[1]) >= std::trunc(pc[2]);
+      };
+      {  // main plane (34-98)
+        double dg = 0.0, ph = phi_tr, acc_next = acc_tr;
+        const double d = -(double)a - 4 * Hd * cf * cf;
+        for (int it = 0; it < kMaxIters; ++it) {
+          dg = dg - ph / d;
+          acc_next = (double)acc_tr + 2 * cf * dg;
+          const double ca = (double)mt.cohesion + Hd * acc_next;
+          ph = (double)ps[0] - (double)ps[2] + ((double)ps[0] + (double)ps[2]) * sf - 2. * ca * cf - (double)a * dg;
+          if (!(std::fabs(ph) > 1.e-6)) break;
+        }
+        const R pc[3] = {(R)((double)ps[0] - dg * (2. * Gd * (1. + (1. / 3.) * sd) + 2. * Kd * sd)),
+                         (R)((double)ps[1] + dg * ((4. / 3.) * Gd - 2. * Kd) * sd),
+                         (R)((double)ps[2] + dg * (2. * Gd * (1. - (1. / 3.) * sd) - 2. * Kd * sd))};
+        if (ordered(pc)) { finish(pc, acc_next); continue; }
+      }
+      bool done = false;
+      for (int edge = 0; edge < 2 && !done; ++edge) {  // right edge, then left (100-229)
+        const bool right = edge == 0;
+        const R s1 = ps[0] - ps[2] + (ps[0] + ps[2]) * sfa;
+        const R s2 = right ? ps[0] - ps[1] + (ps[0] + ps[1]) * sfa : ps[1] - ps[2] + (ps[1] + ps[2]) * sfa;
+        double ph0 = (double)s1 - 2. * (double)coh_tr * cf, ph1 = (double)s2 - 2. * (double)coh_tr * cf;
+        const double b = right ? 2 * Gd * (1. + sf + sd - (1. / 3.) * sf * sd) + 4. * Kd * sf * sd
+                               : 2 * Gd * (1. - sf - sd - (1 / 3.) * sf * sd) + 4. * Kd * sf * sd;
+        double g0 = 0.0, g1 = 0.0, acc_next = acc_tr;
+        for (int it = 0; it < kMaxIters; ++it) {
+          const double d00 = -(double)a - 4. * Hd * cf * cf, d01 = -b - (double)(4 * H * cfa * cfa);
+          const double invdet = 1.0 / (d00 * d00 - d01 * d01);
+          const double i00 = d00 * invdet, i01 = -d01 * invdet;
+          const double n0 = g0 - (i00 * ph0 + i01 * ph1), n1 = g1 - (i01 * ph0 + i00 * ph1);
+          g0 = n0;
+          g1 = n1;
+          acc_next = (double)acc_tr + 2 * cf * (g0 + g1);
+          const double ca = (double)mt.cohesion + Hd * acc_next;
+          ph0 = (double)s1 - 2. * ca * cf - (double)a * g0 - b * g1;
+          ph1 = (double)s2 - 2. * ca * cf - (double)a * g1 - b * g0;
+          if (!(std::fabs(ph0) + std::fabs(ph1) > 1.e-7)) break;
+        }
+        const double c1 = 2. * Gd * (1. + (1. / 3.) * sd) + 2. * Kd * sd, c2 = ((4. / 3.) * Gd - 2. * Kd) * sd,
+                     c3 = 2. * Gd * (1. - (1. / 3.) * sd) - 2. * Kd * sd;
+        R pc[3];
+        if (right) {
+          pc[0] = (R)((double)ps[0] - c1 * (g0 + g1));
+          pc[1] = (R)((double)ps[1] + c2 * g0 + c3 * g1);
+          pc[2] = (R)((double)ps[2] + c3 * g0 + c2 * g1);
+        } else {
+          pc[0] = (R)((double)ps[0] - c1 * g0 + c2 * g1);
+          pc[1] = (R)((double)ps[1] + c2 * g0 - c1 * g1);
+          pc[2] = (R)((double)ps[2] + c3 * (g0 + g1));
+        }
+        if (ordered(pc)) { finish(pc, acc_next); done = true; }
+      }
+      if (done) continue;
+      // apex (231-270)
+      double dv = 0.0, ph = phi_tr, pn = 0.0, acc_next = acc_tr;
+      const double al = cf / sd, cotf = cf / sf;
+      for (int it = 0; it < kMaxIters; ++it) {
+        acc_next = (double)acc_tr + al * dv;
+        const double ca = (double)mt.cohesion + Hd * acc_next;
+        pn = (double)ptr - Kd * dv;
+        ph = ca * cotf - pn;
+        dv = dv - ph / ((Hd * cf * cotf) / sd + Kd);
+        if (!(std::fabs(ph) > 1.e-6)) break;
+      }
+      stresses[p] = M3::zero();
+      for (int i = 0; i < 3; ++i) stresses[p](i, i) = (R)pn;
+      if (mt.do_update_history) {
+        M3 e = M3::zero();
+        for (int i = 0; i < D; ++i) e(i, i) = (R)(pn * (1. / (3. * Kd)));
+        mt.eps_e[p] = e;
+        mt.alpha[p] = (R)acc_next;
+      }
+    }
+  }
+
+  // modifiedcamclay_inline.h:269-413 (update_modifiedcamclay_nonlinear, the
+  // update modifiedcamclay.cpp:147-176 runs). p compression positive.
+  static R mcc_yield(R p, R Pt, R ps, R q, R M) {  // compute_yield_function (:34-41)
+    return (ps - p - Pt) * (ps - p - Pt) + (q / M) * (q / M) - ps * ps;
+  }
+  void update_mcc(Material<R> &mt, int mat_id) {
+    const R K = mt.bulk_modulus, G = mt.shear_modulus, Mm = mt.M, Pt = mt.Pt;
+    for (int p = 0; p < num_particles; ++p) {
+      if (colors[p] != mat_id) continue;
+      const M Lp = velocity_gradient[p];
+      const M eps_prev = block<R, D>(mt.eps_e[p]);
+      M deps;
+      if (mt.is_velgrad_strain_increment) {
+        deps = Lp;
+      } else {
+        const M LT = transpose(Lp);
+        M W;
+        for (int k = 0; k < D * D; ++k) {
+          deps.m[k] = (R)0.5 * (Lp.m[k] + LT.m[k]) * dt;
+          W.m[k] = (R)0.5 * (Lp.m[k] - LT.m[k]) * dt;
+        }
+        const M We = matmul(W, eps_prev), eW = matmul(eps_prev, W);
+        for (int k = 0; k < D * D; ++k) deps.m[k] += -We.m[k] + eW.m[k];
+      }
+      const M3 sref = mt.stress_ref[p];
+      const R pref = -(R)(1.0 / 3.0) * trace(sref);
+      M3 s_ref = sref;
+      for (int i = 0; i < 3; ++i) s_ref(i, i) += pref;
+      M etr;
+      for (int k = 0; k < D * D; ++k) etr.m[k] = eps_prev.m[k] + deps.m[k];
+      const R evt = -trace(etr);
+      M3 dev = M3::zero();
+      for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) dev(i, j) = etr(i, j);
+      for (int i = 0; i < 3; ++i) dev(i, i) += (R)((1.0 / 3.0) * (double)evt);
+      if (D < 3) dev(2, 2) = R(0);
+      const R vspec = volumes[p] / mt.Vs;
+      const R p_prev = (R)(-(double)trace(stresses[p]) / 3.0);
+      const R v_kap = vspec / mt.kap;
+      const R dev_tr = -trace(deps);
+      const R ptr = (R)((double)p_prev / (1.0 - (double)(v_kap * dev_tr)));
+      M3 st;
+      for (int k = 0; k < 9; ++k) st.m[k] = s_ref.m[k] + (R)(2. * (double)G) * dev.m[k];
+      const R q_tr = (R)std::sqrt(3 * 0.5 * (double)trace_mmt(st));
+      const R pc_prev = mt.pc[p];
+      const R ps_tr = (R)(0.5 * (double)(pc_prev + Pt));
+      const R f_tr = mcc_yield(ptr, Pt, ps_tr, q_tr, Mm);
+      if (f_tr <= (R)0.0) {
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) stresses[p](i, j) = st(i, j) - (i == j ? ptr : R(0));
+        if (mt.do_update_history) {
+          M3 e = M3::zero();
+          for (int i = 0; i < D; ++i)
+            for (int j = 0; j < D; ++j) e(i, j) = etr(i, j);
+          mt.eps_e[p] = e;
+        }
+        continue;
+      }
+      const R epv_prev = mt.alpha[p];
+      double sol0 = 0.0, sol1 = epv_prev;
+      double p_next = 0.0, q_next = 0.0, ps_next = 0.0, tol = 1e-2, conv = 1e10, sfs = 1.0;
+      R devn = 0;
+      const double vlt = (double)vspec / (double)(mt.lam - mt.kap);
+      const double MM = (double)(Mm * Mm), G6 = (double)(6 * G), G12 = (double)(-12 * G);
+      int counter = 0;
+      do {
+        const double mult = sol0, epv = sol1;
+        const R dpv = (R)(epv - (double)epv_prev);
+        devn = dev_tr - dpv;
+        p_next = (double)p_prev / (1.0 - (double)(v_kap * devn));
+        q_next = (MM / (MM + G6 * mult)) * (double)q_tr;
+        ps_next = 0.5 * ((double)pc_prev * (1.0 + vlt * (double)dpv) + (double)Pt);
+        sfs = MM / (MM + 6.0 * (double)G * mult);
+        const double r0 = mcc_yield((R)p_next, Pt, (R)ps_next, (R)q_next, Mm);
+        const double r1 = epv - (double)epv_prev + 2.0 * mult * (ps_next - p_next - (double)Pt);
+        const double Hh = 0.5 * vlt * ((double)pc_prev / std::pow(1.0 - vlt * (double)dpv, 2));
+        const double pover = ps_next - p_next - (double)Pt;
+        const double d00 = (G12 / (MM + G6 * mult)) * (q_next / (double)Mm) * (q_next / (double)Mm);
+        const double d01 = (2.0 * pover) * ((double)K + Hh) - 2.0 * ps_next * Hh;
+        const double d10 = 2.0 * pover;
+        const double d11 = 1.0 + (2.0 * mult) * ((double)K + Hh);
+        const double invdet = 1.0 / (d00 * d11 - d10 * d01);
+        const double i00 = d11 * invdet, i10 = -d10 * invdet, i01 = -d01 * invdet, i11 = d00 * invdet;
+        sol0 = sol0 - (i00 * r0 + i01 * r1);
+        sol1 = sol1 - (i10 * r0 + i11 * r1);
+        conv = std::sqrt(r0 * r0 + r1 * r1);
+        if (counter > 1000) tol *= 10;
+        counter += 1;
+      } while (std::fabs(conv) > tol && counter < 4096);
+      M3 sn;
+      for (int k = 0; k < 9; ++k) sn.m[k] = (R)sfs * st.m[k];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) stresses[p](i, j) = (i == j ? (R)((double)sn(i, j) - p_next) : sn(i, j));
+      if (mt.do_update_history) {
+        const R vterm = (R)((1. / 3.) * (double)(trace(eps_prev) + devn));
+        const R g2 = (R)(2.0 * (double)G);
+        M3 e = M3::zero();
+        for (int i = 0; i < D; ++i)
+          for (int j = 0; j < D; ++j) e(i, j) = (sn(i, j) - s_ref(i, j)) / g2 - (i == j ? vterm : R(0));
+        mt.eps_e[p] = e;
+        mt.alpha[p] = (R)sol1;
+        mt.pc[p] = (R)(2.0 * ps_next - (double)Pt);
+      }
+    }
+  }
   void stress_update_material(int mat_id) {
     Material<R> &mt = materials[mat_id];
     switch (mt.kind) {
@@ -966,6 +1377,9 @@ template <class R, int D> struct Sim {
       case MAT_NEWTON_FLUID: update_newton_fluid(mt, mat_id); break;
       case MAT_LOCAL_RHEO: update_local_rheo(mt, mat_id); break;
       case MAT_DP: update_dp(mt, mat_id); break;
+      case MAT_VON_MISES: update_von_mises(mt, mat_id); break;
+      case MAT_MOHR_COULOMB: update_mohr_coulomb(mt, mat_id); break;
+      case MAT_MCC: update_mcc(mt, mat_id); break;
     }
   }
   void stress_update() {  // solver.cpp:110-121
@@ -1196,6 +1610,9 @@ template <class R, int D> struct Sim {
     for (int m = 0; m < (int)materials.size(); ++m) {
       calculate_initial_masses(m, materials[m].density);
       if (materials[m].kind == MAT_DP) dp_initialize(materials[m]);
+      if (materials[m].kind == MAT_VON_MISES || materials[m].kind == MAT_MOHR_COULOMB)
+        plastic_initialize(materials[m]);
+      if (materials[m].kind == MAT_MCC) mcc_initialize(materials[m]);
     }
   }
   void solve() {

@@ -130,8 +130,24 @@ template <class R, int D> void bind_sim(py::module &m, const std::string &name) 
                mt.apex_guard = getp(p, "apex_guard", 1) != 0;
                mt.do_update_history = getp(p, "do_update_history", 1) != 0;
                mt.is_velgrad_strain_increment = getp(p, "is_velgrad_strain_increment", 0) != 0;
+             } else if (kind == "VonMises") {
+               mt = Material<R>::von_mises((R)getp(p, "density", 0), (R)getp(p, "E", 0), (R)getp(p, "pois", 0),
+                                           (R)getp(p, "yield_stress", 0), (R)getp(p, "H", 0));
+             } else if (kind == "MohrCoulomb") {
+               mt = Material<R>::mohr_coulomb((R)getp(p, "density", 0), (R)getp(p, "E", 0), (R)getp(p, "pois", 0),
+                                              (R)getp(p, "cohesion", 0), (R)getp(p, "friction_angle", 0),
+                                              (R)getp(p, "dilatancy_angle", 0), (R)getp(p, "H", 0));
+             } else if (kind == "ModifiedCamClay") {
+               mt = Material<R>::modified_cam_clay((R)getp(p, "density", 0), (R)getp(p, "E", 0),
+                                                   (R)getp(p, "pois", 0), (R)getp(p, "M", 0), (R)getp(p, "lam", 0),
+                                                   (R)getp(p, "kap", 0), (R)getp(p, "Vs", 0), (R)getp(p, "R", 0),
+                                                   (R)getp(p, "Pt", 0), (R)getp(p, "beta", 0));
              } else
                throw std::invalid_argument("unknown material " + kind);
+             if (kind != "LinearElastic" && kind != "NewtonFluid" && kind != "LocalGranularRheology") {
+               mt.do_update_history = getp(p, "do_update_history", 1) != 0;
+               mt.is_velgrad_strain_increment = getp(p, "is_velgrad_strain_increment", 0) != 0;
+             }
              s.materials.push_back(mt);
            })
       .def("add_bc",
@@ -254,6 +270,7 @@ template <class R, int D> void bind_sim(py::module &m, const std::string &name) 
              if (k == "eps_e") return from_mats<R, 3>(mt.eps_e);
              if (k == "stress_ref") return from_mats<R, 3>(mt.stress_ref);
              if (k == "pc") return from_scalars(mt.pc);
+             if (k == "alpha") return from_scalars(mt.alpha);
              if (k == "solid_volume_fraction") return from_scalars(mt.svf);
              if (k == "bulk_modulus") return py::float_((double)mt.bulk_modulus);
              if (k == "shear_modulus") return py::float_((double)mt.shear_modulus);
@@ -268,6 +285,8 @@ template <class R, int D> void bind_sim(py::module &m, const std::string &name) 
              auto &mt = s.materials.at(mat_id);
              if (k == "eps_e") mt.eps_e = to_mats<R, 3>(a);
              else if (k == "stress_ref") mt.stress_ref = to_mats<R, 3>(a);
+             else if (k == "pc") mt.pc = to_scalars<R>(a);
+             else if (k == "alpha") mt.alpha = to_scalars<R>(a);
              // scalar parameters (finite-difference checks of parameter gradients)
              else if (k == "bulk_modulus") mt.bulk_modulus = (R)a.cast<double>();
              else if (k == "shear_modulus") mt.shear_modulus = (R)a.cast<double>();

@@ -19,6 +19,7 @@ NOALLOC_SCRATCH_BYTES = 64 << 20  # PMPM_NOALLOC_SCRATCH_BYTES
 
 SF = {"linear": 0, "quadratic": 1, "cubic": 2}
 MAT_LINEAR_ELASTIC, MAT_NEWTON_FLUID, MAT_LOCAL_RHEO, MAT_DRUCKER_PRAGER = 1, 2, 3, 4
+MAT_VON_MISES, MAT_MOHR_COULOMB, MAT_MODIFIED_CAM_CLAY = 5, 6, 7
 BC_GRAVITY, BC_NODE_DOMAIN, BC_RIGID_SDF, BC_BODY_FORCE, BC_PLANAR_DOMAIN = 1, 2, 4, 5, 6
 MATF_UPDATE_HISTORY, MATF_VELGRAD_IS_INCREMENT, MATF_APEX_GUARD, MATF_HAS_STRESS_REF = 1, 2, 4, 8
 PF_RIGID, PF_ACTIVE = 1, 2
@@ -36,7 +37,7 @@ class Grid(C.Structure):
 class Particles(C.Structure):
     _fields_ = [("n", C.c_int32), ("x", vp), ("v", vp), ("stress", vp), ("F", vp), ("L", vp),
                 ("mass", vp), ("volume", vp), ("volume0", vp), ("color", vp), ("flags", vp), ("id", vp),
-                ("eps_e", vp), ("stress_ref", vp)]
+                ("eps_e", vp), ("stress_ref", vp), ("hist", vp)]
 
 
 def node_vel_stride(dim):

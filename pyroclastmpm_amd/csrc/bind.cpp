@@ -167,6 +167,14 @@ template <int D> void bind_dim(py::module &m, py::module &root) {
   pc.def_property("is_active", [](P &s) { return arr(s.is_active()); },
                   [](P &s, py::object v) { s.set_is_active(to_ints(v)); });
   pc.def_property_readonly("is_rigid", [](P &s) { return arr(s.is_rigid()); });
+  // the plastic history the post-G2P materials keep per particle (the
+  // reference keeps it in the material: acc_eps_p_gpu, alpha_gpu, pc_gpu)
+  pc.def_property_readonly("plastic_history", [](P &s) {
+    auto h = s.plastic_history();
+    py::array_t<float> a({(ssize_t)2, (ssize_t)s.num_particles});
+    if (!h.empty()) std::memcpy(a.mutable_data(), h.data(), h.size() * sizeof(float));
+    return a;
+  });
   pc.def_property("eps_e", [](P &s) { return from_mats<3>(s.eps_e()); },
                   [](P &s, py::object v) { s.set_eps_e(to_mats<3>(v)); });
   pc.def("set_spatialpartition", [](P &s, const N &nodes) { s.set_spatialpartition(nodes.grid); }, py::arg("nodes"));
@@ -236,7 +244,8 @@ template <int D> void bind_dim(py::module &m, py::module &root) {
   nc.def_readwrite("small_mass_cutoff", &N::small_mass_cutoff);
 
   // materials are dimension-free types, registered once on the root module
-  for (const char *k : {"LinearElastic", "NewtonFluid", "LocalGranularRheology", "DP_rheo"})
+  for (const char *k : {"LinearElastic", "NewtonFluid", "LocalGranularRheology", "DP_rheo", "VonMises", "MohrCoulomb",
+                        "ModifiedCamClay"})
     m.attr(k) = root.attr(k);
   // Friction-angle / cohesion form of DP_rheo (outer Mohr-Coulomb cone):
   // M = 6 sin(phi) / (3 - sin(phi)), Pt = c cot(phi). Not in the reference.
@@ -408,6 +417,16 @@ template <class M, int D> void add_update(py::class_<M> &c) {
   c.def("stress_update", [](M &s, ParticlesContainer<D> p, int id) { return standalone_update<D>(s, p, id); },
         py::arg("particles"), py::arg("mat_id") = 0);
 }
+// stress_update of a material whose history (elastic strain, plastic
+// scalars) travels with the particles
+template <class M, int D> void add_plastic_update(py::class_<M> &c) {
+  c.def("stress_update",
+        [](M &s, ParticlesContainer<D> p, int id) {
+          p.ensure_plastic_history();
+          return standalone_update<D>(s, p, id);
+        },
+        py::arg("particles"), py::arg("mat_id") = 0);
+}
 template <int D> void add_dp_methods(py::class_<DP_rheo> &dp) {
   using P = ParticlesContainer<D>;
   dp.def("initialize",
@@ -508,6 +527,59 @@ void bind_materials(py::module &m) {
   add_update<LocalGranularRheology, 1>(lr); add_update<LocalGranularRheology, 2>(lr);
   add_update<LocalGranularRheology, 3>(lr);
   add_dp_methods<1>(dp); add_dp_methods<2>(dp); add_dp_methods<3>(dp);
+  // vonmises_bind.cpp:15-45
+  py::class_<VonMises> vm(m, "VonMises", py::dynamic_attr());
+  vm.def(py::init<float, float, float, float, float>(), py::arg("density"), py::arg("E"), py::arg("pois"),
+         py::arg("yield_stress"), py::arg("H"))
+      .def_readwrite("E", &VonMises::E)
+      .def_readwrite("pois", &VonMises::pois)
+      .def_readwrite("shear_modulus", &VonMises::shear_modulus)
+      .def_readwrite("bulk_modulus", &VonMises::bulk_modulus)
+      .def_readwrite("density", &VonMises::density)
+      .def_readwrite("yield_stress", &VonMises::yield_stress)
+      .def_readwrite("H", &VonMises::H)
+      .def_readwrite("do_update_history", &VonMises::do_update_history)
+      .def_readwrite("is_velgrad_strain_increment", &VonMises::is_velgrad_strain_increment);
+  // mohr_coulomb_bind.cpp:16-52
+  py::class_<MohrCoulomb> mc(m, "MohrCoulomb", py::dynamic_attr());
+  mc.def(py::init<float, float, float, float, float, float, float>(), py::arg("density"), py::arg("E"),
+         py::arg("pois") = 0.f, py::arg("cohesion") = 0.f, py::arg("friction_angle") = 0.f,
+         py::arg("dilatancy_angle") = 0.f, py::arg("H") = 0.f)
+      .def_readwrite("E", &MohrCoulomb::E)
+      .def_readwrite("pois", &MohrCoulomb::pois)
+      .def_readwrite("shear_modulus", &MohrCoulomb::shear_modulus)
+      .def_readwrite("lame_modulus", &MohrCoulomb::lame_modulus)
+      .def_readwrite("bulk_modulus", &MohrCoulomb::bulk_modulus)
+      .def_readwrite("density", &MohrCoulomb::density)
+      .def_readwrite("cohesion", &MohrCoulomb::cohesion)
+      .def_readwrite("friction_angle", &MohrCoulomb::friction_angle)
+      .def_readwrite("dilatancy_angle", &MohrCoulomb::dilatancy_angle)
+      .def_readwrite("H", &MohrCoulomb::H)
+      .def_readwrite("do_update_history", &MohrCoulomb::do_update_history)
+      .def_readwrite("is_velgrad_strain_increment", &MohrCoulomb::is_velgrad_strain_increment);
+  // modified_camclay_bind.cpp:15-49
+  py::class_<ModifiedCamClay> mcc(m, "ModifiedCamClay", py::dynamic_attr());
+  mcc.def(py::init<float, float, float, float, float, float, float, float, float, float>(), py::arg("density"),
+          py::arg("E"), py::arg("pois"), py::arg("M"), py::arg("lam"), py::arg("kap"), py::arg("Vs"), py::arg("R"),
+          py::arg("Pt"), py::arg("beta"))
+      .def_readwrite("E", &ModifiedCamClay::E)
+      .def_readwrite("pois", &ModifiedCamClay::pois)
+      .def_readwrite("shear_modulus", &ModifiedCamClay::shear_modulus)
+      .def_readwrite("bulk_modulus", &ModifiedCamClay::bulk_modulus)
+      .def_readwrite("density", &ModifiedCamClay::density)
+      .def_readwrite("M", &ModifiedCamClay::M)
+      .def_readwrite("lam", &ModifiedCamClay::lam)
+      .def_readwrite("kap", &ModifiedCamClay::kap)
+      .def_readwrite("Vs", &ModifiedCamClay::Vs)
+      .def_readwrite("R", &ModifiedCamClay::R)
+      .def_readwrite("Pt", &ModifiedCamClay::Pt)
+      .def_readwrite("beta", &ModifiedCamClay::beta)
+      .def_readwrite("do_update_history", &ModifiedCamClay::do_update_history)
+      .def_readwrite("is_velgrad_strain_increment", &ModifiedCamClay::is_velgrad_strain_increment);
+  add_plastic_update<VonMises, 3>(vm);
+  add_plastic_update<MohrCoulomb, 1>(mc); add_plastic_update<MohrCoulomb, 2>(mc); add_plastic_update<MohrCoulomb, 3>(mc);
+  add_plastic_update<ModifiedCamClay, 1>(mcc); add_plastic_update<ModifiedCamClay, 2>(mcc);
+  add_plastic_update<ModifiedCamClay, 3>(mcc);
 }
 
 }  // namespace

@@ -443,6 +443,516 @@ __device__ inline void rm_drucker_prager(const pmpm_material &m, float dt, const
   }
 }
 
+// ---------------------------------------------------------------------------
+// Return maps of the materials the step runs in a pass after G2P (their
+// history lives in pmpm_particles.hist): von Mises, Mohr-Coulomb and the
+// modified Cam-Clay. eps is the D x D elastic strain (symmetric storage,
+// components outside D x D zero), h0/h1 the particle's history scalars.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline bool is_ext_material(int kind) {
+  return kind == PMPM_MAT_VON_MISES || kind == PMPM_MAT_MOHR_COULOMB || kind == PMPM_MAT_MODIFIED_CAM_CLAY;
+}
+// Newton iterations of the plastic returns below stop here at the latest; the
+// reference's loops have no cap (see each).
+constexpr int kMaxReturnIters = 100;
+
+// Strain increment of the step: L itself (servo-control form) or
+// 0.5 (L + L^T) dt, both over the D x D block.
+template <int D> __device__ inline void strain_increment(const pmpm_material &m, float dt, const float *L, float de[3][3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const bool in = i < D && j < D;
+      de[i][j] = !in ? 0.f
+                     : ((m.flags & PMPM_MATF_VELGRAD_IS_INCREMENT) ? L[3 * i + j]
+                                                                    : 0.5f * (L[3 * i + j] + L[3 * j + i]) * dt);
+    }
+}
+__device__ inline void sym_to_mat(const Sym &s, float m[3][3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) m[i][j] = sym_at(s, i, j);
+}
+__device__ inline Sym mat_to_sym(const float m[3][3]) { return Sym{m[0][0], m[1][1], m[2][2], m[0][1], m[0][2], m[1][2]}; }
+// (M M^T).trace() of a 3 x 3 matrix, row by row as a coefficient product sums it
+__device__ inline float frob2_rows(const float m[3][3]) {
+  float t = 0.f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float r = __fadd_rn(__fadd_rn(__fmul_rn(m[i][0], m[i][0]), __fmul_rn(m[i][1], m[i][1])),
+                              __fmul_rn(m[i][2], m[i][2]));
+    t = i == 0 ? r : __fadd_rn(t, r);
+  }
+  return t;
+}
+
+// vonmises_inline.h:41-140 (3D): elastic trial on the elastic strain,
+// J2 yield with linear isotropic hardening sigma_y = yield + H acc, Newton on
+// the plastic multiplier in double. h0 = accumulated plastic strain.
+// (The reference's Newton loop has no cap; here it stops after
+// kMaxReturnIters, which the linear hardening never reaches.)
+__device__ inline void rm_von_mises(const pmpm_material &m, float dt, const float *L, Sym &s, Sym &eps, float &h0) {
+  const float K = m.p[0], G = m.p[1], ys = m.p[2], H = m.p[3];
+  float de[3][3], et[3][3];
+  strain_increment<3>(m, dt, L, de);
+  float ep[3][3];
+  sym_to_mat(eps, ep);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) et[i][j] = __fadd_rn(ep[i][j], de[i][j]);
+  const float ev = __fadd_rn(__fadd_rn(et[0][0], et[1][1]), et[2][2]);
+  const float ptr = __fmul_rn(K, ev);
+  const float ev3 = (float)((1 / 3.) * (double)ev);
+  const float twoG = (float)(2. * (double)G);
+  float st[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) st[i][j] = __fmul_rn(twoG, i == j ? __fsub_rn(et[i][j], ev3) : et[i][j]);
+  const float acc_tr = h0;
+  const float sy_tr = __fadd_rn(ys, __fmul_rn(H, acc_tr));
+  const float q = (float)sqrt(1.5 * (double)frob2_rows(st));
+  const float phi = __fsub_rn(q, sy_tr);
+  const bool hist = (m.flags & PMPM_MATF_UPDATE_HISTORY) != 0;
+  if (phi <= 0.f) {
+    s = Sym{__fadd_rn(st[0][0], ptr), __fadd_rn(st[1][1], ptr), __fadd_rn(st[2][2], ptr), st[0][1], st[0][2], st[1][2]};
+    if (hist) eps = mat_to_sym(et);
+    return;
+  }
+  double dg = 0.0, acc = 0.0;
+  const double Gd = G, Hd = H, qd = q;
+  for (int it = 0; it < kMaxReturnIters; ++it) {
+    acc = (double)acc_tr + dg;
+    const double sy = (double)ys + Hd * acc;
+    const double ph = qd - 3.0 * Gd * dg - sy;
+    dg = dg - ph / (-3.0 * Gd - Hd);
+    if (!(fabs(ph) > 1e-7)) break;
+  }
+  const float f = (float)(1.0 - (3.0 * Gd * dg) / qd);
+  float sc[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) sc[i][j] = __fmul_rn(f, st[i][j]);
+  s = Sym{__fadd_rn(sc[0][0], ptr), __fadd_rn(sc[1][1], ptr), __fadd_rn(sc[2][2], ptr), sc[0][1], sc[0][2], sc[1][2]};
+  if (hist) {
+    const float g2 = (float)(2.0 * Gd);
+    float e[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) e[i][j] = __fadd_rn(__fdiv_rn(sc[i][j], g2), i == j ? ev3 : 0.f);
+    eps = mat_to_sym(e);
+    h0 = (float)acc;
+  }
+}
+
+// Eigen-decomposition of a symmetric 3 x 3 matrix by cyclic Jacobi rotations
+// in double: w descending, column k of V the eigenvector of w[k].
+__host__ __device__ inline void sym_eig3_desc(const double A[3][3], double w[3], double V[3][3]) {
+  double a[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      a[i][j] = A[i][j];
+      V[i][j] = i == j ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 12; ++sweep) {
+    const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+    const double dia = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+    if (off <= 1e-32 * dia || off == 0.0) break;
+    for (int p = 0; p < 2; ++p)
+      for (int q = p + 1; q < 3; ++q) {
+        if (a[p][q] == 0.0) continue;
+        const double th = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+        const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+        for (int k = 0; k < 3; ++k) {  // A <- A J (columns p, q)
+          const double akp = a[k][p], akq = a[k][q];
+          a[k][p] = c * akp - sn * akq;
+          a[k][q] = sn * akp + c * akq;
+        }
+        for (int k = 0; k < 3; ++k) {  // A <- J^T A (rows p, q)
+          const double apk = a[p][k], aqk = a[q][k];
+          a[p][k] = c * apk - sn * aqk;
+          a[q][k] = sn * apk + c * aqk;
+        }
+        for (int k = 0; k < 3; ++k) {
+          const double vkp = V[k][p], vkq = V[k][q];
+          V[k][p] = c * vkp - sn * vkq;
+          V[k][q] = sn * vkp + c * vkq;
+        }
+      }
+  }
+  int o[3] = {0, 1, 2};  // descending order
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2 - i; ++j)
+      if (a[o[j]][o[j]] < a[o[j + 1]][o[j + 1]]) { const int t = o[j]; o[j] = o[j + 1]; o[j + 1] = t; }
+  double Vs[3][3];
+  for (int k = 0; k < 3; ++k) {
+    w[k] = a[o[k]][o[k]];
+    for (int i = 0; i < 3; ++i) Vs[i][k] = V[i][o[k]];
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k) V[i][k] = Vs[i][k];
+}
+
+// mohrcoulomb_inline.h:301-457 (update_mohrcoulomb; returns 34-289): elastic
+// trial in principal stresses (sigma_1 >= sigma_2 >= sigma_3, tension
+// positive), yield sigma_1 - sigma_3 + (sigma_1 + sigma_3) sin(phi)
+// - 2 c cos(phi) with c = cohesion + H acc; return to the main plane, then
+// the right edge, then the left edge (each accepted when the truncated
+// principal stresses keep their order), else the apex. Newton loops in
+// double, stopped after kMaxReturnIters at the latest (the reference's have
+// no cap). The principal frame comes from a Jacobi decomposition in double
+// (the reference's Eigen closed form in float agrees to its rounding).
+// h0 = accumulated plastic strain. With dilatancy 0 the apex return divides
+// by zero as the reference's does.
+template <int D>
+__device__ inline void rm_mohr_coulomb(const pmpm_material &m, float dt, const float *L, Sym &s, Sym &eps, float &h0) {
+  const float K = m.p[0], G = m.p[1], coh = m.p[2], fric = m.p[3], dil = m.p[4], H = m.p[5];
+  const bool hist = (m.flags & PMPM_MATF_UPDATE_HISTORY) != 0;
+  float de[3][3], ep[3][3], et[3][3];
+  strain_increment<D>(m, dt, L, de);
+  sym_to_mat(eps, ep);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) et[i][j] = (i < D && j < D) ? __fadd_rn(ep[i][j], de[i][j]) : 0.f;
+  float ev = et[0][0];
+  if (D > 1) ev = __fadd_rn(ev, et[1][1]);
+  if (D > 2) ev = __fadd_rn(ev, et[2][2]);
+  const float ptr = __fmul_rn(K, ev);
+  const float ev3 = (float)((1 / 3.) * (double)ev);
+  const float twoG = (float)(2. * (double)G);
+  float sig[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const float sv = (i < D && j < D) ? __fmul_rn(twoG, i == j ? __fsub_rn(et[i][j], ev3) : et[i][j]) : 0.f;
+      sig[i][j] = i == j ? __fadd_rn(sv, ptr) : sv;
+    }
+  const float acc_tr = h0;
+  const float coh_tr = __fadd_rn(coh, __fmul_rn(H, acc_tr));
+  double A[3][3], w[3], V[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) A[i][j] = sig[i][j];
+  sym_eig3_desc(A, w, V);
+  const float ps[3] = {(float)w[0], (float)w[1], (float)w[2]};
+  const float sfa = sinf(fric), cfa = cosf(fric);
+  const float phi_tr = (float)((double)__fadd_rn(__fsub_rn(ps[0], ps[2]), __fmul_rn(__fadd_rn(ps[0], ps[2]), sfa)) -
+                               2. * (double)coh_tr * (double)cfa);
+  auto store_elastic_eps = [&]() {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) ep[i][j] = (i < D && j < D) ? et[i][j] : 0.f;
+    eps = mat_to_sym(ep);
+  };
+  if (phi_tr <= 0.f) {
+    s = mat_to_sym(sig);
+    if (hist) store_elastic_eps();
+    return;
+  }
+  const float sda = sinf(dil);
+  const double Kd = K, Gd = G, Hd = H, sfad = sfa, cfad = cfa, sdad = sda;
+  const float a = (float)(4. * Gd * (1. + (1. / 3.) * sfad * sdad) +
+                          (double)__fmul_rn(__fmul_rn(__fmul_rn(4.f, K), sfa), sda));
+  const double ad = a;
+  // principal stresses back to the frame, the elastic strain of the result
+  auto finish = [&](const float pc[3], double acc_next) {
+    float out[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) t += V[i][k] * (double)pc[k] * V[j][k];
+        out[i][j] = (float)t;
+      }
+    s = mat_to_sym(out);
+    if (hist) {
+      const float pn = (float)((double)__fadd_rn(__fadd_rn(out[0][0], out[1][1]), out[2][2]) / 3.0);
+      const float i2g = (float)(1. / (2. * Gd)), i3k = (float)(1. / (3. * Kd));
+      float e[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          e[i][j] = (i < D && j < D)
+                        ? __fadd_rn(__fmul_rn(i == j ? __fsub_rn(out[i][j], pn) : out[i][j], i2g),
+                                    i == j ? __fmul_rn(pn, i3k) : 0.f)
+                        : 0.f;
+      eps = mat_to_sym(e);
+      h0 = (float)acc_next;
+    }
+  };
+  auto ordered = [](const float pc[3]) {
+    return truncf(pc[0]) >= truncf(pc[1]) && truncf(pc[1]) >= truncf(pc[2]);
+  };
+  // main plane (mohrcoulomb_inline.h:34-98)
+  {
+    double dg = 0.0, ph = phi_tr, acc_next = acc_tr;
+    const double d = -ad - 4 * Hd * cfad * cfad;
+    for (int it = 0; it < kMaxReturnIters; ++it) {
+      dg = dg - ph / d;
+      acc_next = (double)acc_tr + 2 * cfad * dg;
+      const double ca = (double)coh + Hd * acc_next;
+      ph = (double)ps[0] - (double)ps[2] + ((double)ps[0] + (double)ps[2]) * sfad - 2. * ca * cfad - ad * dg;
+      if (!(fabs(ph) > 1.e-6)) break;
+    }
+    const float pc[3] = {(float)((double)ps[0] - dg * (2. * Gd * (1. + (1. / 3.) * sdad) + 2. * Kd * sdad)),
+                         (float)((double)ps[1] + dg * ((4. / 3.) * Gd - 2. * Kd) * sdad),
+                         (float)((double)ps[2] + dg * (2. * Gd * (1. - (1. / 3.) * sdad) - 2. * Kd * sdad))};
+    if (ordered(pc)) { finish(pc, acc_next); return; }
+  }
+  // edges (mohrcoulomb_inline.h:100-229): right, then left
+  for (int edge = 0; edge < 2; ++edge) {
+    const bool right = edge == 0;
+    const float s1 = __fadd_rn(__fsub_rn(ps[0], ps[2]), __fmul_rn(__fadd_rn(ps[0], ps[2]), sfa));
+    const float s2 = right ? __fadd_rn(__fsub_rn(ps[0], ps[1]), __fmul_rn(__fadd_rn(ps[0], ps[1]), sfa))
+                           : __fadd_rn(__fsub_rn(ps[1], ps[2]), __fmul_rn(__fadd_rn(ps[1], ps[2]), sfa));
+    double ph0 = (double)s1 - 2. * (double)coh_tr * cfad, ph1 = (double)s2 - 2. * (double)coh_tr * cfad;
+    const double b = right ? 2 * Gd * (1. + sfad + sdad - (1. / 3.) * sfad * sdad) + 4. * Kd * sfad * sdad
+                           : 2 * Gd * (1. - sfad - sdad - (1 / 3.) * sfad * sdad) + 4. * Kd * sfad * sdad;
+    double g0 = 0.0, g1 = 0.0, acc_next = acc_tr;
+    for (int it = 0; it < kMaxReturnIters; ++it) {
+      const double d00 = -ad - 4. * Hd * cfad * cfad;
+      const double d01 = -b - (double)__fmul_rn(__fmul_rn(__fmul_rn(4.f, H), cfa), cfa);
+      // dgamma -= d^-1 Phi, the 2 x 2 inverse formed first
+      const double invdet = 1.0 / (d00 * d00 - d01 * d01);
+      const double i00 = d00 * invdet, i01 = -d01 * invdet;
+      const double n0 = g0 - (i00 * ph0 + i01 * ph1), n1 = g1 - (i01 * ph0 + i00 * ph1);
+      g0 = n0;
+      g1 = n1;
+      acc_next = (double)acc_tr + 2 * cfad * (g0 + g1);
+      const double ca = (double)coh + Hd * acc_next;
+      ph0 = (double)s1 - 2. * ca * cfad - ad * g0 - b * g1;
+      ph1 = (double)s2 - 2. * ca * cfad - ad * g1 - b * g0;
+      if (!(fabs(ph0) + fabs(ph1) > 1.e-7)) break;
+    }
+    const double c1 = 2. * Gd * (1. + (1. / 3.) * sdad) + 2. * Kd * sdad, c2 = ((4. / 3.) * Gd - 2. * Kd) * sdad,
+                 c3 = 2. * Gd * (1. - (1. / 3.) * sdad) - 2. * Kd * sdad;
+    float pc[3];
+    if (right) {
+      pc[0] = (float)((double)ps[0] - c1 * (g0 + g1));
+      pc[1] = (float)((double)ps[1] + c2 * g0 + c3 * g1);
+      pc[2] = (float)((double)ps[2] + c3 * g0 + c2 * g1);
+    } else {
+      pc[0] = (float)((double)ps[0] - c1 * g0 + c2 * g1);
+      pc[1] = (float)((double)ps[1] + c2 * g0 - c1 * g1);
+      pc[2] = (float)((double)ps[2] + c3 * (g0 + g1));
+    }
+    if (ordered(pc)) { finish(pc, acc_next); return; }
+  }
+  // apex (mohrcoulomb_inline.h:231-270)
+  double dv = 0.0, ph = phi_tr, pn = 0.0, acc_next = acc_tr;
+  const double al = cfad / sdad, cotf = cfad / sfad;
+  for (int it = 0; it < kMaxReturnIters; ++it) {
+    acc_next = (double)acc_tr + al * dv;
+    const double ca = (double)coh + Hd * acc_next;
+    pn = (double)ptr - Kd * dv;
+    ph = ca * cotf - pn;
+    dv = dv - ph / ((Hd * cfad * cotf) / sdad + Kd);
+    if (!(fabs(ph) > 1.e-6)) break;
+  }
+  const float pf = (float)pn;
+  s = Sym{pf, pf, pf, 0.f, 0.f, 0.f};
+  if (hist) {
+    const float e = (float)(pn * (1. / (3. * Kd)));
+    eps = Sym{e, D > 1 ? e : 0.f, D > 2 ? e : 0.f, 0.f, 0.f, 0.f};
+    h0 = (float)acc_next;
+  }
+}
+
+// modifiedcamclay_inline.h:269-413 (update_modifiedcamclay_nonlinear, the
+// update modifiedcamclay.cpp:147-176 runs): co-rotated elastic-strain trial
+// with pressure-dependent stiffness p = p_prev / (1 - v/kappa de_v), the
+// ellipse (p_s - p - Pt)^2 + (q/M)^2 - p_s^2 with back stress
+// p_s = (pc + Pt)/2, and a 2 x 2 Newton on (plastic multiplier, volumetric
+// plastic strain) in double whose tolerance grows tenfold per iteration
+// after the thousandth, as the reference's does. Compression positive in p.
+// h0 = volumetric plastic strain, h1 = preconsolidation pressure.
+__device__ inline float mcc_yield(float p, float Pt, float ps, float q, float M) {
+  const float a = __fsub_rn(__fsub_rn(ps, p), Pt), b = __fdiv_rn(q, M);
+  return __fsub_rn(__fadd_rn(__fmul_rn(a, a), __fmul_rn(b, b)), __fmul_rn(ps, ps));
+}
+template <int D>
+__device__ inline void rm_modified_cam_clay(const pmpm_material &m, float dt, const float *L, float vol,
+                                            const Sym &sref, Sym &s, Sym &eps, float &h0, float &h1) {
+  const float K = m.p[0], G = m.p[1], M = m.p[2], lam = m.p[3], kap = m.p[4], Pt = m.p[5], Vs = m.p[6];
+  const bool hist = (m.flags & PMPM_MATF_UPDATE_HISTORY) != 0;
+  float de[3][3], ep[3][3];
+  sym_to_mat(eps, ep);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) de[i][j] = 0.f;
+  if (m.flags & PMPM_MATF_VELGRAD_IS_INCREMENT) {
+    strain_increment<D>(m, dt, L, de);
+  } else {
+    float W[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const bool in = i < D && j < D;
+        de[i][j] = in ? 0.5f * (L[3 * i + j] + L[3 * j + i]) * dt : 0.f;
+        W[i][j] = in ? 0.5f * (L[3 * i + j] - L[3 * j + i]) * dt : 0.f;
+      }
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        float we = 0.f, ew = 0.f;
+#pragma unroll
+        for (int k = 0; k < D; ++k) { we += W[i][k] * ep[k][j]; ew += ep[i][k] * W[k][j]; }
+        de[i][j] += -we + ew;
+      }
+  }
+  const float tr_ref = __fadd_rn(__fadd_rn(sref.xx, sref.yy), sref.zz);
+  const float pref = __fmul_rn(-(float)(1.0 / 3.0), tr_ref);
+  float sr[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) sr[i][j] = i == j ? __fadd_rn(sym_at(sref, i, j), pref) : sym_at(sref, i, j);
+  float et[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) et[i][j] = (i < D && j < D) ? __fadd_rn(ep[i][j], de[i][j]) : 0.f;
+  float tre = et[0][0];
+  if (D > 1) tre = __fadd_rn(tre, et[1][1]);
+  if (D > 2) tre = __fadd_rn(tre, et[2][2]);
+  const float evt = -tre;  // compression positive
+  const float ev3 = (float)((1.0 / 3.0) * (double)evt);
+  float dev[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) dev[i][j] = i == j ? __fadd_rn(et[i][j], ev3) : et[i][j];
+  if (D < 3) dev[2][2] = 0.f;
+  const float vspec = __fdiv_rn(vol, Vs);
+  const float p_prev = (float)(-(double)__fadd_rn(__fadd_rn(s.xx, s.yy), s.zz) / 3.0);
+  const float v_kap = __fdiv_rn(vspec, kap);
+  float trd = de[0][0];
+  if (D > 1) trd = __fadd_rn(trd, de[1][1]);
+  if (D > 2) trd = __fadd_rn(trd, de[2][2]);
+  const float dev_tr = -trd;
+  const float ptr = (float)((double)p_prev / (1.0 - (double)__fmul_rn(v_kap, dev_tr)));
+  const float twoG = (float)(2. * (double)G);
+  float st[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) st[i][j] = __fadd_rn(sr[i][j], __fmul_rn(twoG, dev[i][j]));
+  const float q_tr = (float)sqrt(1.5 * (double)frob2_rows(st));
+  const float pc_prev = h1;
+  const float ps_tr = (float)(0.5 * (double)__fadd_rn(pc_prev, Pt));
+  const float f_tr = mcc_yield(ptr, Pt, ps_tr, q_tr, M);
+  if (f_tr <= 0.f) {
+    float o[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) o[i][j] = i == j ? __fsub_rn(st[i][j], ptr) : st[i][j];
+    s = mat_to_sym(o);
+    if (hist) eps = mat_to_sym(et);
+    return;
+  }
+  const float epv_prev = h0;
+  double x0 = 0.0, x1 = epv_prev;  // plastic multiplier, volumetric plastic strain
+  double p_next = 0.0, q_next = 0.0, ps_next = 0.0, tol = 1e-2, sfac_s = 1.0;
+  float devn = 0.f;
+  const double vlt = (double)vspec / (double)__fsub_rn(lam, kap);
+  const double Md = M, Gd = G, Kd = K;
+  // M*M and 6*G are float products in the reference's expressions
+  const double MM = (double)__fmul_rn(M, M), G6 = (double)__fmul_rn(6.f, G), G12 = (double)__fmul_rn(-12.f, G);
+  for (int counter = 0; counter < 4096; ++counter) {
+    const double mult = x0, epv = x1;
+    const float dpv = (float)(epv - (double)epv_prev);
+    devn = __fsub_rn(dev_tr, dpv);
+    p_next = (double)p_prev / (1.0 - (double)__fmul_rn(v_kap, devn));
+    q_next = (MM / (MM + G6 * mult)) * (double)q_tr;
+    sfac_s = MM / (MM + 6.0 * Gd * mult);  // the deviator's factor (6.0 * G in double there)
+    ps_next = 0.5 * ((double)pc_prev * (1.0 + vlt * (double)dpv) + (double)Pt);
+    const double r0 = mcc_yield((float)p_next, Pt, (float)ps_next, (float)q_next, M);
+    const double r1 = epv - (double)epv_prev + 2.0 * mult * (ps_next - p_next - (double)Pt);
+    const double Hh = 0.5 * vlt * ((double)pc_prev / pow(1.0 - vlt * (double)dpv, 2));
+    const double pover = ps_next - p_next - (double)Pt;
+    const double d00 = (G12 / (MM + G6 * mult)) * (q_next / Md) * (q_next / Md);
+    const double d01 = (2.0 * pover) * (Kd + Hh) - 2.0 * ps_next * Hh;
+    const double d10 = 2.0 * pover;
+    const double d11 = 1.0 + (2.0 * mult) * (Kd + Hh);
+    // Solution -= d^-1 R with the 2 x 2 inverse formed first (cofactors over
+    // the determinant)
+    const double invdet = 1.0 / (d00 * d11 - d10 * d01);
+    const double i00 = d11 * invdet, i10 = -d10 * invdet, i01 = -d01 * invdet, i11 = d00 * invdet;
+    x0 = x0 - (i00 * r0 + i01 * r1);
+    x1 = x1 - (i10 * r0 + i11 * r1);
+    const double conv = sqrt(r0 * r0 + r1 * r1);
+    if (counter > 1000) tol *= 10;
+    if (!(fabs(conv) > tol)) break;
+  }
+  // s_next from the last iterate's scale (the reference keeps it from inside
+  // the loop), sigma = s - p I
+  const float sf = (float)sfac_s;
+  float sn[3][3], o[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      sn[i][j] = __fmul_rn(sf, st[i][j]);
+      o[i][j] = i == j ? (float)((double)sn[i][j] - p_next) : sn[i][j];
+    }
+  s = mat_to_sym(o);
+  if (hist) {
+    float tre0 = ep[0][0];
+    if (D > 1) tre0 = __fadd_rn(tre0, ep[1][1]);
+    if (D > 2) tre0 = __fadd_rn(tre0, ep[2][2]);
+    const float vterm = (float)((1. / 3.) * (double)__fadd_rn(tre0, devn));
+    const float g2 = (float)(2.0 * Gd);
+    float e[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        e[i][j] = (i < D && j < D) ? __fsub_rn(__fdiv_rn(__fsub_rn(sn[i][j], sr[i][j]), g2), i == j ? vterm : 0.f)
+                                   : 0.f;
+    eps = mat_to_sym(e);
+    h0 = (float)x1;
+    h1 = (float)(2.0 * ps_next - (double)Pt);
+  }
+}
+
+// The materials of the post-G2P pass, by the particle's colour.
+template <int D>
+__device__ inline void return_map_ext(const pmpm_params &prm, int color, const float *L, float vol, const Sym &sref,
+                                      Sym &s, Sym &eps, float &h0, float &h1) {
+  if (color >= prm.num_materials) return;
+  const pmpm_material &m = prm.materials[color];
+  switch (m.kind) {
+    case PMPM_MAT_VON_MISES:
+      if (D == 3) rm_von_mises(m, prm.dt, L, s, eps, h0);  // 3D only (vonmises.cpp:46-49)
+      break;
+    case PMPM_MAT_MOHR_COULOMB:
+      rm_mohr_coulomb<D>(m, prm.dt, L, s, eps, h0);
+      break;
+    case PMPM_MAT_MODIFIED_CAM_CLAY:
+      rm_modified_cam_clay<D>(m, prm.dt, L, vol, sref, s, eps, h0, h1);
+      break;
+    default:
+      break;
+  }
+}
+
 // Dispatch on the material of the particle's colour (solver.cpp:110-121: each
 // material updates only particles whose colour equals its list index).
 template <int D>

@@ -203,6 +203,7 @@ template <int D> void ParticlesContainer<D>::ensure_second_set() {
   o.mass.resize(N); o.volume.resize(N); o.volume0.resize(N); o.color.resize(N); o.flags.resize(N); o.id.resize(N);
   o.eps_e.resize(c.eps_e.size());
   o.stress_ref.resize(c.stress_ref.size());
+  o.hist.resize(c.hist.size());
 }
 
 template <int D> void ParticlesContainer<D>::ensure_history() {
@@ -211,6 +212,15 @@ template <int D> void ParticlesContainer<D>::ensure_history() {
     if (b.x.size() == 0 && &b != &buf[cur_]) continue;
     if (b.eps_e.size() != 6 * N) { b.eps_e.resize(6 * N); hip_check(hipMemset(b.eps_e.get(), 0, 24 * N), "memset"); }
     if (b.stress_ref.size() != 6 * N) { b.stress_ref.resize(6 * N); hip_check(hipMemset(b.stress_ref.get(), 0, 24 * N), "memset"); }
+  }
+}
+
+template <int D> void ParticlesContainer<D>::ensure_plastic_history() {
+  ensure_history();
+  const size_t N = num_particles;
+  for (auto &b : buf) {
+    if (b.x.size() == 0 && &b != &buf[cur_]) continue;
+    if (b.hist.size() != 2 * N) { b.hist.resize(2 * N); hip_check(hipMemset(b.hist.get(), 0, 8 * N), "memset"); }
   }
 }
 
@@ -224,6 +234,7 @@ template <int D> pmpm_particles ParticlesContainer<D>::view(int which) const {
   p.color = b.color.get(); p.flags = b.flags.get(); p.id = b.id.get();
   p.eps_e = b.eps_e.size() ? b.eps_e.get() : nullptr;
   p.stress_ref = b.stress_ref.size() ? b.stress_ref.get() : nullptr;
+  p.hist = b.hist.size() ? b.hist.get() : nullptr;
   return p;
 }
 
@@ -310,6 +321,11 @@ template <int D> std::vector<int> ParticlesContainer<D>::is_active() const {
   std::vector<int> r(f.size());
   for (size_t i = 0; i < f.size(); ++i) r[i] = (f[i] & PMPM_PF_ACTIVE) ? 1 : 0;
   return r;
+}
+template <int D> std::vector<float> ParticlesContainer<D>::plastic_history() const {
+  const size_t N = num_particles;
+  if (cur().hist.size() != 2 * N) return std::vector<float>(2 * N, 0.f);
+  return to_ref(cur().hist.download(), 2);
 }
 template <int D> std::vector<Mat3> ParticlesContainer<D>::eps_e() const {
   const size_t N = num_particles;
@@ -517,6 +533,53 @@ pmpm_material DP_rheo::device() const {
   return m;
 }
 
+VonMises::VonMises(float density, float E, float pois, float ys, float H)
+    : density(density), E(E), pois(pois), yield_stress(ys), H(H) {
+  bulk_modulus = E / ((float)3.0 * ((float)1.0 - (float)2.0 * pois));
+  shear_modulus = E / ((float)2.0 * ((float)1 + pois));
+}
+pmpm_material VonMises::device() const {
+  pmpm_material m{};
+  m.kind = PMPM_MAT_VON_MISES;
+  m.flags = (do_update_history ? PMPM_MATF_UPDATE_HISTORY : 0) |
+            (is_velgrad_strain_increment ? PMPM_MATF_VELGRAD_IS_INCREMENT : 0);
+  m.p[0] = bulk_modulus; m.p[1] = shear_modulus; m.p[2] = yield_stress; m.p[3] = H;
+  return m;
+}
+MohrCoulomb::MohrCoulomb(float density, float E, float pois, float c, float phi_deg, float psi_deg, float H)
+    : density(density), E(E), pois(pois), cohesion(c), H(H) {
+  bulk_modulus = ((float)1. / (float)3.) * (E / ((float)1. - (float)2. * pois));
+  shear_modulus = ((float)1. / (float)2.) * E / ((float)1 + pois);
+  lame_modulus = (pois * E) / (((float)1.0 + pois) * ((float)1 - (float)2.0 * pois));
+  friction_angle = phi_deg * (float)(3.14159265358979323846 / 180.);
+  dilatancy_angle = psi_deg * (float)(3.14159265358979323846 / 180.);
+}
+pmpm_material MohrCoulomb::device() const {
+  pmpm_material m{};
+  m.kind = PMPM_MAT_MOHR_COULOMB;
+  m.flags = (do_update_history ? PMPM_MATF_UPDATE_HISTORY : 0) |
+            (is_velgrad_strain_increment ? PMPM_MATF_VELGRAD_IS_INCREMENT : 0);
+  m.p[0] = bulk_modulus; m.p[1] = shear_modulus; m.p[2] = cohesion; m.p[3] = friction_angle;
+  m.p[4] = dilatancy_angle; m.p[5] = H;
+  return m;
+}
+ModifiedCamClay::ModifiedCamClay(float density, float E, float pois, float M, float lam, float kap, float Vs, float R,
+                                 float Pt, float beta)
+    : density(density), E(E), pois(pois), M(M), lam(lam), kap(kap), Vs(Vs), R(R), Pt(Pt), beta(beta) {
+  bulk_modulus = E / ((float)3.0 * ((float)1.0 - (float)2.0 * pois));
+  shear_modulus = E / ((float)2.0 * ((float)1 + pois));
+}
+pmpm_material ModifiedCamClay::device() const {
+  pmpm_material m{};
+  m.kind = PMPM_MAT_MODIFIED_CAM_CLAY;
+  // the reference stress is the stress at initialisation (modifiedcamclay.cpp:77-78)
+  m.flags = (do_update_history ? PMPM_MATF_UPDATE_HISTORY : 0) |
+            (is_velgrad_strain_increment ? PMPM_MATF_VELGRAD_IS_INCREMENT : 0) | PMPM_MATF_HAS_STRESS_REF;
+  m.p[0] = bulk_modulus; m.p[1] = shear_modulus; m.p[2] = M; m.p[3] = lam; m.p[4] = kap; m.p[5] = Pt;
+  m.p[6] = Vs;
+  return m;
+}
+
 // ========================= boundary conditions =============================
 template <int D> pmpm_bc Gravity<D>::device(int global_step) {
   const float ramp_factor = ((float)global_step) / static_cast<float>(ramp_step);
@@ -645,9 +708,16 @@ template <int D> Solver<D>::~Solver() { set_global_step(0); }  // solver.cpp:199
 // from its solver (its DP cannot be used there); here every DP material is
 // initialised at construction so it can be.
 template <int D> void Solver<D>::initialize_materials() {
-  bool any_dp = false;
-  for (auto &m : materials)
+  bool any_dp = false, any_ext = false;
+  for (auto &m : materials) {
     if (std::holds_alternative<DP_rheo>(m)) any_dp = true;
+    if (std::holds_alternative<VonMises>(m) || std::holds_alternative<MohrCoulomb>(m) ||
+        std::holds_alternative<ModifiedCamClay>(m))
+      any_ext = true;
+    if (D != 3 && std::holds_alternative<VonMises>(m))
+      throw std::invalid_argument("VonMises material only implemented for 3D");  // vonmises.cpp:46-49
+  }
+  if (any_ext) initialize_plastic_history();
   if (!any_dp) return;
   particles.ensure_history();
   ParticleBuffers &b = particles.cur();
@@ -665,6 +735,31 @@ template <int D> void Solver<D>::initialize_materials() {
       for (size_t i = 0; i < N; ++i) dp->pc[i] = (-(s[i] + s[N + i] + s[2 * N + i]) / 3.f) * dp->R;
     }
   }
+}
+
+// History of the post-G2P materials: zero elastic strain and accumulated
+// plastic strain (vonmises.cpp:55-59, mohrcoulomb.cpp:55-59); for the
+// modified Cam-Clay the reference stress is the stress now and the
+// preconsolidation pressure R p0 (modifiedcamclay.cpp:68-104), per particle of
+// that material's colour.
+template <int D> void Solver<D>::initialize_plastic_history() {
+  particles.ensure_plastic_history();
+  ParticleBuffers &b = particles.cur();
+  const size_t N = particles.num_particles;
+  hip_check(hipMemset(b.eps_e.get(), 0, 24 * N), "memset");
+  b.stress_ref = b.stress;
+  auto s = b.stress.download();
+  auto col = b.color.download();
+  std::vector<float> h(2 * N, 0.f);
+  for (size_t i = 0; i < N; ++i) {
+    const int c = col[i];
+    if (c >= (int)materials.size()) continue;
+    if (auto *mcc = std::get_if<ModifiedCamClay>(&materials[c])) {
+      const float p0 = -(s[i] + s[N + i] + s[2 * N + i]) / 3.f;
+      h[N + i] = p0 * mcc->R;
+    }
+  }
+  b.hist.upload(h);
 }
 
 template <int D> bool Solver<D>::has_levelset() const {

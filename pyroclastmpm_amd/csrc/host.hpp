@@ -132,6 +132,7 @@ template <int D> class NodesContainer {
 // ---------------------------------------------------------------------------
 struct ParticleBuffers {
   DevBuf<float> x, v, stress, F, L, mass, volume, volume0, eps_e, stress_ref;
+  DevBuf<float> hist;  // [2][N] plastic history scalars (von Mises, Mohr-Coulomb, Cam-Clay)
   DevBuf<uint8_t> color, flags;
   DevBuf<int32_t> id;
 };
@@ -161,6 +162,8 @@ template <int D> class ParticlesContainer {
   std::vector<int> is_rigid() const;
   std::vector<int> is_active() const;
   std::vector<Mat3> eps_e() const;
+  // [2][N]: accumulated or volumetric plastic strain, preconsolidation pressure
+  std::vector<float> plastic_history() const;
   void set_positions(const std::vector<VecD<D>> &v);
   void set_velocities(const std::vector<VecD<D>> &v);
   void set_stresses(const std::vector<Mat3> &v);
@@ -183,6 +186,7 @@ template <int D> class ParticlesContainer {
   void calculate_initial_volumes();
   void calculate_initial_masses(int mat_id, float density);
   void ensure_history();  // Drucker-Prager eps_e / stress_ref buffers
+  void ensure_plastic_history();  // + the history scalars of the post-G2P materials
 
   // spatial partition of the last partition() call, reference semantics
   std::vector<VecI<D>> bins() const;
@@ -254,7 +258,34 @@ struct DP_rheo {
   std::vector<float> pc;
   pmpm_material device() const;
 };
-using MaterialType = std::variant<LinearElastic, NewtonFluid, LocalGranularRheology, DP_rheo>;
+// vonmises.cpp:34-50 (3D only)
+struct VonMises {
+  VonMises(float density, float E, float pois, float yield_stress, float H);
+  float density, E, pois, yield_stress, H, shear_modulus, bulk_modulus;
+  bool do_update_history = true;
+  bool is_velgrad_strain_increment = false;
+  pmpm_material device() const;
+};
+// mohrcoulomb.cpp:33-52 (angles given in degrees, kept in radians)
+struct MohrCoulomb {
+  MohrCoulomb(float density, float E, float pois, float cohesion, float friction_angle, float dilatancy_angle,
+              float H);
+  float density, E, pois, cohesion, friction_angle, dilatancy_angle, H, shear_modulus, bulk_modulus, lame_modulus;
+  bool do_update_history = true;
+  bool is_velgrad_strain_increment = false;
+  pmpm_material device() const;
+};
+// modifiedcamclay.cpp:47-63 (beta is accepted and unused, as there)
+struct ModifiedCamClay {
+  ModifiedCamClay(float density, float E, float pois, float M, float lam, float kap, float Vs, float R, float Pt,
+                  float beta);
+  float density, E, pois, M, lam, kap, Vs, R, Pt, beta, shear_modulus, bulk_modulus;
+  bool do_update_history = true;
+  bool is_velgrad_strain_increment = false;
+  pmpm_material device() const;
+};
+using MaterialType = std::variant<LinearElastic, NewtonFluid, LocalGranularRheology, DP_rheo, VonMises, MohrCoulomb,
+                                  ModifiedCamClay>;
 
 // ---------------------------------------------------------------------------
 // Boundary conditions.
@@ -356,6 +387,7 @@ template <int D> class Solver {
   void step_fused();
   void step_composed();
   void initialize_materials();
+  void initialize_plastic_history();
   bool has_levelset() const;
   pmpm_params last_prm_{};
   DevBuf<float> dpsi_orig_, x_orig_, psi_tmp_, rigid_dx_;

@@ -1563,7 +1563,10 @@ __device__ inline bool gather_wrapped_lane(const pmpm_grid &g, const TileGeom &t
   const int nx = g.num_cells[0], ny = g.num_cells[1], nz = D > 2 ? g.num_cells[2] : 1;
   const int span = tg.N[0] - tg.T[0] + 1;
   const int lo = tg.nlo, hi = tg.nlo + span - 1;
-  const int tx = node % nx;
+  // node = tx + nx (ty + ny tz); an outside node (bx, by, bz) with the same
+  // linear index has bx = tx + kx nx and by + ny bz = rest - kx (32-bit
+  // arithmetic: grids stay below 2^27 nodes)
+  const int tx = node % nx, rest = node / nx;
   const int sx = lane & 3, sy = (lane >> 2) & 3, sz = lane >> 4;
   const bool my_lane = D > 2 || lane < 16;
   bool found = false;
@@ -1571,16 +1574,16 @@ __device__ inline bool gather_wrapped_lane(const pmpm_grid &g, const TileGeom &t
   for (int kx = -2; kx <= 2; ++kx) {  // wave-uniform loops over the outside nodes
     const int bx = tx + kx * nx;
     if (bx < lo || bx > nx - 1 + hi) continue;
-    const int64_t q = ((int64_t)node - bx) / nx;  // exact: node - bx is a multiple of nx
-    const int qy = (int)(((q % ny) + ny) % ny);
+    const int q = rest - kx;
+    const int qy = ((q % ny) + ny) % ny, qz0 = (q - qy) / ny;  // q = qy + ny qz0, exactly
 #pragma unroll 1
     for (int ky = -2; ky <= 2; ++ky) {
       const int by = qy + ky * ny;
       if (by < lo || by > ny - 1 + hi) continue;
-      const int64_t qz = (q - by) / ny;
+      const int qz = qz0 - ky;
       if (D == 2 && qz != 0) continue;
       if (D == 3 && (qz < lo || qz > nz - 1 + hi)) continue;
-      const int nb[3] = {bx, by, D > 2 ? (int)qz : 0};
+      const int nb[3] = {bx, by, D > 2 ? qz : 0};
       if (bx >= 0 && bx < nx && by >= 0 && by < ny && (D < 3 || (qz >= 0 && qz < nz))) continue;  // the node itself
       if (tg.wrap == 1) {
         if (lane == 0) found = gather_partials<D, NV>(tg, partials, flags, nb, acc) || found;
@@ -2040,6 +2043,38 @@ __global__ __launch_bounds__(kG2PThreads) void g2p_kernel(pmpm_params prm, pmpm_
   }
 }
 
+// The return maps of the materials that run after G2P (von Mises,
+// Mohr-Coulomb, modified Cam-Clay), on particle j of `out` with the L, F, V,
+// stress and elastic strain G2P wrote there; the history scalars follow the
+// particles into cell order here (from slot sorted[j] of `in_hist` when G2P
+// reordered, from slot j in place), for every particle.
+template <int D>
+__global__ __launch_bounds__(256) void ext_material_kernel(pmpm_params prm, const float *__restrict__ in_hist,
+                                                           pmpm_particles out, const int32_t *__restrict__ sorted,
+                                                           int reorder) {
+  const int n = out.n;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int src = reorder ? sorted[j] : j;
+  float h0 = in_hist[src], h1 = in_hist[n + src];
+  const int color = out.color[j];
+  if (color < prm.num_materials && is_ext_material(prm.materials[color].kind)) {
+    float L[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int b = 0; b < D; ++b) L[3 * a + b] = out.L[(a * D + b) * n + j];
+    Sym s = load_sym(out.stress, n, j), eps = load_sym(out.eps_e, n, j);
+    Sym sref = sym_zero();
+    if (out.stress_ref && (prm.materials[color].flags & PMPM_MATF_HAS_STRESS_REF)) sref = load_sym(out.stress_ref, n, j);
+    return_map_ext<D>(prm, color, L, out.volume[j], sref, s, eps, h0, h1);
+    store_sym(out.stress, n, j, s);
+    store_sym(out.eps_e, n, j, eps);
+  }
+  out.hist[j] = h0;
+  out.hist[n + j] = h1;
+}
+
 // Stand-alone return map of one material from the stored velocity gradient.
 template <int D>
 __global__ void stress_update_kernel(pmpm_params prm, int mat_id, pmpm_particles p) {
@@ -2058,6 +2093,12 @@ __global__ void stress_update_kernel(pmpm_params prm, int mat_id, pmpm_particles
   Sym sref = sym_zero();
   if (p.stress_ref && (prm.materials[color].flags & PMPM_MATF_HAS_STRESS_REF)) sref = load_sym(p.stress_ref, n, i);
   return_map<D>(prm, color, p.flags[i], L, p.mass[i], p.volume[i], p.volume0[i], sref, s, eps);
+  if (is_ext_material(prm.materials[color].kind)) {
+    float h0 = p.hist[i], h1 = p.hist[n + i];
+    return_map_ext<D>(prm, color, L, p.volume[i], sref, s, eps, h0, h1);
+    p.hist[i] = h0;
+    p.hist[n + i] = h1;
+  }
   store_sym(p.stress, n, i, s);
   if (p.eps_e) store_sym(p.eps_e, n, i, eps);
 }
@@ -2354,12 +2395,33 @@ int do_p2g(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const 
   return launch_wrap_gather(ctx, prm, g, part, nodes, 1, s);
 }
 
+// True when a material of the list runs after G2P; then the particles must
+// store L, the elastic strain and the history scalars.
+static bool has_ext_material(const pmpm_params &prm) {
+  for (int k = 0; k < prm.num_materials; ++k)
+    if (is_ext_material(prm.materials[k].kind)) return true;
+  return false;
+}
+static bool ext_buffers_ok(const pmpm_params &prm, const pmpm_particles &p) {
+  return !has_ext_material(prm) || p.n == 0 || (p.L && p.eps_e && p.hist);
+}
+
 int do_g2p(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
            const pmpm_spatial &part, const pmpm_nodes &nodes, int mode, hipStream_t s) {
   if (in.n == 0) return PMPM_OK;
+  if (!ext_buffers_ok(prm, in) || !ext_buffers_ok(prm, out)) return PMPM_ERR_INVALID;
   const int reorder = (out.F != in.F) ? 1 : 0;  // in place when F is updated in place
   PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_g2p, prm, g, in, out, part, nodes, mode, reorder, s);
   PMPM_LAUNCH_CHECK();
+  if (has_ext_material(prm)) {
+    const unsigned blocks = (unsigned)ceil_div(in.n, 256);
+    switch (prm.dim) {
+      case 1: PMPM_LAUNCH(ext_material_kernel<1>, (std::string("ext_material_kernel<1>")), dim3(blocks), dim3(256), 0, s, prm, in.hist, out, part.sorted, reorder); break;
+      case 2: PMPM_LAUNCH(ext_material_kernel<2>, (std::string("ext_material_kernel<2>")), dim3(blocks), dim3(256), 0, s, prm, in.hist, out, part.sorted, reorder); break;
+      default: PMPM_LAUNCH(ext_material_kernel<3>, (std::string("ext_material_kernel<3>")), dim3(blocks), dim3(256), 0, s, prm, in.hist, out, part.sorted, reorder); break;
+    }
+    PMPM_LAUNCH_CHECK();
+  }
   return PMPM_OK;
 }
 
@@ -2442,6 +2504,7 @@ extern "C" int pmpm_musl_g2p(const pmpm_params *prm, const pmpm_grid *grid, cons
 extern "C" int pmpm_stress_update(const pmpm_params *prm, int32_t mat_id, pmpm_particles *p, pmpm_stream_t stream) {
   if (!params_ok(prm) || !particles_ok(p) || !p || (p->n > 0 && !p->L)) return PMPM_ERR_INVALID;
   if (mat_id < 0 || mat_id >= prm->num_materials) return PMPM_ERR_INVALID;
+  if (!ext_buffers_ok(*prm, *p)) return PMPM_ERR_INVALID;
   if (p->n == 0) return PMPM_OK;
   hipStream_t s = (hipStream_t)stream;
   const unsigned blocks = (unsigned)ceil_div(p->n, 256);

@@ -158,6 +158,14 @@ def _mk_material(pm, kind, p):
                        p.get("dilatancy_angle", 0.0), p.get("solid_volume_fraction", 1.0))
         m.apex_guard = bool(p.get("apex_guard", 1))
         return m
+    if kind == "VonMises":
+        return pm.VonMises(p["density"], p["E"], p["pois"], p["yield_stress"], p["H"])
+    if kind == "MohrCoulomb":
+        return pm.MohrCoulomb(p["density"], p["E"], p.get("pois", 0.0), p.get("cohesion", 0.0),
+                              p.get("friction_angle", 0.0), p.get("dilatancy_angle", 0.0), p.get("H", 0.0))
+    if kind == "ModifiedCamClay":
+        return pm.ModifiedCamClay(p["density"], p["E"], p["pois"], p["M"], p["lam"], p["kap"], p["Vs"], p["R"],
+                                  p["Pt"], p.get("beta", 0.0))
     raise ValueError(kind)
 
 
