@@ -106,6 +106,12 @@ typedef struct pmpm_nodes {
                     (dim floats), then dv = (mom_nt - mom) / mass (dim floats),
                     zero padded; all zero where mass <= cutoff. No counterpart
                     in the reference (it divides per gather). */
+  uint8_t *clamp; /* [nn] or NULL: the components a NodeDomain condition set to
+                     zero in the last grid pass that applied node conditions
+                     (bit a: mom, bit 3 + a: mom_nt), written by that pass and
+                     read by pmpm_adjoint_step. NULL: the context keeps one
+                     such record (enough when the adjoint recomputes the
+                     forward). No counterpart in the reference. */
 } pmpm_nodes;
 
 /* Spatial partition (spatialpartition.h:77-132 SpatialPartition fields). */

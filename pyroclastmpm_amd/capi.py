@@ -46,7 +46,7 @@ def node_vel_stride(dim):
 
 
 class Nodes(C.Structure):
-    _fields_ = [("mass", vp), ("mom", vp), ("mom_nt", vp), ("vel", vp)]
+    _fields_ = [("mass", vp), ("mom", vp), ("mom_nt", vp), ("vel", vp), ("clamp", vp)]
 
 
 class Spatial(C.Structure):

@@ -29,6 +29,18 @@ struct pmpm_context {
   // ([fields][cells][64] floats)
   float *wrap_rec = nullptr;
   size_t wrap_rec_cap = 0;
+  // a byte per node: the typed-cubic wrap-around sums reach it this pass
+  // (allocated in words of 4 bytes)
+  float *wrap_touch = nullptr;
+  size_t wrap_touch_cap = 0;
+  // the nodes the typed-cubic wrap-around sums reach this pass: [0] count,
+  // [1] finished gather workgroups, [2..] node indices (each once)
+  int32_t *wrap_list = nullptr;
+  size_t wrap_list_cap = 0;
+  // a byte per node: the components the last grid pass's NodeDomain
+  // condition set to zero (words of 4 bytes), read by the adjoint
+  float *clamp_mask = nullptr;
+  size_t clamp_mask_cap = 0;
   // adjoint step: per-particle scatter records (15 floats, cell order) and
   // node adjoints (one record of 8 floats per node, 4 in 1D)
   float *adj_rec = nullptr;
@@ -127,7 +139,7 @@ class TimedRegion {
 // Fill n floats/ints with zero (a kernel, not hipMemsetAsync, so the step can
 // be captured in a HIP graph).
 int zero_fill(void *ptr, int64_t n_words, hipStream_t s);
-constexpr int kZeroLists = 4;
+constexpr int kZeroLists = 6;
 struct ZeroLists {  // buffers of 32-bit words to clear in one launch
   uint32_t *p[kZeroLists];
   int64_t n[kZeroLists];
@@ -148,7 +160,8 @@ bool step_args_ok(const pmpm_params *prm, const pmpm_grid *grid, const pmpm_part
 // Workspace the P2G passes of one step use on grid g: tile partials (floats)
 // and tile flags, wrap-around records (floats) and wrap cells.
 void p2g_workspace(const pmpm_params &prm, const pmpm_grid &g, size_t *partial_floats, size_t *tiles,
-                   size_t *wrap_floats, size_t *wrap_cells);
+                   size_t *wrap_floats, size_t *wrap_cells, size_t *wrap_touch_words, size_t *wrap_list_words,
+                   size_t *clamp_words);
 // pmpm_partition whose first kernel also clears the buffers of `extra`.
 int partition_clearing(pmpm_context *ctx, int dim, const pmpm_grid *grid, const float *x, int n, pmpm_spatial *part,
                        hipStream_t s, const ZeroLists &extra);

@@ -1,7 +1,7 @@
 // noalloc.hip -- pmpm_step_noalloc: the whole step without a context and
 // without allocating. Its workspace (the radix sort's ping-pong keys, slots and
-// digit histograms, the P2G tile partials and flags, the wrap-around records,
-// the counters) is carved out of one file-scope device array of
+// digit histograms, the P2G tile partials and flags, the wrap-around records
+// and node flags, the grid pass's clamp record, the counters) is carved out of one file-scope device array of
 // PMPM_NOALLOC_SCRATCH_BYTES, which each GPU holds for the life of the process.
 // Every call on a GPU uses that GPU's copy: calls that overlap on one GPU race.
 #include <hip/hip_runtime.h>
@@ -57,8 +57,10 @@ extern "C" int pmpm_step_noalloc(const pmpm_params *prm, const pmpm_grid *grid, 
   const size_t n = (size_t)(a->n > 0 ? a->n : 1);
   const size_t nblocks = (size_t)ceil_div((int64_t)n, kSortTile);
   const size_t hist_len = (size_t)kMaxRadix * nblocks + kMaxRadix;
-  size_t partial_floats = 0, tiles = 0, wrap_floats = 0, wrap_cells = 0;
-  p2g_workspace(*prm, *grid, &partial_floats, &tiles, &wrap_floats, &wrap_cells);
+  size_t partial_floats = 0, tiles = 0, wrap_floats = 0, wrap_cells = 0, touch_words = 0, list_words = 0,
+         clamp_words = 0;
+  p2g_workspace(*prm, *grid, &partial_floats, &tiles, &wrap_floats, &wrap_cells, &touch_words, &list_words,
+                &clamp_words);
   Carver c;
   const size_t o_keys = c.take(n * sizeof(uint32_t));
   const size_t o_vals = c.take(n * sizeof(int32_t));
@@ -67,6 +69,9 @@ extern "C" int pmpm_step_noalloc(const pmpm_params *prm, const pmpm_grid *grid, 
   const size_t o_partials = c.take(partial_floats * sizeof(float));
   const size_t o_flags = c.take(tiles * sizeof(int32_t));
   const size_t o_wrec = c.take(wrap_floats * sizeof(float));
+  const size_t o_touch = c.take(touch_words * sizeof(float));
+  const size_t o_list = c.take(list_words * sizeof(int32_t));
+  const size_t o_clamp = c.take(clamp_words * sizeof(float));
   if (c.used > (size_t)PMPM_NOALLOC_SCRATCH_BYTES) {
     note_error("pmpm_step_noalloc: the step needs more workspace than PMPM_NOALLOC_SCRATCH_BYTES "
                "(fewer particles or a smaller grid, or pmpm_step with a context)");
@@ -92,6 +97,14 @@ extern "C" int pmpm_step_noalloc(const pmpm_params *prm, const pmpm_grid *grid, 
   ctx.tile_flags_cap = tiles;
   ctx.wrap_rec = wrap_floats ? reinterpret_cast<float *>(base + o_wrec) : nullptr;
   ctx.wrap_rec_cap = wrap_floats;
+  // (the step's first kernel clears the node flags and the list counters:
+  // the scratch holds whatever an earlier call left)
+  ctx.wrap_touch = touch_words ? reinterpret_cast<float *>(base + o_touch) : nullptr;
+  ctx.wrap_touch_cap = touch_words;
+  ctx.wrap_list = list_words ? reinterpret_cast<int32_t *>(base + o_list) : nullptr;
+  ctx.wrap_list_cap = list_words;
+  ctx.clamp_mask = reinterpret_cast<float *>(base + o_clamp);
+  ctx.clamp_mask_cap = clamp_words;
   // (the context is a plain struct on this stack frame: nothing frees the
   // scratch pieces it points at)
   return step_on(&ctx, prm, grid, musl, a, b, part, nodes, (hipStream_t)stream);

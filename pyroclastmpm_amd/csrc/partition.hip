@@ -89,20 +89,33 @@ __global__ void hash_kernel(const float *__restrict__ x, int n, pmpm_grid g, uin
 }
 
 // Per-block digit histogram (digits of RB bits), digit-major layout
-// hist[digit * nblocks + block].
+// hist[digit * nblocks + block]. The keys arrive nearly sorted (the previous
+// step's G2P wrote the particles in cell order), so a wave's 64 consecutive
+// keys hold a few runs of one digit: the head lane of each run adds the
+// run's length, one LDS atomic per run instead of one per key.
 template <int RB>
 __global__ __launch_bounds__(kSortBlock) void radix_hist_kernel(const uint32_t *__restrict__ keys, int n, int shift,
                                                                 uint32_t *__restrict__ hist, int nblocks) {
   constexpr int R = 1 << RB;
   __shared__ uint32_t h[R];
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63;
   for (int d = t; d < R; d += kSortBlock) h[d] = 0u;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * kSortTile;
 #pragma unroll 4
   for (int r = 0; r < kSortItems; ++r) {
     const int64_t i = base + (int64_t)r * kSortBlock + t;
-    if (i < n) atomicAdd(&h[(keys[i] >> shift) & (R - 1)], 1u);
+    const bool valid = i < n;  // a prefix of the wave's lanes
+    const uint32_t d = valid ? (keys[i] >> shift) & (R - 1) : 0u;
+    const uint32_t prev = (uint32_t)__shfl_up((int)d, 1, 64);
+    const bool head = valid && (lane == 0 || prev != d);
+    const uint64_t heads = __ballot(head);
+    const int nvalid = __popcll(__ballot(valid));
+    if (head) {
+      const uint64_t later = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
+      const int next = later ? __ffsll((unsigned long long)later) - 1 : nvalid;
+      atomicAdd(&h[d], (uint32_t)(next - lane));
+    }
   }
   __syncthreads();
   for (int d = t; d < R; d += kSortBlock) hist[(int64_t)d * nblocks + blockIdx.x] = h[d];
@@ -288,6 +301,9 @@ extern "C" int pmpm_context_destroy(pmpm_context *ctx) {
   (void)hipFree(ctx->partials);
   (void)hipFree(ctx->tile_flags);
   (void)hipFree(ctx->wrap_rec);
+  (void)hipFree(ctx->wrap_touch);
+  (void)hipFree(ctx->wrap_list);
+  (void)hipFree(ctx->clamp_mask);
   (void)hipFree(ctx->keys_tmp);
   (void)hipFree(ctx->vals_tmp);
   (void)hipFree(ctx->hist);

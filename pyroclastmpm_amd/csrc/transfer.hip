@@ -1117,10 +1117,12 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
 // offset (l % 16) / 4 and the z weight at offset l % 4 for particle l / 16,
 // and owns node (i = l / 16, j = (l % 16) / 4, k = l % 4) of the result.
 // Wave w walks the 16 cells of z layer w one after another: it stages up to
-// 64 of a cell's particles in LDS (lane = particle, coalesced loads; weights'
-// fractions and the products m v, -V sigma computed once per particle), runs
-// the products, and adds each lane's node into the wave's own LDS node block;
-// the four blocks overlap in z and are summed into the tile slice at the end.
+// 32 of a cell's particles in LDS (lane = particle, coalesced loads; the 24
+// span weights and derivatives and the products m v, -V sigma computed once
+// per particle), so an instruction's operands are a record read and one
+// multiply; it runs the products on four independent accumulators and adds
+// each lane's node into the wave's own LDS node block; the four blocks
+// overlap in z and are summed into the tile slice at the end.
 struct MfmaGeom {
   static constexpr int CELLS = 16;                      // cells per wave (x)
   static constexpr int BX = CELLS + 3, BY = 4, BZ = 4;  // a wave's node block
@@ -1128,7 +1130,13 @@ struct MfmaGeom {
   // pk + 20 s1 + 80 s2, fall into 64 distinct banks (19 would pair some up)
   static constexpr int BXP = BX + 1;
   static constexpr int BN = BXP * BY * BZ;
-  static constexpr int NF = 13;                         // staged floats per particle
+  static constexpr int CH = 32;  // particles staged per round
+  // a staged particle's record: per span offset s (7 s + 0..6) the weights
+  // Nx, dNx, Ny, dNy, Nz, dNz and mass / moment field s (m, m vx, m vy,
+  // m vz), then A = -V sigma (28..33: xx, yy, zz, xy, xz, yz) and a zero
+  // (34); the odd pitch puts the 16 records one read touches on distinct
+  // banks
+  static constexpr int RS = 35;
 };
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -1150,9 +1158,10 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
   constexpr int NV = REMAP ? 3 : 7, NTN = G::NTN, NLO = G::NLO;
   static_assert(NV == p2g_nv<3, REMAP ? 1 : 0>(), "slice fields of the matching p2g_kernel mode");
   static_assert(!(REMAP && FEXT), "the re-map carries no force");
-  constexpr int NFX = M::NF + (FEXT ? 3 : 0);  // staged rows (13..15: wall force)
+  constexpr int RS = M::RS + (FEXT ? 4 : 0);  // record pitch (35..37: wall force)
   __shared__ float block[4][NV][M::BN];
-  __shared__ float stage[4][NFX][64];
+  __shared__ float stage[4][M::CH * RS];
+  __shared__ __attribute__((aligned(16))) float s_coef[5][4][4];
 
   const int n = in.n;
   const float inv_h = g.inv_cell_size;
@@ -1171,148 +1180,236 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
       cs = part.cell_start[cell];
     }
   }
+  if (t < 80) (&s_coef[0][0][0])[t] = (&kCubic[0][0][0])[t];
   if (!__syncthreads_or(ce > cs)) {
     if (t == 0) flags[blockIdx.x] = 0;
     return;
   }
   float *blk = &block[wave][0][0];
   for (int i = lane; i < NV * M::BN; i += 64) blk[i] = 0.f;
-  float *st = &stage[wave][0][0];
+  float *st = &stage[wave][0];
 
   // lane roles in the products
   const int s1 = (lane & 15) >> 2, s2 = lane & 3, pk = lane >> 4;
-  // staged fields of row a = s2 of A = -V sigma (xx 7, yy 8, zz 9, xy 10,
-  // xz 11, yz 12); row 3 of the product is padding
-  const int ax = s2 == 1 ? 10 : (s2 == 2 ? 11 : 7);
-  const int ay = s2 == 1 ? 8 : (s2 == 2 ? 12 : 10);
-  const int az = s2 == 1 ? 12 : (s2 == 2 ? 9 : 11);
-  // y and z coefficients of this lane's offsets (piece 3 - s of the node's
-  // type), fixed for the wave's row
-  float yc[4], zc[4];
-  {
-    const int ty = cubic_node_type(cy + NLO + s1, g.num_cells[1], prm.node_types_built);
-    const int tz = cubic_node_type(cz + NLO + s2, g.num_cells[2], prm.node_types_built);
+  // staged entries this lane reads (a particle's record, RS floats): the
+  // weights of its offsets (x and y at s1, z at s2), mass / moment field s2,
+  // row a = s2 of A = -V sigma (row 3 of the product is padding and reads the
+  // record's zero) and, with walls, force component s2
+  const int ex = 7 * s1, ez = 7 * s2;
+  const int ax = s2 == 0 ? 28 : (s2 == 1 ? 31 : (s2 == 2 ? 32 : 34));
+  const int ay = s2 == 0 ? 31 : (s2 == 1 ? 29 : (s2 == 2 ? 33 : 34));
+  const int az = s2 == 0 ? 32 : (s2 == 1 ? 33 : (s2 == 2 ? 30 : 34));
+  const int af = s2 < 3 ? 35 + s2 : 34;
+  // span coefficients: piece 3 - s of the type of the node at offset s,
+  // read from the LDS copy of the table (the types: y and z fixed for the
+  // wave's row, x per cell; wave-uniform)
+  int tys[4], tzs[4], txs[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      yc[q] = kCubic[ty][3 - s1][q];
-      zc[q] = kCubic[tz][3 - s2][q];
-    }
+  for (int s = 0; s < 4; ++s) {
+    tys[s] = cubic_node_type(cy + NLO + s, g.num_cells[1], prm.node_types_built);
+    tzs[s] = cubic_node_type(cz + NLO + s, g.num_cells[2], prm.node_types_built);
+    txs[s] = 0;
   }
+  auto coef = [&](int type, int s) { return reinterpret_cast<const float4 *>(&s_coef[type][3 - s][0])[0]; };
   // cubic piece at r (offset s of the span), value and derivative; offset 0
   // at frac = 1 - 2^-24 rounds to r = 2, where the spline is 0 (span_eval)
-  auto piece = [&](const float *c, float r, bool first, float &N, float &dN) {
-    N = ((c[3] * r + c[2]) * r + c[1]) * r + c[0];
-    dN = inv_h * ((3.f * c[3] * r + 2.f * c[2]) * r + c[1]);
+  auto piece = [&](const float4 c, float r, bool first, float &N, float &dN) {
+    N = ((c.w * r + c.z) * r + c.y) * r + c.x;
+    dN = inv_h * ((3.f * c.w * r + 2.f * c.z) * r + c.y);
     if (first && !(r < 2.f)) N = dN = 0.f;
   };
 
-  for (int lx = 0; lx < M::CELLS; ++lx) {  // wave-uniform
-    const int c0 = __shfl(cs, lx), c1 = __shfl(ce, lx);
-    if (c1 <= c0) continue;
+  // The wave walks its cells' particles in chunks of up to CH (lane =
+  // particle), cell by cell. Loads run ahead of the work: while a chunk's
+  // products run, the next chunk's particle fields and the slots of the one
+  // after are in flight, so the latency of the dependent slot -> field loads
+  // hides behind the products of the chunk before.
+  // a chunk: cell lx of the row, particles [w0, w0 + CH) of its range; the
+  // ranges are read from lanes 0..15 with scalar lane reads (the chunk walk
+  // stays in scalar registers)
+  auto cs_of = [&](int l) { return __builtin_amdgcn_readlane(cs, l); };
+  auto ce_of = [&](int l) { return __builtin_amdgcn_readlane(ce, l); };
+  auto first_chunk_from = [&](int l0, int &lx, int &w0) {  // wave-uniform
+    lx = M::CELLS;
+    for (int l = l0; l < M::CELLS; ++l) {
+      const int a = cs_of(l), b = ce_of(l);
+      if (b > a) { lx = l; w0 = a; return; }
+    }
+  };
+  auto next_chunk = [&](int lx, int w0, int &nlx, int &nw0) {
+    if (w0 + M::CH < ce_of(lx)) { nlx = lx; nw0 = w0 + M::CH; return; }
+    first_chunk_from(lx + 1, nlx, nw0);
+  };
+  // raw fields of one particle of a chunk, held in registers until staged.
+  // The loads are unconditional (lanes past the chunk read the chunk's last
+  // particle) so that no branch or select waits for them: they stay in
+  // flight while the chunk before is staged and multiplied.
+  struct Raw {
+    int slot;
+    bool valid;
+    float x[3], v[3], m, vol;
+    Sym sg;
+  };
+  // (lx == CELLS: no chunk; its loads read particle 0 and stage nothing, so
+  // that every step issues the same loads)
+  auto chunk_len = [&](int lx, int w0) { return lx < M::CELLS ? min(M::CH, ce_of(lx) - w0) : 0; };
+  auto chunk_j = [&](int lx, int w0) {  // this lane's sorted index, clamped to the chunk
+    return lx < M::CELLS ? w0 + min(lane, chunk_len(lx, w0) - 1) : 0;
+  };
+  auto chunk_slot = [&](int lx, int w0) { return part.sorted[chunk_j(lx, w0)]; };
+  auto load_raw = [&](int lx, int w0, int slot, Raw &r) {
+    r.slot = slot;
+    r.valid = lane < chunk_len(lx, w0);
+    const int j = chunk_j(lx, w0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      r.x[a] = in.x[a * n + slot];
+      r.v[a] = REMAP ? out.v[a * n + j] : in.v[a * n + slot];
+    }
+    r.m = in.mass[slot];
+    r.vol = REMAP ? 0.f : in.volume[slot];
+    r.sg = REMAP ? sym_zero() : load_sym(in.stress, n, slot);
+  };
+  // a chunk's particle records into the LDS stage (lane = particle); the
+  // records past the chunk's particles are zero
+  auto stage_raw = [&](int lx, const Raw &r) {
+    if (lane >= M::CH) return;
+    float *sp = st + lane * RS;
+    if (!r.valid) {
+#pragma unroll
+      for (int e = 0; e < RS; ++e) sp[e] = 0.f;
+      return;
+    }
     const int cx = bx * M::CELLS + lx;
-    float xc[4];
-    {
-      const int tx = cubic_node_type(cx + NLO + s1, g.num_cells[0], prm.node_types_built);
+    float fr[3];
+    bool own = true;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) xc[q] = kCubic[tx][3 - s1][q];
+    for (int a = 0; a < 3; ++a) {
+      const float ta = cell_coord(r.x[a], g.origin[a], inv_h);
+      const int ba = (int)ta;
+      own = own && ta >= 0.f && ba == (a == 0 ? cx : (a == 1 ? cy : cz));
+      fr[a] = __fsub_rn(ta, (float)ba);
     }
-    v4f C1 = {0.f, 0.f, 0.f, 0.f}, C2 = {0.f, 0.f, 0.f, 0.f};
-    for (int w0 = c0; w0 < c1; w0 += 64) {
-      const int wn = min(64, c1 - w0);
-      if (lane < wn) {
-        const int j = w0 + lane;
-        const int slot = part.sorted[j];
-        float x[3], fr[3];
-        bool own = true;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          x[a] = in.x[a * n + slot];
-          const float ta = cell_coord(x[a], g.origin[a], inv_h);
-          const int ba = (int)ta;
-          own = own && ta >= 0.f && ba == (a == 0 ? cx : (a == 1 ? cy : cz));
-          fr[a] = __fsub_rn(ta, (float)ba);
-        }
-        const float m = in.mass[slot];
-        float v[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) v[a] = REMAP ? out.v[a * n + j] : in.v[a * n + slot];
-        Sym A = sym_zero();
-        float fe[3] = {0.f, 0.f, 0.f};
-        if (!REMAP) {
-          const Sym sg = load_sym(in.stress, n, slot);
-          const float vol = in.volume[slot];
-          const float mvol = -vol;
-          A = Sym{mvol * sg.xx, mvol * sg.yy, mvol * sg.zz, mvol * sg.xy, mvol * sg.xz, mvol * sg.yz};
-          if (FEXT) planar_force<3>(prm, g, x, v, vol, m, fe);
-        }
-        if (!own) {
-          // listed in this cell but binned elsewhere (a partition of other
-          // positions): deposited over its own window, no product terms here
+    Sym A = sym_zero();
+    float fe[3] = {0.f, 0.f, 0.f};
+    if (!REMAP) {
+      const float mvol = -r.vol;
+      A = Sym{mvol * r.sg.xx, mvol * r.sg.yy, mvol * r.sg.zz, mvol * r.sg.xy, mvol * r.sg.xz, mvol * r.sg.yz};
+      if (FEXT) planar_force<3>(prm, g, r.x, r.v, r.vol, r.m, fe);
+    }
+    if (!own) {
+      // listed in this cell but binned elsewhere (a partition of other
+      // positions): deposited over its own window, no product terms here
 #pragma unroll 1
-          for (int sy = 0; sy < 4; ++sy)
-            p2g_stray<3, SF, REMAP ? 1 : 0>(prm, g, nodes, in.x, n, slot, sy, m, v, A, fe);
-        }
-        const float keep = own ? 1.f : 0.f;
-        st[0 * 64 + lane] = fr[0];
-        st[1 * 64 + lane] = fr[1];
-        st[2 * 64 + lane] = fr[2];
-        st[3 * 64 + lane] = REMAP ? 0.f : keep * m;  // (the re-map's row 0 is padding)
-#pragma unroll
-        for (int a = 0; a < 3; ++a) st[(4 + a) * 64 + lane] = keep * m * v[a];
-        if (!REMAP) {
-          st[7 * 64 + lane] = keep * A.xx;
-          st[8 * 64 + lane] = keep * A.yy;
-          st[9 * 64 + lane] = keep * A.zz;
-          st[10 * 64 + lane] = keep * A.xy;
-          st[11 * 64 + lane] = keep * A.xz;
-          st[12 * 64 + lane] = keep * A.yz;
-        }
-        if (FEXT) {
-#pragma unroll
-          for (int a = 0; a < 3; ++a) st[(M::NF + a) * 64 + lane] = keep * fe[a];
-        }
-      }
-      wave_lds_sync();
-      for (int k0 = 0; k0 < wn; k0 += 4) {
-        const int q = k0 + pk;
-        const bool valid = q < wn;
-        const int qq = valid ? q : 0;
-        const float fx = st[0 * 64 + qq], fy = st[1 * 64 + qq], fz = st[2 * 64 + qq];
-        // F_f (f = s2): m, m vx, m vy, m vz; A_{a, .} (a = s2; row 3 is padding)
-        const float Ff = st[(3 + s2) * 64 + qq];
-        const float Aax = REMAP ? 0.f : st[ax * 64 + qq], Aay = REMAP ? 0.f : st[ay * 64 + qq];
-        const float Aaz = REMAP ? 0.f : st[az * 64 + qq];
-        float Nx, dNx, Ny, dNy, Nz, dNz;
-        piece(xc, __fsub_rn(fx, (float)(NLO + s1)), s1 == 0, Nx, dNx);
-        piece(yc, __fsub_rn(fy, (float)(NLO + s1)), s1 == 0, Ny, dNy);
-        piece(zc, __fsub_rn(fz, (float)(NLO + s2)), s2 == 0, Nz, dNz);
-        const float pad = (valid && s2 < 3) ? 1.f : 0.f;
-        const float a1 = valid ? Nx * Ff : 0.f;
-        float a2x = pad * dNx * Aax;
-        const float a2y = pad * Nx * Aay, a2z = pad * Nx * Aaz;
-        if (FEXT) a2x += pad * Nx * st[(M::NF + (s2 < 3 ? s2 : 0)) * 64 + qq];  // + w f_a
-        const float b1 = valid ? Ny * Nz : 0.f;
-        const float b2y = valid ? dNy * Nz : 0.f, b2z = valid ? Ny * dNz : 0.f;
-        C1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, C1, 0, 0, 0);
-        if (!REMAP) {
-          C2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2x, b1, C2, 0, 0, 0);
-          C2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2y, b2y, C2, 0, 0, 0);
-          C2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2z, b2z, C2, 0, 0, 0);
-        }
-      }
-      wave_lds_sync();
+      for (int sy = 0; sy < 4; ++sy)
+        p2g_stray<3, SF, REMAP ? 1 : 0>(prm, g, nodes, in.x, n, r.slot, sy, r.m, r.v, A, fe);
     }
-    // node (i, j, k) = (l / 16, (l % 16) / 4, l % 4) of the cell's window
-    const int nb = (lx + pk) + M::BXP * (s1 + M::BY * s2);
-    if (REMAP) {  // rows 1..3: m v'
+    const float keep = own ? 1.f : 0.f;
 #pragma unroll
-      for (int r = 1; r < 4; ++r) blk[(r - 1) * M::BN + nb] += C1[r];
-    } else {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) blk[r * M::BN + nb] += C1[r];
-#pragma unroll
-      for (int r = 0; r < 3; ++r) blk[(4 + r) * M::BN + nb] += C2[r];
+    for (int s = 0; s < 4; ++s) {
+      float N, dN;
+      piece(coef(txs[s], s), __fsub_rn(fr[0], (float)(NLO + s)), s == 0, N, dN);
+      sp[7 * s + 0] = N;
+      sp[7 * s + 1] = dN;
+      piece(coef(tys[s], s), __fsub_rn(fr[1], (float)(NLO + s)), s == 0, N, dN);
+      sp[7 * s + 2] = N;
+      sp[7 * s + 3] = dN;
+      piece(coef(tzs[s], s), __fsub_rn(fr[2], (float)(NLO + s)), s == 0, N, dN);
+      sp[7 * s + 4] = N;
+      sp[7 * s + 5] = dN;
     }
+    sp[6] = REMAP ? 0.f : keep * r.m;  // (the re-map's field 0 is padding)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) sp[7 * (a + 1) + 6] = keep * r.m * r.v[a];
+    if (!REMAP) {
+      sp[28] = keep * A.xx;
+      sp[29] = keep * A.yy;
+      sp[30] = keep * A.zz;
+      sp[31] = keep * A.xy;
+      sp[32] = keep * A.xz;
+      sp[33] = keep * A.yz;
+    }
+    sp[34] = 0.f;
+    if (FEXT) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) sp[35 + a] = keep * fe[a];
+    }
+  };
+
+  // The chunk walk: chunk k's fields in one register set while chunk k+1's
+  // load into the other and chunk k+2's slots load (the two sets alternate,
+  // so no register copy waits for a load in flight). Each step issues the
+  // slots before the fields, so the in-order load counter lets a wait for
+  // the slots leave the older fields in flight.
+  int lx, w0 = 0;
+  first_chunk_from(0, lx, w0);
+  int nlx = M::CELLS, nw0 = 0;
+  next_chunk(lx, w0, nlx, nw0);
+  Raw ra, rb;
+  int sa = chunk_slot(lx, w0), sb = chunk_slot(nlx, nw0);
+  load_raw(lx, w0, sa, ra);
+  int xc_cell = -1;
+  // four independent accumulation chains: mass and moments, and the three
+  // terms of the force (summed when the cell is done)
+  v4f C1 = {0.f, 0.f, 0.f, 0.f}, C2x = C1, C2y = C1, C2z = C1;
+  auto chunk = [&](Raw &cur, int &s_cur, Raw &nxt, int &s_nxt) {
+    int nnlx = M::CELLS, nnw0 = 0;
+    if (nlx < M::CELLS) next_chunk(nlx, nw0, nnlx, nnw0);
+    s_cur = chunk_slot(nnlx, nnw0);      // the slots of chunk k+2
+    load_raw(nlx, nw0, s_nxt, nxt);      // in flight during this chunk's products
+    if (xc_cell != lx) {  // x node types of the new cell's span
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        txs[s] = cubic_node_type(bx * M::CELLS + lx + NLO + s, g.num_cells[0], prm.node_types_built);
+      xc_cell = lx;
+    }
+    stage_raw(lx, cur);
+    wave_lds_sync();
+    const int wn = min(M::CH, ce_of(lx) - w0);
+#pragma unroll
+    for (int k0 = 0; k0 < M::CH; k0 += 4) {
+      if (k0 >= wn) break;  // wave-uniform
+      const float *sp = st + (k0 + pk) * RS;
+      const float Nx = sp[ex], dNx = sp[ex + 1], Ny = sp[ex + 2], dNy = sp[ex + 3];
+      const float Nz = sp[ez + 4], dNz = sp[ez + 5], Ff = sp[ez + 6];
+      const float b1 = Ny * Nz;
+      C1 = __builtin_amdgcn_mfma_f32_16x16x4f32(Nx * Ff, b1, C1, 0, 0, 0);
+      if (!REMAP) {
+        float a2x = dNx * sp[ax];
+        if (FEXT) a2x += Nx * sp[af];  // + w f_a
+        C2x = __builtin_amdgcn_mfma_f32_16x16x4f32(a2x, b1, C2x, 0, 0, 0);
+        C2y = __builtin_amdgcn_mfma_f32_16x16x4f32(Nx * sp[ay], dNy * Nz, C2y, 0, 0, 0);
+        C2z = __builtin_amdgcn_mfma_f32_16x16x4f32(Nx * sp[az], Ny * dNz, C2z, 0, 0, 0);
+      }
+    }
+    wave_lds_sync();  // the stage is read; the next chunk may overwrite it
+    if (nlx != lx) {
+      // the cell is done: node (i, j, k) = (l / 16, (l % 16) / 4, l % 4) of
+      // its window
+      const int nb = (lx + pk) + M::BXP * (s1 + M::BY * s2);
+      if (REMAP) {  // rows 1..3: m v'
+#pragma unroll
+        for (int r = 1; r < 4; ++r) blk[(r - 1) * M::BN + nb] += C1[r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) blk[r * M::BN + nb] += C1[r];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) blk[(4 + r) * M::BN + nb] += (C2x[r] + C2y[r]) + C2z[r];
+      }
+      C1 = v4f{0.f, 0.f, 0.f, 0.f};
+      C2x = C1;
+      C2y = C1;
+      C2z = C1;
+    }
+    lx = nlx;
+    w0 = nw0;
+    nlx = nnlx;
+    nw0 = nnw0;
+  };
+  while (lx < M::CELLS) {  // wave-uniform
+    chunk(ra, sa, rb, sb);
+    if (lx >= M::CELLS) break;
+    chunk(rb, sb, ra, sa);
   }
   __syncthreads();
   // slice node (ix, iy, iz): the blocks of waves iz - 3 .. iz
@@ -1339,128 +1436,180 @@ template <int D, int MODE, int NV, int NS>
 __device__ inline void wrap_cell_sums(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in,
                                       const pmpm_particles &out, const pmpm_spatial &part, float *rec, int wcells,
                                       int w, const int *c, int cs, int ce, int lane, float (*s_frac)[64],
-                                      float (*s_v)[64], float (*s_A)[64], float *s_m, int *s_own) {
+                                      float (*s_v)[64], float (*s_A)[64], float *s_m, int *s_own, int *s_list,
+                                      uint8_t *touch, int32_t *wlist) {
   constexpr int SPAN = 4, NLO = -1;
-  const int s[3] = {lane % SPAN, D > 1 ? (lane / SPAN) % SPAN : 0, D > 2 ? lane / (SPAN * SPAN) : 0};
   const int nlanes = D == 3 ? 64 : (D == 2 ? 16 : 4);
-  int nb[3] = {0, 0, 0};
-  bool inside = true;
-#pragma unroll
-  for (int a = 0; a < D; ++a) {
-    nb[a] = c[a] + NLO + s[a];
-    inside = inside && nb[a] >= 0 && nb[a] < g.num_cells[a];
-  }
   const int nn = g.num_cells_total;
-  const int64_t lin = lin_index(g, nb, D);
-  const bool active = lane < nlanes && !inside && lin >= 0 && lin < nn;
-  // r = frac - (NLO + s) lies in piece 3 - s of the wrapped node's spline
-  // (the same piece argument as span_eval): its coefficients are fixed per
-  // lane
-  float cf[3][4];
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int k = 0; k < 4; ++k) cf[a][k] = 0.f;
-  if (active) {
-    int ids[3];
-    node_ids<D>(g, lin, ids);
+  // the node of span offset o (o = sx + 4 (sy + 4 sz)); active: outside the
+  // grid with its linear index in range
+  auto offset_node = [&](int o, int *sv, int64_t &lin) {
+    sv[0] = o % SPAN;
+    sv[1] = D > 1 ? (o / SPAN) % SPAN : 0;
+    sv[2] = D > 2 ? o / (SPAN * SPAN) : 0;
+    int nb[3] = {0, 0, 0};
+    bool inside = true;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      const int type = cubic_node_type(ids[a], g.num_cells[a], prm.node_types_built);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) cf[a][k] = kCubic[type][3 - s[a]][k];
+      nb[a] = c[a] + NLO + sv[a];
+      inside = inside && nb[a] >= 0 && nb[a] < g.num_cells[a];
     }
+    lin = lin_index(g, nb, D);
+    return !inside && lin >= 0 && lin < nn;
+  };
+  int sv[3];
+  int64_t lin;
+  const bool active = lane < nlanes && offset_node(lane, sv, lin);
+  // the node's flag (one byte of a 32-bit word); the first writer also
+  // appends the node to the gather's list (wlist[0] = count, entries from
+  // wlist[2]), so the list holds each reached node once
+  bool fresh = false;
+  if (active) {
+    const uint32_t bit = 1u << (8 * (int)(lin & 3));
+    fresh = (atomicOr(reinterpret_cast<uint32_t *>(touch) + (lin >> 2), bit) & bit) == 0u;
   }
+  const uint64_t fm = __ballot(fresh);
+  if (fm) {  // wave-uniform
+    const int first = __ffsll((unsigned long long)fm) - 1;
+    int at = 0;
+    if (lane == first) at = atomicAdd(&wlist[0], __popcll(fm));
+    at = __shfl(at, first);
+    if (fresh) wlist[2 + at + __popcll(fm & ((1ull << lane) - 1ull))] = (int)lin;
+  }
+  // the active offsets in order (s_list[rank] = offset)
+  const uint64_t act = __ballot(active);
+  const int nact = __popcll(act);
+  if (active) s_list[__popcll(act & ((1ull << lane) - 1ull))] = lane;
+  wave_lds_sync();
 
   const int n = in.n;
   const float inv_h = g.inv_cell_size;
-  float acc[NV];
+  const int q = lane & 3;  // a quarter of the cell's particles per lane
+  // rounds of 16 active offsets, four lanes per offset
+  for (int r0 = 0; r0 < nact; r0 += 16) {  // wave-uniform
+    const int idx = r0 + (lane >> 2);
+    const bool mine = idx < nact;
+    const int off = mine ? s_list[idx] : 0;
+    // r = frac - (NLO + s) lies in piece 3 - s of the wrapped node's spline
+    // (the same piece argument as span_eval): its coefficients are fixed per
+    // lane
+    float cf[3][4];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) acc[k] = 0.f;
-  for (int base = cs; base < ce; base += 64) {
-    const int cnt = min(64, ce - base);
-    if (lane < cnt) {
-      const int j = base + lane;
-      const int slot = part.sorted[j];
-      bool own = true;
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) cf[a][k] = 0.f;
+    int so[3] = {0, 0, 0};
+    if (mine) {
+      int64_t ln;
+      (void)offset_node(off, so, ln);
+      int ids[3];
+      node_ids<D>(g, ln, ids);
 #pragma unroll
       for (int a = 0; a < D; ++a) {
-        const float ta = cell_coord(in.x[a * n + slot], g.origin[a], inv_h);
-        const int ba = (int)ta;
-        own = own && ta >= 0.f && ba == c[a];
-        s_frac[a][lane] = __fsub_rn(ta, (float)ba);
-      }
-      s_own[lane] = own;
-      P2GRec r;
-      p2g_load<D, MODE>(prm, g, MODE == 0 && has_planar(prm), in, out, nullptr, n, slot, j, r);
-      s_m[lane] = r.m;
+        const int type = cubic_node_type(ids[a], g.num_cells[a], prm.node_types_built);
 #pragma unroll
-      for (int a = 0; a < D; ++a) s_v[a][lane] = r.v[a];
-      if constexpr (MODE == 0) {
-        s_A[0][lane] = r.A.xx; s_A[1][lane] = r.A.yy; s_A[2][lane] = r.A.zz;
-        s_A[3][lane] = r.A.xy; s_A[4][lane] = r.A.xz; s_A[5][lane] = r.A.yz;
-        s_A[6][lane] = r.f[0]; s_A[7][lane] = r.f[1]; s_A[8][lane] = r.f[2];  // wall force
+        for (int k = 0; k < 4; ++k) cf[a][k] = kCubic[type][3 - so[a]][k];
       }
     }
-    wave_lds_sync();
-    if (active) {
-      for (int k = 0; k < cnt; ++k) {
-        if (!s_own[k]) continue;  // deposited whole by p2g_stray
-        float N[3] = {1.f, 1.f, 1.f}, dN[3] = {0.f, 0.f, 0.f};
+    float acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.f;
+    for (int base = cs; base < ce; base += 64) {
+      const int cnt = min(64, ce - base);
+      wave_lds_sync();  // the stage of the previous round is read
+      if (lane < cnt) {
+        const int j = base + lane;
+        const int slot = part.sorted[j];
+        bool own = true;
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-          const float r = __fsub_rn(s_frac[a][k], (float)(NLO + s[a]));
-          N[a] = ((cf[a][3] * r + cf[a][2]) * r + cf[a][1]) * r + cf[a][0];
-          dN[a] = inv_h * ((3.f * cf[a][3] * r + 2.f * cf[a][2]) * r + cf[a][1]);
-          if (s[a] == 0 && !(r < 2.f)) N[a] = dN[a] = 0.f;  // frac = 1 - 2^-24
+          const float ta = cell_coord(in.x[a * n + slot], g.origin[a], inv_h);
+          const int ba = (int)ta;
+          own = own && ta >= 0.f && ba == c[a];
+          s_frac[a][lane] = __fsub_rn(ta, (float)ba);
         }
-        const float wk = (N[0] * N[1]) * N[2];
-        const float wm = wk * s_m[k];
-        const float gr[3] = {(dN[0] * N[1]) * N[2], (dN[1] * N[0]) * N[2], (dN[2] * N[0]) * N[1]};
+        s_own[lane] = own;
+        P2GRec r;
+        p2g_load<D, MODE>(prm, g, MODE == 0 && has_planar(prm), in, out, nullptr, n, slot, j, r);
+        s_m[lane] = r.m;
+#pragma unroll
+        for (int a = 0; a < D; ++a) s_v[a][lane] = r.v[a];
         if constexpr (MODE == 0) {
-          const Sym A{s_A[0][k], s_A[1][k], s_A[2][k], s_A[3][k], s_A[4][k], s_A[5][k]};
-          acc[0] += wm;
+          s_A[0][lane] = r.A.xx; s_A[1][lane] = r.A.yy; s_A[2][lane] = r.A.zz;
+          s_A[3][lane] = r.A.xy; s_A[4][lane] = r.A.xz; s_A[5][lane] = r.A.yz;
+          s_A[6][lane] = r.f[0]; s_A[7][lane] = r.f[1]; s_A[8][lane] = r.f[2];  // wall force
+        }
+      }
+      wave_lds_sync();
+      if (mine) {
+        for (int k = q; k < cnt; k += 4) {
+          if (!s_own[k]) continue;  // deposited whole by p2g_stray
+          float N[3] = {1.f, 1.f, 1.f}, dN[3] = {0.f, 0.f, 0.f};
 #pragma unroll
           for (int a = 0; a < D; ++a) {
-            acc[1 + a] += wm * s_v[a][k];
-            acc[1 + D + a] += wk * s_A[6 + a][k];
-#pragma unroll
-            for (int b = 0; b < D; ++b) acc[1 + D + a] += sym_at(A, a, b) * gr[b];
+            const float r = __fsub_rn(s_frac[a][k], (float)(NLO + so[a]));
+            N[a] = ((cf[a][3] * r + cf[a][2]) * r + cf[a][1]) * r + cf[a][0];
+            dN[a] = inv_h * ((3.f * cf[a][3] * r + 2.f * cf[a][2]) * r + cf[a][1]);
+            if (so[a] == 0 && !(r < 2.f)) N[a] = dN[a] = 0.f;  // frac = 1 - 2^-24
           }
-        } else {
+          const float wk = (N[0] * N[1]) * N[2];
+          const float wm = wk * s_m[k];
+          const float gr[3] = {(dN[0] * N[1]) * N[2], (dN[1] * N[0]) * N[2], (dN[2] * N[0]) * N[1]};
+          if constexpr (MODE == 0) {
+            const Sym A{s_A[0][k], s_A[1][k], s_A[2][k], s_A[3][k], s_A[4][k], s_A[5][k]};
+            acc[0] += wm;
 #pragma unroll
-          for (int a = 0; a < D; ++a) acc[a] += wm * s_v[a][k];
+            for (int a = 0; a < D; ++a) {
+              acc[1 + a] += wm * s_v[a][k];
+              acc[1 + D + a] += wk * s_A[6 + a][k];
+#pragma unroll
+              for (int b = 0; b < D; ++b) acc[1 + D + a] += sym_at(A, a, b) * gr[b];
+            }
+          } else {
+#pragma unroll
+            for (int a = 0; a < D; ++a) acc[a] += wm * s_v[a][k];
+          }
         }
       }
     }
-    wave_lds_sync();
-  }
+    // the four quarters of an offset, combined in a fixed order
 #pragma unroll
-  for (int k = 0; k < NV; ++k) rec[((int64_t)k * wcells + w) * 64 + lane] = active ? acc[k] : 0.f;
+    for (int k = 0; k < NV; ++k) {
+      acc[k] += __shfl_xor(acc[k], 1);
+      acc[k] += __shfl_xor(acc[k], 2);
+    }
+    if (mine && q == 0)
+#pragma unroll
+      for (int k = 0; k < NV; ++k) rec[((int64_t)k * wcells + w) * 64 + off] = acc[k];
+  }
 }
 
 // Wrap-around sums of typed cubic splines: one wave per cell that can wrap
-// (four per workgroup), lane = node offset (sx, sy, sz) of the cell's span. A
-// lane whose node lies outside the grid but whose linear index is in range
-// re-sums the cell's particles with the types of the node that index reaches
-// (shapefunction.cpp:96-112). The sums of a non-empty cell are stored per
-// (cell, lane) in `rec` ([NV][wcells][64], zero for the other lanes);
+// (four per workgroup). A node offset (sx, sy, sz) of the cell's span whose
+// node lies outside the grid but whose linear index is in range re-sums the
+// cell's particles with the types of the node that index reaches
+// (shapefunction.cpp:96-112); four lanes share such an offset, each summing a
+// quarter of the particles, and the quarters are combined in a fixed order.
+// The sums of a non-empty cell are stored per (cell, offset) in `rec`
+// ([NV][wcells][64]; only the offsets the gather reads are written);
 // wrap_gather_kernel adds them to their nodes in a fixed order, so the node
 // sums do not depend on scheduling. Empty cells write nothing (the gather
 // skips them by their cell ranges). The cell's particles are first loaded
 // into LDS by the wave (lane = particle, up to 64 per round), so the per-node
 // loops read LDS instead of walking a chain of dependent global loads.
 constexpr int kWrapCellsPerBlock = 4;
+constexpr int kWrapGatherBlocks = 1024;  // workgroups of the list gather (4 waves each)
 template <int D, int MODE>
 __global__ __launch_bounds__(64 * kWrapCellsPerBlock) void p2g_wrap_kernel(pmpm_params prm, pmpm_grid g,
                                                                            pmpm_particles in, pmpm_particles out,
                                                                            pmpm_spatial part, float *__restrict__ rec,
-                                                                           int wcells) {
+                                                                           uint8_t *__restrict__ touch,
+                                                                           int32_t *__restrict__ wlist, int wcells) {
   constexpr int NV = MODE == 0 ? 1 + 2 * D : D;
   constexpr int NS = MODE == 0 ? 9 : 1;  // staged -V sigma (6) and wall force (3)
   constexpr int WPB = kWrapCellsPerBlock;
   __shared__ float s_frac_all[WPB][3][64], s_v_all[WPB][3][64], s_m_all[WPB][64], s_A_all[WPB][NS][64];
-  __shared__ int s_own_all[WPB][64];
+  __shared__ int s_own_all[WPB][64], s_list_all[WPB][64];
   // every wave works on its own cell; only wave-level synchronisation below
   const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
   const int w = (int)blockIdx.x * WPB + wv;
@@ -1471,7 +1620,7 @@ __global__ __launch_bounds__(64 * kWrapCellsPerBlock) void p2g_wrap_kernel(pmpm_
   const int cs = part.cell_start[cell];
   if (ce <= cs) return;
   wrap_cell_sums<D, MODE, NV, NS>(prm, g, in, out, part, rec, wcells, w, c, cs, ce, lane, s_frac_all[wv],
-                                  s_v_all[wv], s_A_all[wv], s_m_all[wv], s_own_all[wv]);
+                                  s_v_all[wv], s_A_all[wv], s_m_all[wv], s_own_all[wv], s_list_all[wv], touch, wlist);
 }
 
 // Geometry of the P2G tiles for the grid kernel's gather.
@@ -1609,16 +1758,23 @@ __device__ inline bool gather_wrapped_lane(const pmpm_grid &g, const TileGeom &t
 // the wave sums them (the same lanes in the same order every run) and lane 0
 // adds the result to the node arrays, which hold nothing else but the (rare)
 // stray deposits at this point; the grid kernel adds the tile partials next.
-// Fixed order, no atomics.
+// Fixed order, no atomics. With typed cubic splines (touch != nullptr) only
+// the nodes the wrap kernel flagged are gathered (and their flags cleared):
+// the others receive nothing this pass.
 template <int D, int NV>
 __global__ __launch_bounds__(256) void wrap_gather_kernel(pmpm_grid g, TileGeom tg, const float *__restrict__ partials,
-                                                          const int32_t *__restrict__ flags, pmpm_nodes nodes) {
+                                                          const int32_t *__restrict__ flags, pmpm_nodes nodes,
+                                                          uint8_t *__restrict__ touch) {
   const int lane = (int)threadIdx.x & 63;
   const int t = (int)blockIdx.x * 4 + ((int)threadIdx.x >> 6);  // wave-uniform
   int ids[3] = {0, 0, 0};
   if (t >= wrap_target_count<D>(g) || !wrap_target<D>(g, t, ids)) return;
   const int nn = g.num_cells_total;
   const int i = ids[0] + g.num_cells[0] * (ids[1] + (D > 2 ? g.num_cells[1] * ids[2] : 0));
+  if (touch) {
+    if (!touch[i]) return;  // uniform
+    if (lane == 0) touch[i] = 0;
+  }
   float acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.f;
@@ -1642,6 +1798,55 @@ __global__ __launch_bounds__(256) void wrap_gather_kernel(pmpm_grid g, TileGeom 
   }
 }
 
+// The typed-cubic wrap-around sums, gathered for the nodes the wrap kernel
+// listed (each once): waves take list entries in turn, gather a node's
+// candidates as wrap_gather_kernel does (same candidates, same order, so the
+// node sums do not depend on the list order) and clear its flag; the last
+// workgroup to finish empties the list for the next pass.
+template <int D, int NV>
+__global__ __launch_bounds__(256) void wrap_gather_list_kernel(pmpm_grid g, TileGeom tg,
+                                                               const float *__restrict__ partials,
+                                                               const int32_t *__restrict__ flags, pmpm_nodes nodes,
+                                                               uint8_t *__restrict__ touch, int32_t *__restrict__ wlist) {
+  const int lane = (int)threadIdx.x & 63;
+  const int wv = (int)blockIdx.x * 4 + ((int)threadIdx.x >> 6), nw = (int)gridDim.x * 4;
+  const int nn = g.num_cells_total;
+  const int count = wlist[0];
+  for (int e = wv; e < count; e += nw) {  // wave-uniform
+    const int i = wlist[2 + e];
+    float acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.f;
+    const bool found = gather_wrapped_lane<D, NV>(g, tg, partials, flags, i, lane, acc);
+    if (lane == 0) touch[i] = 0;
+    if (!__ballot(found)) continue;  // uniform
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o);
+    if (lane != 0) continue;
+    if constexpr (NV == 1 + 2 * D) {
+      nodes.mass[i] += acc[0];
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        nodes.mom[a * nn + i] += acc[1 + a];
+        nodes.mom_nt[a * nn + i] += acc[1 + D + a];
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a < D; ++a) nodes.mom_nt[a * nn + i] += acc[a];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&wlist[1], 1) == (int)gridDim.x - 1) {
+      wlist[0] = 0;
+      wlist[1] = 0;
+    }
+  }
+}
+
 // Per node: cutoff + forward moment (finalize), boundary conditions
 // [bc_begin, bc_end) in list order, then the node velocities G2P reads.
 // remap_pass: mom_nt holds the MUSL re-mapped momentum; only kinematic
@@ -1649,10 +1854,13 @@ __global__ __launch_bounds__(256) void wrap_gather_kernel(pmpm_grid g, TileGeom 
 // clear_mom_nt: mom_nt is left at zero instead of the forward moment (the
 // MUSL step's first pass: the re-map accumulates into it next, and nothing
 // reads the forward moment but the velocity records written here).
+// clamp (optional): per node, the components a NodeDomain condition set to
+// zero (bit a: moment, bit 3 + a: forward moment; a slip clamp counts when
+// its input was not on the kept side), for the adjoint.
 template <int D>
 __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int finalize, int bc_begin, int bc_end,
                             int remap_pass, TileGeom tg, const float *__restrict__ partials,
-                            const int32_t *__restrict__ flags, int clear_mom_nt) {
+                            const int32_t *__restrict__ flags, int clear_mom_nt, uint8_t *__restrict__ clamp) {
   const int nn = g.num_cells_total;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nn) return;
@@ -1714,6 +1922,7 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
   }
   int ids[3];
   node_ids<D>(g, i, ids);
+  unsigned cbits = 0u;
   for (int k = bc_begin; k < bc_end; ++k) {
     const pmpm_bc &bc = prm.bcs[k];
     if (bc.kind == PMPM_BC_GRAVITY) {
@@ -1730,7 +1939,10 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
           if (bc.i[a] == 0) {
 #pragma unroll
             for (int b = 0; b < D; ++b) { P[b] = 0.f; Q[b] = 0.f; }
+            cbits |= 0x3fu;
           } else if (bc.i[a] == 1) {
+            if (!(P[a] > 0.f)) cbits |= 1u << a;
+            if (!(Q[a] > 0.f)) cbits |= 8u << a;
             P[a] = fmaxf(0.f, P[a]);
             Q[a] = fmaxf(0.f, Q[a]);
           }
@@ -1738,7 +1950,10 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
           if (bc.i[3 + a] == 0) {
 #pragma unroll
             for (int b = 0; b < D; ++b) { P[b] = 0.f; Q[b] = 0.f; }
+            cbits |= 0x3fu;
           } else if (bc.i[3 + a] == 1) {
+            if (!(P[a] < 0.f)) cbits |= 1u << a;
+            if (!(Q[a] < 0.f)) cbits |= 8u << a;
             P[a] = fminf(0.f, P[a]);
             Q[a] = fminf(0.f, Q[a]);
           }
@@ -1819,6 +2034,7 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
     nodes.mom[a * nn + i] = P[a];
     nodes.mom_nt[a * nn + i] = clear_mom_nt ? 0.f : Q[a];
   }
+  if (clamp) clamp[i] = (uint8_t)cbits;
   const bool live = m > prm.small_mass_cutoff;
   float vnt[3] = {0.f, 0.f, 0.f}, dv[3] = {0.f, 0.f, 0.f};
 #pragma unroll
@@ -2216,7 +2432,7 @@ static size_t tile_count(const TileGeom &tg) { return (size_t)tg.tiles[0] * tg.t
 template <int D, int SF>
 void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in, const pmpm_particles &out,
                 const pmpm_spatial &part, const pmpm_nodes &nodes, int mode, float *partials, int32_t *flags,
-                float *wrec, hipStream_t s,
+                float *wrec, uint8_t *wtouch, int32_t *wlist, hipStream_t s,
                 const float *rec = nullptr) {
   const TileGeom tg = tile_geom_of<D, SF>(g, 1);
   const dim3 grid((unsigned)tile_count(tg)), block(P2GGeom<D, SF>::THREADS);
@@ -2259,10 +2475,10 @@ void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles
     const dim3 wgrid((unsigned)ceil_div(wcells, kWrapCellsPerBlock)), wblock(64 * kWrapCellsPerBlock);
     if (mode == 0)
       PMPM_LAUNCH((p2g_wrap_kernel<D, 0>), (::pmpm::kname("p2g_wrap_kernel", {(long)(D), (long)(0)})), wgrid, wblock,
-                  0, s, prm, g, in, out, part, wrec, wcells);
+                  0, s, prm, g, in, out, part, wrec, wtouch, wlist, wcells);
     else
       PMPM_LAUNCH((p2g_wrap_kernel<D, 1>), (::pmpm::kname("p2g_wrap_kernel", {(long)(D), (long)(1)})), wgrid, wblock,
-                  0, s, prm, g, in, out, part, wrec, wcells);
+                  0, s, prm, g, in, out, part, wrec, wtouch, wlist, wcells);
   }
 }
 template <int D, int SF>
@@ -2288,11 +2504,16 @@ int launch_grid(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, c
   TileGeom tg = tile_geom(prm.dim, prm.shape_fn, g, gather);
   const float *partials = gather ? ctx->partials : nullptr;
   const int32_t *flags = gather ? ctx->tile_flags : nullptr;
+  // the clamp record of a pass that applies node conditions (not the MUSL
+  // re-map pass), for pmpm_adjoint_step
+  uint8_t *clamp = nullptr;
+  if (!remap && b1 > b0)
+    clamp = nodes.clamp ? nodes.clamp : (ctx ? reinterpret_cast<uint8_t *>(ctx->clamp_mask) : nullptr);
 
   switch (prm.dim) {
-    case 1: PMPM_LAUNCH(grid_kernel<1>, (::pmpm::kname("grid_kernel", {(long)(1)}) + " gather " + std::to_string(gather)), dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt); break;
-    case 2: PMPM_LAUNCH(grid_kernel<2>, (::pmpm::kname("grid_kernel", {(long)(2)}) + " gather " + std::to_string(gather)), dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt); break;
-    case 3: PMPM_LAUNCH(grid_kernel<3>, (::pmpm::kname("grid_kernel", {(long)(3)}) + " gather " + std::to_string(gather)), dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt); break;
+    case 1: PMPM_LAUNCH(grid_kernel<1>, (::pmpm::kname("grid_kernel", {(long)(1)}) + " gather " + std::to_string(gather)), dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt, clamp); break;
+    case 2: PMPM_LAUNCH(grid_kernel<2>, (::pmpm::kname("grid_kernel", {(long)(2)}) + " gather " + std::to_string(gather)), dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt, clamp); break;
+    case 3: PMPM_LAUNCH(grid_kernel<3>, (::pmpm::kname("grid_kernel", {(long)(3)}) + " gather " + std::to_string(gather)), dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt, clamp); break;
     default: return PMPM_ERR_INVALID;
   }
   PMPM_LAUNCH_CHECK();
@@ -2332,18 +2553,46 @@ static int reserve_tiles(pmpm_context *ctx, const pmpm_params &prm, const pmpm_g
   const size_t tiles = tile_count(tg);
   const size_t ntn = (size_t)tg.N[0] * tg.N[1] * tg.N[2];
   int rc = ensure_partials(ctx, tiles * tg.nv * ntn, tiles);
-  if (rc != PMPM_OK || wrap_mode(prm, g) != 2) return rc;
+  if (rc != PMPM_OK) return rc;
+  // the grid pass's clamp record (one byte per node, in words of 4)
+  if ((rc = ensure_buffer(ctx->clamp_mask, ctx->clamp_mask_cap, ((size_t)g.num_cells_total + 3) / 4)) != PMPM_OK)
+    return rc;
+  if (wrap_mode(prm, g) != 2) return rc;
   const size_t wcells = (size_t)(prm.dim == 2 ? wrap_cell_count<2>(g) : wrap_cell_count<3>(g));
-  return ensure_buffer(ctx->wrap_rec, ctx->wrap_rec_cap, (size_t)(1 + 2 * prm.dim) * wcells * 64);
+  if ((rc = ensure_buffer(ctx->wrap_rec, ctx->wrap_rec_cap, (size_t)(1 + 2 * prm.dim) * wcells * 64)) != PMPM_OK)
+    return rc;
+  // node flags, one byte per node, cleared once here (the gather clears
+  // what it visits; a stale flag would only cost the gather a visit)
+  const size_t words = ((size_t)g.num_cells_total + 3) / 4;
+  if (words > ctx->wrap_touch_cap) {
+    if ((rc = ensure_buffer(ctx->wrap_touch, ctx->wrap_touch_cap, words)) != PMPM_OK) return rc;
+    PMPM_HIP_CHECK(hipMemset(ctx->wrap_touch, 0, words * sizeof(float)));
+  }
+  // the gather's list: count, finished workgroups, one entry per node
+  const size_t lwords = (size_t)g.num_cells_total + 2;
+  if (lwords > ctx->wrap_list_cap) {
+    float *lb = reinterpret_cast<float *>(ctx->wrap_list);
+    size_t cap = ctx->wrap_list_cap;
+    if ((rc = ensure_buffer(lb, cap, lwords)) != PMPM_OK) return rc;
+    ctx->wrap_list = reinterpret_cast<int32_t *>(lb);
+    ctx->wrap_list_cap = cap;
+    PMPM_HIP_CHECK(hipMemset(ctx->wrap_list, 0, 2 * sizeof(int32_t)));
+  }
+  return PMPM_OK;
 }
+static uint8_t *wrap_touch(pmpm_context *ctx) { return reinterpret_cast<uint8_t *>(ctx->wrap_touch); }
 
 void p2g_workspace(const pmpm_params &prm, const pmpm_grid &g, size_t *partial_floats, size_t *tiles,
-                   size_t *wrap_floats, size_t *wrap_cells) {
+                   size_t *wrap_floats, size_t *wrap_cells, size_t *wrap_touch_words, size_t *wrap_list_words,
+                   size_t *clamp_words) {
   const TileGeom tg = tile_geom(prm.dim, prm.shape_fn, g, 1);
   *tiles = tile_count(tg);
   *partial_floats = *tiles * tg.nv * (size_t)tg.N[0] * tg.N[1] * tg.N[2];
   *wrap_cells = (size_t)(prm.dim == 3 ? wrap_cell_count<3>(g) : (prm.dim == 2 ? wrap_cell_count<2>(g) : 0));
   *wrap_floats = wrap_mode(prm, g) == 2 ? (size_t)(1 + 2 * prm.dim) * *wrap_cells * 64 : 0;
+  *wrap_touch_words = wrap_mode(prm, g) == 2 ? ((size_t)g.num_cells_total + 3) / 4 : 0;
+  *wrap_list_words = wrap_mode(prm, g) == 2 ? (size_t)g.num_cells_total + 2 : 0;
+  *clamp_words = ((size_t)g.num_cells_total + 3) / 4;
 }
 
 // The wrap-around sums of a P2G pass (gather 1) or MUSL re-map (gather 2)
@@ -2360,21 +2609,43 @@ static int launch_wrap_gather(pmpm_context *ctx, const pmpm_params &prm, const p
   tg.cend = part.cell_end;
   const int targets = prm.dim == 2 ? wrap_target_count<2>(g) : wrap_target_count<3>(g);
   if (targets <= 0) return PMPM_OK;
+  if (mode == 2) {  // typed cubic: the nodes the wrap kernel listed
+    const dim3 lgrid((unsigned)std::min<int64_t>(kWrapGatherBlocks, ceil_div(targets, 4))), block(256);
+    uint8_t *touch = wrap_touch(ctx);
+    if (prm.dim == 2) {
+      if (gather == 1)
+        PMPM_LAUNCH((wrap_gather_list_kernel<2, 5>), (std::string("wrap_gather_list_kernel<2, 5>")), lgrid, block, 0,
+                    s, g, tg, ctx->partials, ctx->tile_flags, nodes, touch, ctx->wrap_list);
+      else
+        PMPM_LAUNCH((wrap_gather_list_kernel<2, 2>), (std::string("wrap_gather_list_kernel<2, 2>")), lgrid, block, 0,
+                    s, g, tg, ctx->partials, ctx->tile_flags, nodes, touch, ctx->wrap_list);
+    } else {
+      if (gather == 1)
+        PMPM_LAUNCH((wrap_gather_list_kernel<3, 7>), (std::string("wrap_gather_list_kernel<3, 7>")), lgrid, block, 0,
+                    s, g, tg, ctx->partials, ctx->tile_flags, nodes, touch, ctx->wrap_list);
+      else
+        PMPM_LAUNCH((wrap_gather_list_kernel<3, 3>), (std::string("wrap_gather_list_kernel<3, 3>")), lgrid, block, 0,
+                    s, g, tg, ctx->partials, ctx->tile_flags, nodes, touch, ctx->wrap_list);
+    }
+    PMPM_LAUNCH_CHECK();
+    return PMPM_OK;
+  }
+  uint8_t *touch = nullptr;
   const dim3 grid((unsigned)ceil_div(targets, 4)), block(256);  // one wave per target node
   if (prm.dim == 2) {
     if (gather == 1)
       PMPM_LAUNCH((wrap_gather_kernel<2, 5>), (std::string("wrap_gather_kernel<2, 5>")), grid, block, 0, s, g, tg,
-                  ctx->partials, ctx->tile_flags, nodes);
+                  ctx->partials, ctx->tile_flags, nodes, touch);
     else
       PMPM_LAUNCH((wrap_gather_kernel<2, 2>), (std::string("wrap_gather_kernel<2, 2>")), grid, block, 0, s, g, tg,
-                  ctx->partials, ctx->tile_flags, nodes);
+                  ctx->partials, ctx->tile_flags, nodes, touch);
   } else {
     if (gather == 1)
       PMPM_LAUNCH((wrap_gather_kernel<3, 7>), (std::string("wrap_gather_kernel<3, 7>")), grid, block, 0, s, g, tg,
-                  ctx->partials, ctx->tile_flags, nodes);
+                  ctx->partials, ctx->tile_flags, nodes, touch);
     else
       PMPM_LAUNCH((wrap_gather_kernel<3, 3>), (std::string("wrap_gather_kernel<3, 3>")), grid, block, 0, s, g, tg,
-                  ctx->partials, ctx->tile_flags, nodes);
+                  ctx->partials, ctx->tile_flags, nodes, touch);
   }
   PMPM_LAUNCH_CHECK();
   return PMPM_OK;
@@ -2390,7 +2661,7 @@ int do_p2g(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const 
   if (rc != PMPM_OK) return rc;
   if (!zeroed && (rc = do_zero_nodes(prm, g, nodes, s)) != PMPM_OK) return rc;
   PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, p, p, part, nodes, 0, ctx->partials, ctx->tile_flags,
-                    ctx->wrap_rec, s);
+                    ctx->wrap_rec, wrap_touch(ctx), ctx->wrap_list, s);
   PMPM_LAUNCH_CHECK();
   return launch_wrap_gather(ctx, prm, g, part, nodes, 1, s);
 }
@@ -2437,7 +2708,7 @@ int do_musl_map(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, c
   // for the re-map's wrap-around atomics first
   if (!zeroed && (rc = zero_fill(nodes.mom_nt, (int64_t)g.num_cells_total * prm.dim, s)) != PMPM_OK) return rc;
   PMPM_DISPATCH_DSF(prm.dim, prm.shape_fn, launch_p2g, prm, g, in, out, part, nodes, 1, ctx->partials,
-                    ctx->tile_flags, ctx->wrap_rec, s);
+                    ctx->tile_flags, ctx->wrap_rec, wrap_touch(ctx), ctx->wrap_list, s);
   PMPM_LAUNCH_CHECK();
   return launch_wrap_gather(ctx, prm, g, part, nodes, 2, s);
 }
@@ -2785,12 +3056,12 @@ __device__ inline void store_node_adj(float *__restrict__ nbar, int node, float 
 
 // Per node: adjoints of the node velocities (scattered in the tile partials)
 // -> adjoints of mass, moment and force. Boundary conditions in reverse list
-// order with the forward's active sets (read from the post-condition
-// values: a slip clamp passed a component iff it is nonzero on the kept side).
+// order with the forward's active sets, which the forward grid pass recorded
+// per node in `clamp` (components a NodeDomain condition set to zero).
 template <int D>
 __global__ void grid_adjoint_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, TileGeom tg,
                                     const float *__restrict__ partials, const int32_t *__restrict__ flags,
-                                    float *__restrict__ nbar) {
+                                    const uint8_t *__restrict__ clamp, float *__restrict__ nbar) {
   const int nn = g.num_cells_total;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nn) return;
@@ -2801,13 +3072,11 @@ __global__ void grid_adjoint_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nod
   const float m = nodes.mass[i];
   float mb = 0.f, Pb[3] = {0.f, 0.f, 0.f}, Qb[3] = {0.f, 0.f, 0.f}, fb[3] = {0.f, 0.f, 0.f};
   if (m > prm.small_mass_cutoff) {
-    float P[3] = {0.f, 0.f, 0.f}, Q[3] = {0.f, 0.f, 0.f};
     float vnt[3] = {0.f, 0.f, 0.f}, dvn[3] = {0.f, 0.f, 0.f};
     load_node_vel<D, true>(nodes.vel, i, vnt, dvn);
+    const unsigned cm = clamp[i];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      P[a] = nodes.mom[a * nn + i];
-      Q[a] = nodes.mom_nt[a * nn + i];
       const float vb = acc[a], db = acc[D + a];
       Qb[a] = (vb + db) / m;
       Pb[a] = -db / m;
@@ -2816,31 +3085,17 @@ __global__ void grid_adjoint_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nod
     for (int k = prm.num_bcs - 1; k >= 0; --k) {
       const pmpm_bc &bc = prm.bcs[k];
       if (bc.kind == PMPM_BC_NODE_DOMAIN) {
-        if (m <= 0.00001f) continue;
-        for (int a = D - 1; a >= 0; --a) {
-          int mode = -1;
-          bool low = false;
-          if (ids[a] < 1) { mode = bc.i[a]; low = true; }
-          else if (ids[a] >= g.num_cells[a] - 1) mode = bc.i[3 + a];
-          if (mode == 0) {
+        // (one NodeDomain per list: pmpm_adjoint_step checks)
 #pragma unroll
-            for (int b = 0; b < D; ++b) { Pb[b] = 0.f; Qb[b] = 0.f; }
-          } else if (mode == 1) {
-            if (!(low ? P[a] > 0.f : P[a] < 0.f)) Pb[a] = 0.f;
-            if (!(low ? Q[a] > 0.f : Q[a] < 0.f)) Qb[a] = 0.f;
-          }
+        for (int a = 0; a < D; ++a) {
+          if (cm & (1u << a)) Pb[a] = 0.f;
+          if (cm & (8u << a)) Qb[a] = 0.f;
         }
       } else if (bc.kind == PMPM_BC_GRAVITY) {
         if (m <= 0.000000001f) continue;
-        // Q += g m dt: its adjoint, then its undo, so that a slip clamp
-        // earlier in the list is tested on the value it saw. The increment is
-        // rounded as grid_kernel rounds it, so a clamped component that
-        // gravity moved off zero returns to exactly zero.
+        // Q += g m dt
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-          mb += bc.f[a] * prm.dt * Qb[a];
-          Q[a] = __fsub_rn(Q[a], __fmul_rn(__fmul_rn(bc.f[a], m), prm.dt));
-        }
+        for (int a = 0; a < D; ++a) mb += bc.f[a] * prm.dt * Qb[a];
       }
     }
     // Q = P + dt f
@@ -3165,10 +3420,13 @@ extern "C" int pmpm_adjoint_step(pmpm_context *ctx, const pmpm_params *prm, cons
     const int kind = prm->materials[k].kind;
     if (kind != PMPM_MAT_LINEAR_ELASTIC && kind != PMPM_MAT_DRUCKER_PRAGER) return PMPM_ERR_UNSUPPORTED;
   }
+  int walls = 0;
   for (int k = 0; k < prm->num_bcs; ++k) {
     const int kind = prm->bcs[k].kind;
     if (kind != PMPM_BC_GRAVITY && kind != PMPM_BC_NODE_DOMAIN) return PMPM_ERR_UNSUPPORTED;
+    walls += kind == PMPM_BC_NODE_DOMAIN;
   }
+  if (walls > 1) return PMPM_ERR_UNSUPPORTED;  // the clamp record holds one condition's set
   hipStream_t s = (hipStream_t)stream;
   const pmpm_grid &g = *grid;
   const int D = prm->dim;
@@ -3199,15 +3457,16 @@ extern "C" int pmpm_adjoint_step(pmpm_context *ctx, const pmpm_params *prm, cons
     PMPM_LAUNCH_CHECK();
   }
   PMPM_DISPATCH_DSF(D, prm->shape_fn, launch_p2g, *prm, g, *state, *state, *part, *nodes, 2, ctx->partials,
-                    ctx->tile_flags, nullptr, s, ctx->adj_rec);
+                    ctx->tile_flags, nullptr, nullptr, nullptr, s, ctx->adj_rec);
   PMPM_LAUNCH_CHECK();
-  // grid adjoint
+  // grid adjoint, with the forward grid pass's clamp record
+  const uint8_t *clamp = nodes->clamp ? nodes->clamp : reinterpret_cast<const uint8_t *>(ctx->clamp_mask);
   const TileGeom tg = tile_geom(D, prm->shape_fn, g, 3);
   const unsigned nblocks = (unsigned)ceil_div(g.num_cells_total, 256);
   switch (D) {
-    case 1: PMPM_LAUNCH(grid_adjoint_kernel<1>, (::pmpm::kname("grid_adjoint_kernel", {(long)(1)})), dim3(nblocks), dim3(256), 0, s, *prm, g, *nodes, tg, ctx->partials, ctx->tile_flags, ctx->adj_nodes); break;
-    case 2: PMPM_LAUNCH(grid_adjoint_kernel<2>, (::pmpm::kname("grid_adjoint_kernel", {(long)(2)})), dim3(nblocks), dim3(256), 0, s, *prm, g, *nodes, tg, ctx->partials, ctx->tile_flags, ctx->adj_nodes); break;
-    default: PMPM_LAUNCH(grid_adjoint_kernel<3>, (::pmpm::kname("grid_adjoint_kernel", {(long)(3)})), dim3(nblocks), dim3(256), 0, s, *prm, g, *nodes, tg, ctx->partials, ctx->tile_flags, ctx->adj_nodes); break;
+    case 1: PMPM_LAUNCH(grid_adjoint_kernel<1>, (::pmpm::kname("grid_adjoint_kernel", {(long)(1)})), dim3(nblocks), dim3(256), 0, s, *prm, g, *nodes, tg, ctx->partials, ctx->tile_flags, clamp, ctx->adj_nodes); break;
+    case 2: PMPM_LAUNCH(grid_adjoint_kernel<2>, (::pmpm::kname("grid_adjoint_kernel", {(long)(2)})), dim3(nblocks), dim3(256), 0, s, *prm, g, *nodes, tg, ctx->partials, ctx->tile_flags, clamp, ctx->adj_nodes); break;
+    default: PMPM_LAUNCH(grid_adjoint_kernel<3>, (::pmpm::kname("grid_adjoint_kernel", {(long)(3)})), dim3(nblocks), dim3(256), 0, s, *prm, g, *nodes, tg, ctx->partials, ctx->tile_flags, clamp, ctx->adj_nodes); break;
   }
   PMPM_LAUNCH_CHECK();
   // P2G adjoint

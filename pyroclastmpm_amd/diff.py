@@ -64,7 +64,10 @@ class StepBuffers:
         f = lambda m: torch.zeros(m, dtype=torch.float32, device=device)
         self.nd = {"mass": f(nn), "mom": f(dim * nn), "mom_nt": f(dim * nn),
                    "vel": f(capi.node_vel_stride(dim) * nn)}
-        self.nodes = capi.Nodes(*[self.nd[k].data_ptr() for k in ["mass", "mom", "mom_nt", "vel"]])
+        # the grid pass's clamp record, kept with the step's nodes for the
+        # adjoint of that step
+        self.nd["clamp"] = torch.zeros(nn, dtype=torch.uint8, device=device)
+        self.nodes = capi.Nodes(*[self.nd[k].data_ptr() for k in ["mass", "mom", "mom_nt", "vel", "clamp"]])
 
 
 class AdjointSet:

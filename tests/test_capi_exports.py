@@ -82,3 +82,17 @@ def test_package_import_has_reference_api():
     assert pm3.global_dimension == 3 and pm2.global_dimension == 2
     with pytest.raises(ValueError):
         pm3.set_global_shapefunction("quartic")
+
+
+def test_reference_import_path():
+    """`import pyroclastmpm.MPM3D as pm` and `from pyroclastmpm import USL`
+    (the reference's module path) reach the same classes."""
+    import pyroclastmpm
+    import pyroclastmpm.MPM2D as pm2
+    import pyroclastmpm_amd.MPM2D as amd2
+    from pyroclastmpm import USL, global_dimension
+    from pyroclastmpm_amd import MPM3D
+
+    assert pm2.USL is amd2.USL and pm2.global_dimension == 2
+    assert USL is MPM3D.USL and global_dimension == 3
+    assert pyroclastmpm.MPM1D.global_dimension == 1

@@ -38,14 +38,12 @@ def test_ensemble_matches_scenes_run_alone(graph):
         assert torch.allclose(xs[i], x, rtol=0, atol=1e-6), (i, float((xs[i] - x).abs().max()))
         assert float((x - x0s[i]).abs().max()) > 1e-6, "the column must move"
         ((x - x0s[i]) ** 2).mean().backward()
-        # The wrap-around deposits at the domain boundary add with float
-        # atomics, so two runs of one scene differ in the last bits of v and
-        # stress. The cohesion gradient collects only from particles at the
-        # Drucker-Prager apex (c = 0: the tension-free surface), whose branch
-        # follows those bits: a scene run alone twice differs there by about
-        # 1%. phi and E collect from every yielding particle.
+        # Every node sum is formed in a fixed order (the wrap-around deposits
+        # included), so a scene gives the same bits batched or alone; the
+        # cohesion gradient, which collects only from particles at the
+        # Drucker-Prager apex (c = 0) and so follows the last bits of the
+        # stress, is held to the same bar as phi and E.
         rel = [abs(float(g_ens[i, j]) - float(t.grad[j])) / abs(float(t.grad[j])) for j in range(3)]
         print(f"scene {i}: n={m.n} max |dx|={float((xs[i] - x).abs().max()):.3g} grad rel diff={rel}", flush=True)
-        assert rel[0] <= 1e-4 and rel[2] <= 1e-4, (i, g_ens[i], t.grad)
-        assert rel[1] <= 5e-2, (i, g_ens[i], t.grad)
+        assert max(rel) <= 1e-5, (i, g_ens[i], t.grad)
         assert all(float(v) != 0.0 for v in t.grad), t.grad
