@@ -34,7 +34,7 @@ struct pmpm_context {
   float *wrap_touch = nullptr;
   size_t wrap_touch_cap = 0;
   // the nodes the typed-cubic wrap-around sums reach this pass: [0] count,
-  // [1] finished gather workgroups, [2..] node indices (each once)
+  // [1] unused, [2..] node indices (each once)
   int32_t *wrap_list = nullptr;
   size_t wrap_list_cap = 0;
   // a byte per node: the components the last grid pass's NodeDomain

@@ -1124,8 +1124,14 @@ __global__ __launch_bounds__((P2GGeom<D, SF>::THREADS)) __attribute__((amdgpu_wa
 // each lane's node into the wave's own LDS node block; the four blocks
 // overlap in z and are summed into the tile slice at the end.
 struct MfmaGeom {
-  static constexpr int CELLS = 16;                      // cells per wave (x)
+  // The tile: CELLS x 1 x 4 cells, one z layer per wave. Eight cells per
+  // wave keep a workgroup short enough that the chip's workgroup slots are
+  // refilled evenly (a tile is the unit of work, and a column whose x
+  // extent ends inside a tile leaves it partly empty).
+  static constexpr int CELLS = 8;                       // cells per wave (x)
   static constexpr int BX = CELLS + 3, BY = 4, BZ = 4;  // a wave's node block
+  static constexpr int TZ = 4, NZ = TZ + 3;             // tile layers, slice node layers
+  static constexpr int NTN = BX * BY * NZ;              // nodes of one partial slice
   // x pitch of the block in LDS: the 64 lanes of a flush, offsets
   // pk + 20 s1 + 80 s2, fall into 64 distinct banks (19 would pair some up)
   static constexpr int BXP = BX + 1;
@@ -1147,15 +1153,16 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 // (planar_force) is staged too and w f joins the force product through its
 // A operand (Nx f_a next to dNx A_a.x, both against Ny Nz).
 template <int SF, bool REMAP = false, bool FEXT = false>
-__global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles in,
+__global__ __launch_bounds__(256, 4) void p2g_mfma_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles in,
                                                        pmpm_particles out, pmpm_spatial part, pmpm_nodes nodes,
                                                        int tiles_x, int tiles_y, float *__restrict__ partials,
                                                        int32_t *__restrict__ flags) {
   using G = P2GGeom<3, SF>;
   using M = MfmaGeom;
-  static_assert(G::TX == M::CELLS && G::TZ == 4 && G::NX == M::BX && G::NY == M::BY && G::SPAN == 4,
-                "the tile of p2g_kernel<3, SF, 0>");
-  constexpr int NV = REMAP ? 3 : 7, NTN = G::NTN, NLO = G::NLO;
+  static_assert(G::SPAN == 4 && G::NLO == -1, "cubic spans");
+  static_assert((M::BX - 1) / M::CELLS + 1 <= kCoverX && (M::NZ - 1) / M::TZ + 1 <= kCoverZ,
+                "the grid kernel's gather visits at most kCover tiles per axis");
+  constexpr int NV = REMAP ? 3 : 7, NTN = M::NTN, NLO = G::NLO;
   static_assert(NV == p2g_nv<3, REMAP ? 1 : 0>(), "slice fields of the matching p2g_kernel mode");
   static_assert(!(REMAP && FEXT), "the re-map carries no force");
   constexpr int RS = M::RS + (FEXT ? 4 : 0);  // record pitch (35..37: wall force)
@@ -1365,10 +1372,13 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
     }
     stage_raw(lx, cur);
     wave_lds_sync();
+    // the products in two branch-free halves (the records past the chunk's
+    // particles are zero), so the record reads of later products are issued
+    // ahead of the earlier ones
     const int wn = min(M::CH, ce_of(lx) - w0);
 #pragma unroll
     for (int k0 = 0; k0 < M::CH; k0 += 4) {
-      if (k0 >= wn) break;  // wave-uniform
+      if (k0 == M::CH / 2 && wn <= M::CH / 2) break;  // wave-uniform
       const float *sp = st + (k0 + pk) * RS;
       const float Nx = sp[ex], dNx = sp[ex + 1], Ny = sp[ex + 2], dNy = sp[ex + 3];
       const float Nz = sp[ez + 4], dNz = sp[ez + 5], Ff = sp[ez + 6];
@@ -1598,7 +1608,6 @@ __device__ inline void wrap_cell_sums(const pmpm_params &prm, const pmpm_grid &g
 // into LDS by the wave (lane = particle, up to 64 per round), so the per-node
 // loops read LDS instead of walking a chain of dependent global loads.
 constexpr int kWrapCellsPerBlock = 4;
-constexpr int kWrapGatherBlocks = 1024;  // workgroups of the list gather (4 waves each)
 template <int D, int MODE>
 __global__ __launch_bounds__(64 * kWrapCellsPerBlock) void p2g_wrap_kernel(pmpm_params prm, pmpm_grid g,
                                                                            pmpm_particles in, pmpm_particles out,
@@ -1798,11 +1807,86 @@ __global__ __launch_bounds__(256) void wrap_gather_kernel(pmpm_grid g, TileGeom 
   }
 }
 
+// gather_wrapped_lane for typed cubic records on grids of at least 3 cells
+// per axis, with one round of loads: there the outside nodes whose linear
+// index is `node` number at most four (at most two x images, x = tx or
+// tx -/+ nx, and for each at most two y images), so they are found first by
+// index arithmetic alone; then every lane issues the cell-range and record
+// loads of its (image, offset) pairs together, and an empty cell's (stale)
+// record is dropped by a select. The same pairs and summation order as
+// gather_wrapped_lane.
+template <int D, int NV>
+__device__ inline bool gather_wrapped_typed(const pmpm_grid &g, const TileGeom &tg, int node, int lane, float *acc) {
+  const int nx = g.num_cells[0], ny = g.num_cells[1], nz = D > 2 ? g.num_cells[2] : 1;
+  constexpr int lo = -1, hi = 2;  // cubic spans: node offsets -1 .. +2
+  const int tx = node % nx, rest = node / nx;
+  int img[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, nimg = 0;
+#pragma unroll
+  for (int kx = -1; kx <= 1; ++kx) {
+    const int bx = tx + kx * nx;
+    if (bx < lo || bx > nx - 1 + hi) continue;
+    const int q = rest - kx;
+    const int qy = ((q % ny) + ny) % ny, qz0 = (q - qy) / ny;
+#pragma unroll
+    for (int ky = -1; ky <= 1; ++ky) {
+      const int by = qy + ky * ny;
+      if (by < lo || by > ny - 1 + hi) continue;
+      const int qz = qz0 - ky;
+      if (D == 2 && qz != 0) continue;
+      if (D == 3 && (qz < lo || qz > nz - 1 + hi)) continue;
+      if (bx >= 0 && bx < nx && by >= 0 && by < ny && (D < 3 || (qz >= 0 && qz < nz))) continue;  // the node itself
+      // (slot by selects: no indexed register array)
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (nimg == k) {
+          img[k][0] = bx;
+          img[k][1] = by;
+          img[k][2] = D > 2 ? qz : 0;
+        }
+      ++nimg;
+    }
+  }
+  const int sx = lane & 3, sy = (lane >> 2) & 3, sz = lane >> 4;
+  const bool my_lane = D > 2 || lane < 16;
+  int cl[4], wi[4];
+  bool ok[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int c[3] = {img[k][0] + 1 - sx, img[k][1] + 1 - sy, D > 2 ? img[k][2] + 1 - sz : 0};
+    ok[k] = k < nimg && my_lane && c[0] >= 0 && c[0] < nx && c[1] >= 0 && c[1] < ny &&
+            (D < 3 || (c[2] >= 0 && c[2] < nz));
+    wi[k] = ok[k] ? wrap_index<D>(g, c) : -1;
+    ok[k] = ok[k] && wi[k] >= 0;
+    cl[k] = ok[k] ? c[0] + nx * (c[1] + ny * c[2]) : 0;
+    if (!ok[k]) wi[k] = 0;
+  }
+  const int rl = sx + 4 * (sy + 4 * sz);
+  int cs[4], ce[4];
+  float r[4][NV];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    cs[k] = tg.cstart[cl[k]];
+    ce[k] = tg.cend[cl[k]];
+#pragma unroll
+    for (int f = 0; f < NV; ++f) r[k][f] = tg.wrec[((int64_t)f * tg.wcells + wi[k]) * 64 + rl];
+  }
+  bool found = false;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const bool use = ok[k] && ce[k] > cs[k];
+#pragma unroll
+    for (int f = 0; f < NV; ++f) acc[f] += use ? r[k][f] : 0.f;
+    found = found || use;
+  }
+  return found;
+}
+
 // The typed-cubic wrap-around sums, gathered for the nodes the wrap kernel
-// listed (each once): waves take list entries in turn, gather a node's
+// listed (each once; the grid pass that follows empties the list): wave e
+// takes list entry e (waves past the count exit after one load), gathers the
+// node's
 // candidates as wrap_gather_kernel does (same candidates, same order, so the
-// node sums do not depend on the list order) and clear its flag; the last
-// workgroup to finish empties the list for the next pass.
+// node sums do not depend on the list order) and clears its flag.
 template <int D, int NV>
 __global__ __launch_bounds__(256) void wrap_gather_list_kernel(pmpm_grid g, TileGeom tg,
                                                                const float *__restrict__ partials,
@@ -1811,13 +1895,16 @@ __global__ __launch_bounds__(256) void wrap_gather_list_kernel(pmpm_grid g, Tile
   const int lane = (int)threadIdx.x & 63;
   const int wv = (int)blockIdx.x * 4 + ((int)threadIdx.x >> 6), nw = (int)gridDim.x * 4;
   const int nn = g.num_cells_total;
-  const int count = wlist[0];
+  const int count = __builtin_amdgcn_readfirstlane(wlist[0]);
+  // grids below 3 cells on an axis have more images per node: generic path
+  const bool small = g.num_cells[0] < 3 || g.num_cells[1] < 3 || (D > 2 && g.num_cells[2] < 3);
   for (int e = wv; e < count; e += nw) {  // wave-uniform
-    const int i = wlist[2 + e];
+    const int i = __builtin_amdgcn_readfirstlane(wlist[2 + e]);
     float acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) acc[k] = 0.f;
-    const bool found = gather_wrapped_lane<D, NV>(g, tg, partials, flags, i, lane, acc);
+    const bool found = small ? gather_wrapped_lane<D, NV>(g, tg, partials, flags, i, lane, acc)
+                             : gather_wrapped_typed<D, NV>(g, tg, i, lane, acc);
     if (lane == 0) touch[i] = 0;
     if (!__ballot(found)) continue;  // uniform
 #pragma unroll
@@ -1837,14 +1924,6 @@ __global__ __launch_bounds__(256) void wrap_gather_list_kernel(pmpm_grid g, Tile
       for (int a = 0; a < D; ++a) nodes.mom_nt[a * nn + i] += acc[a];
     }
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    if (atomicAdd(&wlist[1], 1) == (int)gridDim.x - 1) {
-      wlist[0] = 0;
-      wlist[1] = 0;
-    }
-  }
 }
 
 // Per node: cutoff + forward moment (finalize), boundary conditions
@@ -1856,13 +1935,17 @@ __global__ __launch_bounds__(256) void wrap_gather_list_kernel(pmpm_grid g, Tile
 // reads the forward moment but the velocity records written here).
 // clamp (optional): per node, the components a NodeDomain condition set to
 // zero (bit a: moment, bit 3 + a: forward moment; a slip clamp counts when
-// its input was not on the kept side), for the adjoint.
+// its input was not on the kept side), for the adjoint. wlist (optional): the
+// typed-cubic wrap-around node list, emptied here for the next pass (its
+// gather ran before this pass).
 template <int D>
 __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int finalize, int bc_begin, int bc_end,
                             int remap_pass, TileGeom tg, const float *__restrict__ partials,
-                            const int32_t *__restrict__ flags, int clear_mom_nt, uint8_t *__restrict__ clamp) {
+                            const int32_t *__restrict__ flags, int clear_mom_nt, uint8_t *__restrict__ clamp,
+                            int32_t *__restrict__ wlist) {
   const int nn = g.num_cells_total;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (wlist && i == 0) wlist[0] = 0;  // the wrap-around list of this pass is consumed
   if (i >= nn) return;
   float m = nodes.mass[i];
   float P[3] = {0.f, 0.f, 0.f}, Q[3] = {0.f, 0.f, 0.f};
@@ -2426,6 +2509,28 @@ TileGeom tile_geom(int dim, int sf, const pmpm_grid &g, int gather) {
   }
 }
 static size_t tile_count(const TileGeom &tg) { return (size_t)tg.tiles[0] * tg.tiles[1] * tg.tiles[2]; }
+// The forward P2G and MUSL re-map of 3D typed cubic run on the matrix cores
+// with their own tiles (MfmaGeom); every other pass uses p2g_kernel's.
+static bool mfma_tiles(const pmpm_params &prm, int gather) {
+  return prm.dim == 3 && prm.shape_fn == PMPM_SF_CUBIC && prm.node_types_built && (gather == 1 || gather == 2);
+}
+static TileGeom mfma_tile_geom(const pmpm_grid &g, int gather) {
+  using M = MfmaGeom;
+  TileGeom tg{};
+  const int T[3] = {M::CELLS, 1, M::TZ}, N[3] = {M::BX, M::BY, M::NZ};
+  tg.nlo = -1;
+  for (int a = 0; a < 3; ++a) {
+    tg.T[a] = T[a];
+    tg.N[a] = N[a];
+    tg.tiles[a] = (int)ceil_div(g.num_cells[a], T[a]);
+  }
+  tg.gather = gather;
+  tg.nv = gather == 2 ? 3 : 7;
+  return tg;
+}
+static TileGeom tile_geom(const pmpm_params &prm, const pmpm_grid &g, int gather) {
+  return mfma_tiles(prm, gather) ? mfma_tile_geom(g, gather) : tile_geom(prm.dim, prm.shape_fn, g, gather);
+}
 
 // mode 0: P2G, 1: MUSL first map + re-map, 2: adjoint scatter of the
 // records `rec`
@@ -2437,23 +2542,26 @@ void launch_p2g(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles
   const TileGeom tg = tile_geom_of<D, SF>(g, 1);
   const dim3 grid((unsigned)tile_count(tg)), block(P2GGeom<D, SF>::THREADS);
   // 3D typed cubic P2G and MUSL re-map on the matrix cores (the re-map after
-  // a G2P-style first map); everything else on the VALU kernel
+  // a G2P-style first map), on their own tiles; everything else on the VALU
+  // kernel
   bool mfma = false;
   if constexpr (D == 3 && SF == PMPM_SF_CUBIC) {
     mfma = (mode == 0 || mode == 1) && prm.node_types_built;
+    const TileGeom mg = mfma_tile_geom(g, mode + 1);
+    const dim3 grid((unsigned)tile_count(mg));
     if (mfma && mode == 0 && has_planar(prm)) {
       PMPM_LAUNCH((p2g_mfma_kernel<SF, false, true>),
                   (::pmpm::kname("p2g_mfma_kernel", {(long)(SF), (long)(false), (long)(true)})), grid, dim3(256), 0, s,
-                  prm, g, in, out, part, nodes, tg.tiles[0], tg.tiles[1], partials, flags);
+                  prm, g, in, out, part, nodes, mg.tiles[0], mg.tiles[1], partials, flags);
     } else if (mfma && mode == 0) {
       PMPM_LAUNCH((p2g_mfma_kernel<SF, false>), (::pmpm::kname("p2g_mfma_kernel", {(long)(SF), (long)(false)})), grid, dim3(256), 0, s, prm, g, in, out, part, nodes,
-                         tg.tiles[0], tg.tiles[1], partials, flags);
+                         mg.tiles[0], mg.tiles[1], partials, flags);
     } else if (mfma && mode == 1) {
       if (in.n > 0)
         PMPM_LAUNCH((g2p_kernel<D, SF, 1>), (::pmpm::kname("g2p_kernel", {(long)(D), (long)(SF), (long)(1)})), dim3((unsigned)ceil_div(in.n, kG2PThreads)), dim3(kG2PThreads), 0,
                            s, prm, g, in, out, part, nodes, 1);
       PMPM_LAUNCH((p2g_mfma_kernel<SF, true>), (::pmpm::kname("p2g_mfma_kernel", {(long)(SF), (long)(true)})), grid, dim3(256), 0, s, prm, g, in, out, part, nodes,
-                         tg.tiles[0], tg.tiles[1], partials, flags);
+                         mg.tiles[0], mg.tiles[1], partials, flags);
     }
   }
   // (each mode launches exactly one tile kernel; the adjoint scatter reads
@@ -2501,19 +2609,20 @@ void launch_shape(const pmpm_grid &g, int types_built, const float *x, int n, fl
 int launch_grid(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g, const pmpm_nodes &nodes, int finalize,
                 int b0, int b1, int remap, int gather, hipStream_t s, int clear_mom_nt = 0) {
   const unsigned blocks = (unsigned)ceil_div(g.num_cells_total, 256);
-  TileGeom tg = tile_geom(prm.dim, prm.shape_fn, g, gather);
+  TileGeom tg = tile_geom(prm, g, gather);
   const float *partials = gather ? ctx->partials : nullptr;
   const int32_t *flags = gather ? ctx->tile_flags : nullptr;
   // the clamp record of a pass that applies node conditions (not the MUSL
   // re-map pass), for pmpm_adjoint_step
+  int32_t *wl = (ctx && gather) ? ctx->wrap_list : nullptr;
   uint8_t *clamp = nullptr;
   if (!remap && b1 > b0)
     clamp = nodes.clamp ? nodes.clamp : (ctx ? reinterpret_cast<uint8_t *>(ctx->clamp_mask) : nullptr);
 
   switch (prm.dim) {
-    case 1: PMPM_LAUNCH(grid_kernel<1>, (::pmpm::kname("grid_kernel", {(long)(1)}) + " gather " + std::to_string(gather)), dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt, clamp); break;
-    case 2: PMPM_LAUNCH(grid_kernel<2>, (::pmpm::kname("grid_kernel", {(long)(2)}) + " gather " + std::to_string(gather)), dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt, clamp); break;
-    case 3: PMPM_LAUNCH(grid_kernel<3>, (::pmpm::kname("grid_kernel", {(long)(3)}) + " gather " + std::to_string(gather)), dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt, clamp); break;
+    case 1: PMPM_LAUNCH(grid_kernel<1>, (::pmpm::kname("grid_kernel", {(long)(1)}) + " gather " + std::to_string(gather)), dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt, clamp, wl); break;
+    case 2: PMPM_LAUNCH(grid_kernel<2>, (::pmpm::kname("grid_kernel", {(long)(2)}) + " gather " + std::to_string(gather)), dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt, clamp, wl); break;
+    case 3: PMPM_LAUNCH(grid_kernel<3>, (::pmpm::kname("grid_kernel", {(long)(3)}) + " gather " + std::to_string(gather)), dim3(blocks), dim3(256), 0, s, prm, g, nodes, finalize, b0, b1, remap, tg, partials, flags, clear_mom_nt, clamp, wl); break;
     default: return PMPM_ERR_INVALID;
   }
   PMPM_LAUNCH_CHECK();
@@ -2548,11 +2657,23 @@ int do_zero_nodes(const pmpm_params &prm, const pmpm_grid &g, const pmpm_nodes &
   return PMPM_OK;
 }
 
-static int reserve_tiles(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g) {
+// Partials and tile flags for the larger of the two tilings a step can use
+// (p2g_kernel's; the matrix-core passes' own).
+static void partial_sizes(const pmpm_params &prm, const pmpm_grid &g, size_t *floats, size_t *tiles) {
   const TileGeom tg = tile_geom(prm.dim, prm.shape_fn, g, 1);
-  const size_t tiles = tile_count(tg);
-  const size_t ntn = (size_t)tg.N[0] * tg.N[1] * tg.N[2];
-  int rc = ensure_partials(ctx, tiles * tg.nv * ntn, tiles);
+  *tiles = tile_count(tg);
+  *floats = *tiles * tg.nv * (size_t)tg.N[0] * tg.N[1] * tg.N[2];
+  if (mfma_tiles(prm, 1)) {
+    const TileGeom mg = mfma_tile_geom(g, 1);
+    const size_t mt = tile_count(mg);
+    *tiles = std::max(*tiles, mt);
+    *floats = std::max(*floats, mt * mg.nv * (size_t)mg.N[0] * mg.N[1] * mg.N[2]);
+  }
+}
+static int reserve_tiles(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g) {
+  size_t floats = 0, tiles = 0;
+  partial_sizes(prm, g, &floats, &tiles);
+  int rc = ensure_partials(ctx, floats, tiles);
   if (rc != PMPM_OK) return rc;
   // the grid pass's clamp record (one byte per node, in words of 4)
   if ((rc = ensure_buffer(ctx->clamp_mask, ctx->clamp_mask_cap, ((size_t)g.num_cells_total + 3) / 4)) != PMPM_OK)
@@ -2585,9 +2706,7 @@ static uint8_t *wrap_touch(pmpm_context *ctx) { return reinterpret_cast<uint8_t 
 void p2g_workspace(const pmpm_params &prm, const pmpm_grid &g, size_t *partial_floats, size_t *tiles,
                    size_t *wrap_floats, size_t *wrap_cells, size_t *wrap_touch_words, size_t *wrap_list_words,
                    size_t *clamp_words) {
-  const TileGeom tg = tile_geom(prm.dim, prm.shape_fn, g, 1);
-  *tiles = tile_count(tg);
-  *partial_floats = *tiles * tg.nv * (size_t)tg.N[0] * tg.N[1] * tg.N[2];
+  partial_sizes(prm, g, partial_floats, tiles);
   *wrap_cells = (size_t)(prm.dim == 3 ? wrap_cell_count<3>(g) : (prm.dim == 2 ? wrap_cell_count<2>(g) : 0));
   *wrap_floats = wrap_mode(prm, g) == 2 ? (size_t)(1 + 2 * prm.dim) * *wrap_cells * 64 : 0;
   *wrap_touch_words = wrap_mode(prm, g) == 2 ? ((size_t)g.num_cells_total + 3) / 4 : 0;
@@ -2601,7 +2720,7 @@ static int launch_wrap_gather(pmpm_context *ctx, const pmpm_params &prm, const p
                               const pmpm_nodes &nodes, int gather, hipStream_t s) {
   const int mode = wrap_mode(prm, g);
   if (mode == 0) return PMPM_OK;
-  TileGeom tg = tile_geom(prm.dim, prm.shape_fn, g, gather);
+  TileGeom tg = tile_geom(prm, g, gather);
   tg.wrap = mode;
   tg.wcells = prm.dim == 2 ? wrap_cell_count<2>(g) : wrap_cell_count<3>(g);
   tg.wrec = ctx->wrap_rec;
@@ -2610,7 +2729,7 @@ static int launch_wrap_gather(pmpm_context *ctx, const pmpm_params &prm, const p
   const int targets = prm.dim == 2 ? wrap_target_count<2>(g) : wrap_target_count<3>(g);
   if (targets <= 0) return PMPM_OK;
   if (mode == 2) {  // typed cubic: the nodes the wrap kernel listed
-    const dim3 lgrid((unsigned)std::min<int64_t>(kWrapGatherBlocks, ceil_div(targets, 4))), block(256);
+    const dim3 lgrid((unsigned)ceil_div(targets, 4)), block(256);  // a wave per possible entry
     uint8_t *touch = wrap_touch(ctx);
     if (prm.dim == 2) {
       if (gather == 1)
