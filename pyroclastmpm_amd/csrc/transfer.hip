@@ -1153,7 +1153,7 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 // (planar_force) is staged too and w f joins the force product through its
 // A operand (Nx f_a next to dNx A_a.x, both against Ny Nz).
 template <int SF, bool REMAP = false, bool FEXT = false>
-__global__ __launch_bounds__(256, 4) void p2g_mfma_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles in,
+__global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles in,
                                                        pmpm_particles out, pmpm_spatial part, pmpm_nodes nodes,
                                                        int tiles_x, int tiles_y, float *__restrict__ partials,
                                                        int32_t *__restrict__ flags) {
@@ -1372,13 +1372,10 @@ __global__ __launch_bounds__(256, 4) void p2g_mfma_kernel(pmpm_params prm, pmpm_
     }
     stage_raw(lx, cur);
     wave_lds_sync();
-    // the products in two branch-free halves (the records past the chunk's
-    // particles are zero), so the record reads of later products are issued
-    // ahead of the earlier ones
     const int wn = min(M::CH, ce_of(lx) - w0);
 #pragma unroll
     for (int k0 = 0; k0 < M::CH; k0 += 4) {
-      if (k0 == M::CH / 2 && wn <= M::CH / 2) break;  // wave-uniform
+      if (k0 >= wn) break;  // wave-uniform
       const float *sp = st + (k0 + pk) * RS;
       const float Nx = sp[ex], dNx = sp[ex + 1], Ny = sp[ex + 2], dNy = sp[ex + 3];
       const float Nz = sp[ez + 4], dNz = sp[ez + 5], Ff = sp[ez + 6];
