@@ -1284,7 +1284,8 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
     float *sp = st + lane * RS;
     if (!r.valid) {
 #pragma unroll
-      for (int e = 0; e < RS; ++e) sp[e] = 0.f;
+      for (int e = 0; e < RS; ++e)
+        if (!REMAP || (e < 28 && (e % 7) % 2 == 0)) sp[e] = 0.f;  // (the entries the products read)
       return;
     }
     const int cx = bx * M::CELLS + lx;
@@ -1315,15 +1316,16 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       float N, dN;
+      // (the re-map's products use the weights only: no derivatives)
       piece(coef(txs[s], s), __fsub_rn(fr[0], (float)(NLO + s)), s == 0, N, dN);
       sp[7 * s + 0] = N;
-      sp[7 * s + 1] = dN;
+      if (!REMAP) sp[7 * s + 1] = dN;
       piece(coef(tys[s], s), __fsub_rn(fr[1], (float)(NLO + s)), s == 0, N, dN);
       sp[7 * s + 2] = N;
-      sp[7 * s + 3] = dN;
+      if (!REMAP) sp[7 * s + 3] = dN;
       piece(coef(tzs[s], s), __fsub_rn(fr[2], (float)(NLO + s)), s == 0, N, dN);
       sp[7 * s + 4] = N;
-      sp[7 * s + 5] = dN;
+      if (!REMAP) sp[7 * s + 5] = dN;
     }
     sp[6] = REMAP ? 0.f : keep * r.m;  // (the re-map's field 0 is padding)
 #pragma unroll
@@ -1437,8 +1439,12 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
   }
 }
 
+// Floats per (cell, offset) record of the wrap-around sums: the NV fields
+// padded to 16 or 32 bytes, so a record is one or two vector loads.
+__host__ __device__ constexpr int wrap_rec_stride(int nv) { return nv <= 4 ? 4 : 8; }
+
 // The wrap-around sums of one non-empty wrap cell c (index w), one lane per
-// node offset; writes all 64 records of the cell.
+// node offset; writes the records of the offsets the gather reads.
 template <int D, int MODE, int NV, int NS>
 __device__ inline void wrap_cell_sums(const pmpm_params &prm, const pmpm_grid &g, const pmpm_particles &in,
                                       const pmpm_particles &out, const pmpm_spatial &part, float *rec, int wcells,
@@ -1585,9 +1591,14 @@ __device__ inline void wrap_cell_sums(const pmpm_params &prm, const pmpm_grid &g
       acc[k] += __shfl_xor(acc[k], 1);
       acc[k] += __shfl_xor(acc[k], 2);
     }
-    if (mine && q == 0)
+    if (mine && q == 0) {  // one 16- or 32-byte record, fields past NV zero
+      constexpr int RST = wrap_rec_stride(NV);
+      float4 *dst = reinterpret_cast<float4 *>(rec + ((int64_t)w * 64 + off) * RST);
 #pragma unroll
-      for (int k = 0; k < NV; ++k) rec[((int64_t)k * wcells + w) * 64 + off] = acc[k];
+      for (int v = 0; v < RST / 4; ++v)
+        dst[v] = float4{4 * v + 0 < NV ? acc[4 * v + 0] : 0.f, 4 * v + 1 < NV ? acc[4 * v + 1] : 0.f,
+                        4 * v + 2 < NV ? acc[4 * v + 2] : 0.f, 4 * v + 3 < NV ? acc[4 * v + 3] : 0.f};
+    }
   }
 }
 
@@ -1598,7 +1609,8 @@ __device__ inline void wrap_cell_sums(const pmpm_params &prm, const pmpm_grid &g
 // (shapefunction.cpp:96-112); four lanes share such an offset, each summing a
 // quarter of the particles, and the quarters are combined in a fixed order.
 // The sums of a non-empty cell are stored per (cell, offset) in `rec`
-// ([NV][wcells][64]; only the offsets the gather reads are written);
+// ([wcells][64] records of wrap_rec_stride(NV) floats; only the offsets the
+// gather reads are written);
 // wrap_gather_kernel adds them to their nodes in a fixed order, so the node
 // sums do not depend on scheduling. Empty cells write nothing (the gather
 // skips them by their cell ranges). The cell's particles are first loaded
@@ -1627,6 +1639,21 @@ __global__ __launch_bounds__(64 * kWrapCellsPerBlock) void p2g_wrap_kernel(pmpm_
   if (ce <= cs) return;
   wrap_cell_sums<D, MODE, NV, NS>(prm, g, in, out, part, rec, wcells, w, c, cs, ce, lane, s_frac_all[wv],
                                   s_v_all[wv], s_A_all[wv], s_m_all[wv], s_own_all[wv], s_list_all[wv], touch, wlist);
+}
+
+// The NV fields of the wrap record of (wrap cell w, offset rl).
+template <int NV>
+__device__ inline void wrap_rec_load(const float *wrec, int w, int rl, float *r) {
+  constexpr int RST = wrap_rec_stride(NV);
+  const float4 *src = reinterpret_cast<const float4 *>(wrec + ((int64_t)w * 64 + rl) * RST);
+#pragma unroll
+  for (int v = 0; v < RST / 4; ++v) {
+    const float4 q = src[v];
+    if (4 * v + 0 < NV) r[4 * v + 0] = q.x;
+    if (4 * v + 1 < NV) r[4 * v + 1] = q.y;
+    if (4 * v + 2 < NV) r[4 * v + 2] = q.z;
+    if (4 * v + 3 < NV) r[4 * v + 3] = q.w;
+  }
 }
 
 // Geometry of the P2G tiles for the grid kernel's gather.
@@ -1752,8 +1779,10 @@ __device__ inline bool gather_wrapped_lane(const pmpm_grid &g, const TileGeom &t
       const int cl = c[0] + nx * (c[1] + ny * c[2]);
       if (tg.cend[cl] <= tg.cstart[cl]) continue;
       const int rl = sx + 4 * (sy + 4 * sz);
+      float r[NV];
+      wrap_rec_load<NV>(tg.wrec, w, rl, r);
 #pragma unroll
-      for (int k = 0; k < NV; ++k) acc[k] += tg.wrec[((int64_t)k * tg.wcells + w) * 64 + rl];
+      for (int k = 0; k < NV; ++k) acc[k] += r[k];
       found = true;
     }
   }
@@ -1864,8 +1893,7 @@ __device__ inline bool gather_wrapped_typed(const pmpm_grid &g, const TileGeom &
   for (int k = 0; k < 4; ++k) {
     cs[k] = tg.cstart[cl[k]];
     ce[k] = tg.cend[cl[k]];
-#pragma unroll
-    for (int f = 0; f < NV; ++f) r[k][f] = tg.wrec[((int64_t)f * tg.wcells + wi[k]) * 64 + rl];
+    wrap_rec_load<NV>(tg.wrec, wi[k], rl, r[k]);
   }
   bool found = false;
 #pragma unroll
@@ -2677,7 +2705,7 @@ static int reserve_tiles(pmpm_context *ctx, const pmpm_params &prm, const pmpm_g
     return rc;
   if (wrap_mode(prm, g) != 2) return rc;
   const size_t wcells = (size_t)(prm.dim == 2 ? wrap_cell_count<2>(g) : wrap_cell_count<3>(g));
-  if ((rc = ensure_buffer(ctx->wrap_rec, ctx->wrap_rec_cap, (size_t)(1 + 2 * prm.dim) * wcells * 64)) != PMPM_OK)
+  if ((rc = ensure_buffer(ctx->wrap_rec, ctx->wrap_rec_cap, (size_t)wrap_rec_stride(1 + 2 * prm.dim) * wcells * 64)) != PMPM_OK)
     return rc;
   // node flags, one byte per node, cleared once here (the gather clears
   // what it visits; a stale flag would only cost the gather a visit)
@@ -2705,7 +2733,7 @@ void p2g_workspace(const pmpm_params &prm, const pmpm_grid &g, size_t *partial_f
                    size_t *clamp_words) {
   partial_sizes(prm, g, partial_floats, tiles);
   *wrap_cells = (size_t)(prm.dim == 3 ? wrap_cell_count<3>(g) : (prm.dim == 2 ? wrap_cell_count<2>(g) : 0));
-  *wrap_floats = wrap_mode(prm, g) == 2 ? (size_t)(1 + 2 * prm.dim) * *wrap_cells * 64 : 0;
+  *wrap_floats = wrap_mode(prm, g) == 2 ? (size_t)wrap_rec_stride(1 + 2 * prm.dim) * *wrap_cells * 64 : 0;
   *wrap_touch_words = wrap_mode(prm, g) == 2 ? ((size_t)g.num_cells_total + 3) / 4 : 0;
   *wrap_list_words = wrap_mode(prm, g) == 2 ? (size_t)g.num_cells_total + 2 : 0;
   *clamp_words = ((size_t)g.num_cells_total + 3) / 4;
