@@ -193,8 +193,13 @@ int pmpm_context_out_of_grid(pmpm_context *ctx, pmpm_stream_t stream, int32_t *c
 
 /* ---- spatial binning: replaces SpatialPartition::calculate_hash,
  * sort_hashes and bin_particles (spatialpartition.cpp:90-136,
- * spatialpartition_inline.h:49-115). Stable LSD radix sort of (hash, slot):
- * sorted[] equals thrust::stable_sort_by_key over slots 0..n-1. */
+ * spatialpartition_inline.h:49-115). sorted[] equals
+ * thrust::stable_sort_by_key over slots 0..n-1. On entry, part->hash_sorted
+ * is read as a hint: when it holds the previous call's sorted keys and the
+ * particles come in that call's cell order (as pmpm_step leaves them), only
+ * the particles that changed cell are sorted. Any content gives the same
+ * result; only the time differs. A context created with the environment
+ * variable PMPM_PARTITION_RADIX=1 always runs the full LSD radix sort. */
 int pmpm_partition(pmpm_context *ctx, int32_t dim, const pmpm_grid *grid, const float *x,
                    int32_t n, pmpm_spatial *part, pmpm_stream_t stream);
 

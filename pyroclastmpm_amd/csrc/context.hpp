@@ -18,7 +18,7 @@ struct pmpm_context {
   int32_t *vals_tmp = nullptr;   // radix sort ping-pong values
   uint32_t *hist = nullptr;      // [radix][blocks] digit histograms
   size_t hist_len = 0;
-  uint32_t *counters = nullptr;  // [4]: out-of-grid particles, ...
+  uint32_t *counters = nullptr;  // [4]: out-of-grid particles, hint unsorted flag, movers, -
   // P2G tile partials: per tile, the LDS node block of each accumulated field
   // (plain stores; the grid kernel sums the <= 2^D tiles covering a node)
   float *partials = nullptr;
@@ -50,6 +50,24 @@ struct pmpm_context {
   // per-workgroup partial sums of the material parameter adjoints
   float *adj_pb = nullptr;
   size_t adj_pb_cap = 0;
+  // partition from a sorted hint (the previous call's sorted keys): false
+  // always runs the full radix sort (PMPM_PARTITION_RADIX=1 at creation)
+  bool sort_by_hint = true;
+  // per cell (ncells + 1, the last for the out-of-grid particles): counts
+  // of stayers and movers (zero between calls), and per block of 1024 cells
+  float *hint_cnt = nullptr;
+  size_t hint_cnt_cap = 0;
+  float *hint_cbt = nullptr;
+  size_t hint_cbt_cap = 0;
+  // per cell: first slot, stayers before the cell, stayers, movers
+  float *hint_info = nullptr;
+  size_t hint_info_cap = 0;
+  // stayers per block of 1024 particles, then five arrays of n words:
+  // stayers before each particle (sign bit: a stayer), the movers' keys in
+  // slot order, the stayers before each mover, the sorted movers' keys and
+  // positions in the mover list
+  float *hint_part = nullptr;
+  size_t hint_part_cap = 0;
 };
 
 namespace pmpm {

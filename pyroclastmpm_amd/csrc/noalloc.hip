@@ -91,6 +91,7 @@ extern "C" int pmpm_step_noalloc(const pmpm_params *prm, const pmpm_grid *grid, 
   ctx.hist = reinterpret_cast<uint32_t *>(base + o_hist);
   ctx.hist_len = hist_len;
   ctx.counters = reinterpret_cast<uint32_t *>(base + o_counters);
+  ctx.sort_by_hint = false;  // the full radix sort: its workspace is all carved above
   ctx.partials = reinterpret_cast<float *>(base + o_partials);
   ctx.partials_cap = partial_floats;
   ctx.tile_flags = reinterpret_cast<int32_t *>(base + o_flags);

@@ -1133,7 +1133,7 @@ struct MfmaGeom {
   static constexpr int TZ = 4, NZ = TZ + 3;             // tile layers, slice node layers
   static constexpr int NTN = BX * BY * NZ;              // nodes of one partial slice
   // x pitch of the block in LDS: the 64 lanes of a flush, offsets
-  // pk + 20 s1 + 80 s2, fall into 64 distinct banks (19 would pair some up)
+  // pk + 12 s1 + 48 s2, fall into 64 distinct banks
   static constexpr int BXP = BX + 1;
   static constexpr int BN = BXP * BY * BZ;
   static constexpr int CH = 32;  // particles staged per round
@@ -1260,13 +1260,16 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
   // (lx == CELLS: no chunk; its loads read particle 0 and stage nothing, so
   // that every step issues the same loads)
   auto chunk_len = [&](int lx, int w0) { return lx < M::CELLS ? min(M::CH, ce_of(lx) - w0) : 0; };
+  // lanes l and l + CH hold the same particle (see stage_raw)
+  static_assert(M::CH == 32, "two lanes per staged particle");
+  const int prow = lane & (M::CH - 1);
   auto chunk_j = [&](int lx, int w0) {  // this lane's sorted index, clamped to the chunk
-    return lx < M::CELLS ? w0 + min(lane, chunk_len(lx, w0) - 1) : 0;
+    return lx < M::CELLS ? w0 + min(prow, chunk_len(lx, w0) - 1) : 0;
   };
   auto chunk_slot = [&](int lx, int w0) { return part.sorted[chunk_j(lx, w0)]; };
   auto load_raw = [&](int lx, int w0, int slot, Raw &r) {
     r.slot = slot;
-    r.valid = lane < chunk_len(lx, w0);
+    r.valid = prow < chunk_len(lx, w0);
     const int j = chunk_j(lx, w0);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -1277,15 +1280,21 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
     r.vol = REMAP ? 0.f : in.volume[slot];
     r.sg = REMAP ? sym_zero() : load_sym(in.stress, n, slot);
   };
-  // a chunk's particle records into the LDS stage (lane = particle); the
-  // records past the chunk's particles are zero
+  // a chunk's particle records into the LDS stage: lanes l and l + 32 both
+  // hold particle l (row l) and each writes half of its record, half h =
+  // lane / 32 the weights of span offsets s = 2h, 2h + 1 with their fields,
+  // and half of A (the same instructions on both halves, so a record costs
+  // half the instructions of one lane per particle). The records past the
+  // chunk's particles are zero.
   auto stage_raw = [&](int lx, const Raw &r) {
-    if (lane >= M::CH) return;
-    float *sp = st + lane * RS;
+    const int h = lane >> 5;
+    float *sp = st + prow * RS;
     if (!r.valid) {
+      if (h == 0) {
 #pragma unroll
-      for (int e = 0; e < RS; ++e)
-        if (!REMAP || (e < 28 && (e % 7) % 2 == 0)) sp[e] = 0.f;  // (the entries the products read)
+        for (int e = 0; e < RS; ++e)
+          if (!REMAP || (e < 28 && (e % 7) % 2 == 0)) sp[e] = 0.f;  // (the entries the products read)
+      }
       return;
     }
     const int cx = bx * M::CELLS + lx;
@@ -1305,7 +1314,7 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
       A = Sym{mvol * r.sg.xx, mvol * r.sg.yy, mvol * r.sg.zz, mvol * r.sg.xy, mvol * r.sg.xz, mvol * r.sg.yz};
       if (FEXT) planar_force<3>(prm, g, r.x, r.v, r.vol, r.m, fe);
     }
-    if (!own) {
+    if (!own && h == 0) {
       // listed in this cell but binned elsewhere (a partition of other
       // positions): deposited over its own window, no product terms here
 #pragma unroll 1
@@ -1314,34 +1323,35 @@ __global__ __launch_bounds__(256) void p2g_mfma_kernel(pmpm_params prm, pmpm_gri
     }
     const float keep = own ? 1.f : 0.f;
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
+    for (int j = 0; j < 2; ++j) {
+      const int s = 2 * h + j;
+      const int tx = h ? txs[2 + j] : txs[j], ty = h ? tys[2 + j] : tys[j], tz = h ? tzs[2 + j] : tzs[j];
       float N, dN;
       // (the re-map's products use the weights only: no derivatives)
-      piece(coef(txs[s], s), __fsub_rn(fr[0], (float)(NLO + s)), s == 0, N, dN);
+      piece(coef(tx, s), __fsub_rn(fr[0], (float)(NLO + s)), s == 0, N, dN);
       sp[7 * s + 0] = N;
       if (!REMAP) sp[7 * s + 1] = dN;
-      piece(coef(tys[s], s), __fsub_rn(fr[1], (float)(NLO + s)), s == 0, N, dN);
+      piece(coef(ty, s), __fsub_rn(fr[1], (float)(NLO + s)), s == 0, N, dN);
       sp[7 * s + 2] = N;
       if (!REMAP) sp[7 * s + 3] = dN;
-      piece(coef(tzs[s], s), __fsub_rn(fr[2], (float)(NLO + s)), s == 0, N, dN);
+      piece(coef(tz, s), __fsub_rn(fr[2], (float)(NLO + s)), s == 0, N, dN);
       sp[7 * s + 4] = N;
       if (!REMAP) sp[7 * s + 5] = dN;
+      // field s: the mass (the re-map's field 0 is padding), then m v_a
+      const float va = s == 1 ? r.v[0] : (s == 2 ? r.v[1] : r.v[2]);
+      sp[7 * s + 6] = s == 0 ? (REMAP ? 0.f : keep * r.m) : keep * r.m * va;
     }
-    sp[6] = REMAP ? 0.f : keep * r.m;  // (the re-map's field 0 is padding)
-#pragma unroll
-    for (int a = 0; a < 3; ++a) sp[7 * (a + 1) + 6] = keep * r.m * r.v[a];
-    if (!REMAP) {
-      sp[28] = keep * A.xx;
-      sp[29] = keep * A.yy;
-      sp[30] = keep * A.zz;
-      sp[31] = keep * A.xy;
-      sp[32] = keep * A.xz;
-      sp[33] = keep * A.yz;
+    if (!REMAP) {  // half 0: xx, yy, zz; half 1: xy, xz, yz
+      sp[28 + 3 * h] = keep * (h ? A.xy : A.xx);
+      sp[29 + 3 * h] = keep * (h ? A.xz : A.yy);
+      sp[30 + 3 * h] = keep * (h ? A.yz : A.zz);
     }
-    sp[34] = 0.f;
-    if (FEXT) {
+    if (h == 0) {
+      sp[34] = 0.f;
+      if (FEXT) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) sp[35 + a] = keep * fe[a];
+        for (int a = 0; a < 3; ++a) sp[35 + a] = keep * fe[a];
+      }
     }
   };
 
