@@ -198,8 +198,10 @@ int pmpm_context_out_of_grid(pmpm_context *ctx, pmpm_stream_t stream, int32_t *c
  * is read as a hint: when it holds the previous call's sorted keys and the
  * particles come in that call's cell order (as pmpm_step leaves them), only
  * the particles that changed cell are sorted. Any content gives the same
- * result; only the time differs. A context created with the environment
- * variable PMPM_PARTITION_RADIX=1 always runs the full LSD radix sort. */
+ * result; only the time differs. Below 131,072 particles (or the value of
+ * the environment variable PMPM_PARTITION_HINT_MIN when the context was
+ * created) and for a context created with PMPM_PARTITION_RADIX=1 it runs
+ * the full LSD radix sort. */
 int pmpm_partition(pmpm_context *ctx, int32_t dim, const pmpm_grid *grid, const float *x,
                    int32_t n, pmpm_spatial *part, pmpm_stream_t stream);
 

@@ -51,8 +51,11 @@ struct pmpm_context {
   float *adj_pb = nullptr;
   size_t adj_pb_cap = 0;
   // partition from a sorted hint (the previous call's sorted keys): false
-  // always runs the full radix sort (PMPM_PARTITION_RADIX=1 at creation)
+  // always runs the full radix sort (PMPM_PARTITION_RADIX=1 at creation);
+  // below hint_min_particles particles the radix sort runs too (its fewer
+  // launches win there; PMPM_PARTITION_HINT_MIN at creation)
   bool sort_by_hint = true;
+  int32_t hint_min_particles = 131072;
   // per cell (ncells + 1, the last for the out-of-grid particles): counts
   // of stayers and movers (zero between calls), and per block of 1024 cells
   float *hint_cnt = nullptr;

@@ -669,6 +669,8 @@ extern "C" int pmpm_context_create(pmpm_context **ctx) {
   if (!hip_ok(hipGetDevice(&c->device), "hipGetDevice")) { delete c; return PMPM_ERR_HIP; }
   const char *radix_only = getenv("PMPM_PARTITION_RADIX");
   c->sort_by_hint = !(radix_only && radix_only[0] == '1');
+  const char *hint_min = getenv("PMPM_PARTITION_HINT_MIN");
+  if (hint_min && hint_min[0]) c->hint_min_particles = (int32_t)strtol(hint_min, nullptr, 10);
   if (!hip_ok(hipMalloc(&c->counters, 4 * sizeof(uint32_t)), "hipMalloc")) { delete c; return PMPM_ERR_HIP; }
   if (!hip_ok(hipMemset(c->counters, 0, 4 * sizeof(uint32_t)), "hipMemset")) {
     (void)hipFree(c->counters);
@@ -749,7 +751,7 @@ extern "C" int pmpm_context_reserve(pmpm_context *ctx, int32_t max_particles, in
     ctx->cap_particles = max_particles;
   }
   if (max_cells > ctx->cap_cells) ctx->cap_cells = max_cells;
-  if (ctx->sort_by_hint && max_particles > 0 && max_cells > 0) {
+  if (ctx->sort_by_hint && max_particles >= ctx->hint_min_particles && max_particles > 0 && max_cells > 0) {
     // the hint partition's workspace too, so that stepping within these
     // sizes allocates nothing (the counts are cleared on the null stream,
     // which is waited for)
@@ -914,7 +916,7 @@ int pmpm::partition_clearing(pmpm_context *ctx, int dim, const pmpm_grid *grid, 
     z.add(part->cell_end, ncells);
     return zero_fill_lists(z, s);
   }
-  if (ctx->sort_by_hint) return partition_by_hint(ctx, dim, g, x, n, part, s, extra);
+  if (ctx->sort_by_hint && n >= ctx->hint_min_particles) return partition_by_hint(ctx, dim, g, x, n, part, s, extra);
   // enough threads for the particles, and for the clears at 4 words each
   int64_t words = ncells;
   for (int q = 0; q < extra.count; ++q) words = extra.n[q] > words ? extra.n[q] : words;

@@ -31,16 +31,21 @@ def _need_gpu():
 
 
 def _context(n, ncells, radix_only):
+    """A context on the radix path, or on the hint path at every particle
+    count (PMPM_PARTITION_HINT_MIN=0; by default small counts take the radix
+    path)."""
     from pyroclastmpm_amd import capi
-    old = os.environ.get("PMPM_PARTITION_RADIX")
-    os.environ["PMPM_PARTITION_RADIX"] = "1" if radix_only else "0"
+    env = {"PMPM_PARTITION_RADIX": "1" if radix_only else "0", "PMPM_PARTITION_HINT_MIN": "0"}
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         return capi.Context(n, ncells)
     finally:
-        if old is None:
-            del os.environ["PMPM_PARTITION_RADIX"]
-        else:
-            os.environ["PMPM_PARTITION_RADIX"] = old
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
 
 
 def _partition(ctx, g, dim, x, hint):
