@@ -3286,6 +3286,90 @@ template <int D> __device__ inline void cofactor(const float *F, float *C) {
 // run branch-free (clamped node, zero weights outside, as gather_grid); the
 // wrap-around entries of boundary particles, which only the deposit reaches,
 // take a separate compact loop. Adds to ib.
+// LDS staging of the node records the 3D cubic adjoint P2G reads (one lane
+// per particle, kG2PThreads per workgroup, consecutive in cell order): as in
+// g2p_gather_staged, the workgroup's particles form runs (same y and z bins,
+// x bins not decreasing and within one 32-aligned block), whose windows lie
+// in 4 x 4 node rows over the run's x range - 1 .. + 2, at most 16 x 35
+// nodes; the runs are staged in rounds of at most kAdjStageNodes nodes, the
+// node adjoint record (8 floats) and velocity record (8 floats) of each
+// clamped node. gather(adj, vel, box0, X) is called by every live thread in
+// the round that holds its run: the node of window entry (ex, sy, sz) is
+// box0 + ex + X (sy + 4 sz) of adj and vel. Every thread of the workgroup
+// calls this (barriers); `live` is false past the particles.
+constexpr int kAdjStageNodes = 576;  // 16 x 36: one run always fits
+static_assert(kAdjStageNodes >= 16 * (32 + 3), "a run spans at most 32 x bins");
+struct AdjRunTable {
+  int y[kG2PThreads], z[kG2PThreads], x0[kG2PThreads], x1[kG2PThreads], base[kG2PThreads];
+  int wave_sum[kG2PThreads / 64];
+};
+template <class PWT, class F>
+__device__ inline void adjoint_stage_rounds(const pmpm_grid &g, const pmpm_nodes &nodes,
+                                            const float *__restrict__ nbar, const PWT &w, bool live, F &&gather) {
+  constexpr int CAP = kAdjStageNodes;
+  __shared__ float4 s_adj[CAP * 2], s_vel[CAP * 2];
+  __shared__ AdjRunTable R;
+  __shared__ int rb[3][kG2PThreads];
+  const int t = (int)threadIdx.x;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) rb[a][t] = w.bin[a];
+  const int nv = __syncthreads_count(live);  // live threads are a prefix
+  auto starts_at = [&](int q) {
+    return rb[1][q - 1] != rb[1][q] || rb[2][q - 1] != rb[2][q] || rb[0][q] < rb[0][q - 1] ||
+           (rb[0][q] >> 5) != (rb[0][q - 1] >> 5);
+  };
+  bool start = false, end = false;
+  if (live) {
+    start = t == 0 || starts_at(t);
+    end = t == nv - 1 || starts_at(t + 1);
+  }
+  int nruns = 0;
+  const int k = block_inclusive_sum(start ? 1 : 0, R.wave_sum, &nruns) - 1;
+  if (start) {
+    R.y[k] = w.bin[1];
+    R.z[k] = w.bin[2];
+    R.x0[k] = w.bin[0];
+  }
+  if (end) R.x1[k] = w.bin[0];
+  __syncthreads();
+  const int cnt = t < nruns ? 16 * (R.x1[t] - R.x0[t] + 4) : 0;
+  int all = 0;
+  const int incl = block_inclusive_sum(cnt, R.wave_sum, &all);
+  if (t < nruns) R.base[t] = incl - cnt;
+  __syncthreads();
+  const int myk = live ? k : -1;
+  for (int k0 = 0; k0 < nruns;) {  // uniform over the workgroup
+    const int b0 = R.base[k0];
+    int k1 = k0 + 1;
+    while (k1 < nruns && R.base[k1] + 16 * (R.x1[k1] - R.x0[k1] + 4) - b0 <= CAP) ++k1;
+    const int tot = R.base[k1 - 1] + 16 * (R.x1[k1 - 1] - R.x0[k1 - 1] + 4) - b0;
+    for (int q = t; q < tot; q += kG2PThreads) {
+      int kk = k0;
+      while (kk + 1 < k1 && R.base[kk + 1] - b0 <= q) ++kk;
+      const int X = R.x1[kk] - R.x0[kk] + 4, qq = q - (R.base[kk] - b0);
+      const int row = qq / X, x = qq - row * X;  // row = sz * 4 + sy
+      const int ix = min(max(R.x0[kk] - 1 + x, 0), g.num_cells[0] - 1);
+      const int iy = min(max(R.y[kk] - 1 + (row & 3), 0), g.num_cells[1] - 1);
+      const int iz = min(max(R.z[kk] - 1 + (row >> 2), 0), g.num_cells[2] - 1);
+      const uint32_t node = (uint32_t)ix + (uint32_t)g.num_cells[0] * ((uint32_t)iy + (uint32_t)g.num_cells[1] * (uint32_t)iz);
+      const float4 *ar = reinterpret_cast<const float4 *>(nbar + 8 * (size_t)node);
+      const float4 *vr = reinterpret_cast<const float4 *>(nodes.vel + 8 * (size_t)node);
+      s_adj[2 * q] = ar[0];
+      s_adj[2 * q + 1] = ar[1];
+      s_vel[2 * q] = vr[0];
+      s_vel[2 * q + 1] = vr[1];
+    }
+    __syncthreads();
+    if (myk >= k0 && myk < k1) {
+      const int off = R.base[myk] - b0, X = R.x1[myk] - R.x0[myk] + 4;
+      gather(reinterpret_cast<const float *>(s_adj), reinterpret_cast<const float *>(s_vel),
+             off + (w.bin[0] - R.x0[myk]), X);
+    }
+    k0 = k1;
+    __syncthreads();  // the next round overwrites the staging
+  }
+}
+
 template <int D, int SF, int LANES>
 __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_grid g, pmpm_particles st,
                                                           pmpm_adjoint ib, pmpm_spatial part, pmpm_nodes nodes,
@@ -3301,10 +3385,17 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
   // each. The particle's sums are combined by shuffles; the wider layouts are
   // for launches too small to fill the chip otherwise.
   static_assert(LANES == 1 || LANES == 4 || (LANES == 16 && D == 3), "lanes per particle");
+  // 3D cubic with one lane per particle: the workgroup's node records are
+  // staged in LDS (adjoint_stage_rounds), so every thread takes part in the
+  // staging barriers; a thread past the particles works on the last one and
+  // writes nothing
+  constexpr bool STAGED = D == 3 && LANES == 1 && SF == PMPM_SF_CUBIC;
   const int j = (int)blockIdx.x * ((int)blockDim.x / LANES) + (int)threadIdx.x / LANES;
   const int sub0 = (int)threadIdx.x % LANES;
-  if (j >= n) return;  // (all lanes of a particle together)
-  const int src = part.sorted[j];
+  if (!STAGED && j >= n) return;  // (all lanes of a particle together)
+  const bool live = j < n;
+  const int jj = live ? j : n - 1;
+  const int src = part.sorted[jj];
   PW<D, SF> w;
   float xp[3];
   particle_weights<D, SF>(prm, g, st.x, n, src, w, xp);
@@ -3322,13 +3413,13 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
   float vpb[3] = {0.f, 0.f, 0.f}, dvb[3] = {0.f, 0.f, 0.f}, Lb[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int a = 0; a < D; ++a) {
-    vpb[a] = rec[a * n + j];
-    dvb[a] = rec[(3 + a) * n + j];
+    vpb[a] = rec[a * n + jj];
+    dvb[a] = rec[(3 + a) * n + jj];
   }
 #pragma unroll
   for (int a = 0; a < D; ++a)
 #pragma unroll
-    for (int b = 0; b < D; ++b) Lb[3 * a + b] = rec[(6 + 3 * a + b) * n + j];
+    for (int b = 0; b < D; ++b) Lb[3 * a + b] = rec[(6 + 3 * a + b) * n + jj];
   // per node: the deposit adjoints (m, P, f) and, for the G2P weights, the
   // node velocities; G = sum f (x) grad w gives the stress and volume adjoints
   // after the loop
@@ -3340,11 +3431,14 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
   float vb[3] = {0.f, 0.f, 0.f}, xb[3] = {0.f, 0.f, 0.f}, G[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f},
                                                                        {0.f, 0.f, 0.f}};
   // one entry's contributions (g2p = 1 adds the G2P weights' position term)
-  auto entry = [&](int node, const float *N3, const float *dN3, const float *d2N3, bool g2p) {
+  // (adj, vel: the node adjoint and velocity records, in global memory or
+  // staged in LDS, indexed by `node`)
+  auto entry = [&](const float *adj, const float *vel, int node, const float *N3, const float *dN3,
+                   const float *d2N3, bool g2p) {
     float wgt, gr[3], H[6];
     entry_derivs(N3, dN3, d2N3, wgt, gr, H);
     float mb, Pb[3] = {0.f, 0.f, 0.f}, fb[3] = {0.f, 0.f, 0.f};
-    load_node_adj<D>(nbar, node, mb, Pb, fb);
+    load_node_adj<D>(adj, node, mb, Pb, fb);
     // gs: the weight's scalar factor; wv = L_bar v_nt - V sigma^T f_bar
     float gs = mb, wv[3] = {0.f, 0.f, 0.f};
 #pragma unroll
@@ -3356,7 +3450,7 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
       for (int a = 0; a < D; ++a) wv[b] -= Sv[a][b] * fb[a];
     if (g2p) {
       float vnt[3] = {0.f, 0.f, 0.f}, dvn[3] = {0.f, 0.f, 0.f};
-      load_node_vel<D, true>(nodes.vel, node, vnt, dvn);
+      load_node_vel<D, true>(vel, node, vnt, dvn);
 #pragma unroll
       for (int b = 0; b < D; ++b) gs += vnt[b] * vpb[b] + dvn[b] * dvb[b];
 #pragma unroll
@@ -3426,7 +3520,7 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
       spline_eval<SF>(r, type, g.inv_cell_size, n3[a], d3[a]);
       h3[a] = spline_d2<SF>(r, type, g.inv_cell_size);
     }
-    entry((int)lin, n3, d3, h3, false);
+    entry(nbar, nodes.vel, (int)lin, n3, d3, h3, false);
   };
   // the window's offsets along y (and z) are picked by selects, so the
   // in-grid entries stay unrolled without indexing registers at run time
@@ -3440,7 +3534,7 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
 #pragma unroll
       for (int ex = 0; ex < E; ++ex) {
         const float n3[3] = {N[0][ex], Ny, Nz}, d3[3] = {dN[0][ex], dNy, dNz}, h3[3] = {d2N[0][ex], d2Ny, d2Nz};
-        entry(off[0][ex] + oyz, n3, d3, h3, true);
+        entry(nbar, nodes.vel, off[0][ex] + oyz, n3, d3, h3, true);
       }
       if (leaves && prm.wrap_quirk) {
 #pragma unroll 1
@@ -3448,6 +3542,30 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
           const int e3[3] = {ex, sy, sz};
           wrap_entry(e3);
         }
+      }
+    }
+  } else if constexpr (STAGED) {
+    // in-grid entries from the LDS records of the workgroup's runs (the
+    // clamped nodes, as off[]), then the wrap-around entries
+    adjoint_stage_rounds(g, nodes, nbar, w, live, [&](const float *adj, const float *vel, int box0, int X) {
+#pragma unroll 1
+      for (int sy = 0; sy < 4; ++sy) {
+        const float Ny = pick(N[1], sy), dNy = pick(dN[1], sy), d2Ny = pick(d2N[1], sy);
+#pragma unroll
+        for (int ex = 0; ex < 4; ++ex)
+#pragma unroll
+          for (int ez = 0; ez < 4; ++ez) {
+            const float n3[3] = {N[0][ex], Ny, N[2][ez]}, d3[3] = {dN[0][ex], dNy, dN[2][ez]},
+                        h3[3] = {d2N[0][ex], d2Ny, d2N[2][ez]};
+            entry(adj, vel, box0 + ex + X * (sy + 4 * ez), n3, d3, h3, true);
+          }
+      }
+    });
+    if (leaves && prm.wrap_quirk) {
+#pragma unroll 1
+      for (int e = 0; e < 64; ++e) {
+        const int e3[3] = {e & 3, (e >> 2) & 3, e >> 4};
+        wrap_entry(e3);
       }
     }
   } else {
@@ -3461,7 +3579,7 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
         for (int ez = 0; ez < EZ; ++ez) {
           const float n3[3] = {N[0][ex], Ny, N[2][ez]}, d3[3] = {dN[0][ex], dNy, dN[2][ez]},
                       h3[3] = {d2N[0][ex], d2Ny, d2N[2][ez]};
-          entry(off[0][ex] + oy + off[2][ez], n3, d3, h3, true);
+          entry(nbar, nodes.vel, off[0][ex] + oy + off[2][ez], n3, d3, h3, true);
         }
       if (leaves && prm.wrap_quirk) {
 #pragma unroll 1
@@ -3487,6 +3605,7 @@ __global__ __launch_bounds__(256) void p2g_adjoint_kernel(pmpm_params prm, pmpm_
     }
     if (sub0 != 0) return;
   }
+  if (!live) return;  // (a staging helper past the particles)
   // M_bar = -V G, V_bar = -sigma : G
   float Mb[3][3], Vb = 0.f;
 #pragma unroll
