@@ -60,8 +60,9 @@ KERNEL_BYTES = {
     "grid_kernel<3> gather 1": (0, 88),     # m, P, Q in 28; m, P, Q out 28, velocity record 32
     "grid_kernel<3> gather 2": (0, 72),     # m, P, Q in 28; Q out 12, velocity record 32
 }
-# The step's algorithmic bytes per particle (SURVEY-style budget: sort 16,
-# P2G 56, MUSL map 52, second map + return map 188 + L 36, grid ~23).
+# The step's algorithmic bytes per particle (compulsory traffic of a fused
+# step: sort 16, P2G 56, MUSL map 52, second map + return map 188 + L 36,
+# grid ~23).
 STEP_BYTES_PER_PARTICLE = 335
 # HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE)
 # and scaled by the same counters on a plain device copy of known size.
