@@ -226,9 +226,18 @@ def cpu_baseline(spec, steps):
         sim.step()
     dt = time.perf_counter() - t0
     n = len(spec.positions)
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": n * steps / dt, "unit": "particle-updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle FP32 single thread, {steps} steps of the full {n}-particle workload, "
-                      f"{dt:.1f} s"}
+            "sample": f"oracle FP32 single thread (g++ -O3), {steps} steps of the full {n}-particle workload, "
+                      f"{dt:.1f} s, on {model} ({os.cpu_count()} logical CPUs visible, 1 used)"}
 
 
 def adjoint_ratio(steps, every=20, h=0.02, per_axis=2):
