@@ -434,6 +434,31 @@ __device__ inline uint32_t block_sum(uint32_t v, uint32_t *w) {
   return total;
 }
 
+// The same scan of two 32-bit counts at once, packed as (high << 32) | low
+// (no carry crosses: each count stays below 2^31). `w` holds kHintBlock / 64
+// 64-bit words of LDS.
+__device__ inline uint64_t block_exclusive_scan2(uint64_t v, uint64_t *w, uint64_t &total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint64_t incl = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t u = (uint64_t)__shfl_up((unsigned long long)incl, o, 64);
+    if (lane >= o) incl += u;
+  }
+  __syncthreads();  // w may still be read by an earlier call
+  if (lane == 63) w[wv] = incl;
+  __syncthreads();
+  uint64_t before = 0u, all = 0u;
+#pragma unroll
+  for (int q = 0; q < kHintBlock / 64; ++q) {
+    const uint64_t x = w[q];
+    before += q < wv ? x : 0u;
+    all += x;
+  }
+  total = all;
+  return before + incl - v;
+}
+
 // Cell key per particle (as hash_kernel), its class against the hint, and the
 // counts: per cell and per block of cells, stayers and movers (a run of lanes
 // with one key and one class adds its length once); stayers per workgroup.
@@ -532,7 +557,7 @@ __global__ __launch_bounds__(kHintBlock) void hint_scan_kernel(
     uint32_t *__restrict__ cnt, const uint32_t *__restrict__ cbt, const uint32_t *__restrict__ pblk,
     uint32_t *__restrict__ counters, int4 *__restrict__ info, int32_t *__restrict__ cstart,
     int32_t *__restrict__ cend, int32_t *__restrict__ ss, uint32_t *__restrict__ mk, int32_t *__restrict__ sm) {
-  __shared__ uint32_t w[kHintBlock / 64];
+  __shared__ uint64_t w2[kHintBlock / 64];
   const bool unsorted = counters[1] != 0u;
   const int t = threadIdx.x;
   if ((int)blockIdx.x < ncb) {
@@ -544,13 +569,12 @@ __global__ __launch_bounds__(kHintBlock) void hint_scan_kernel(
       st = cnt[2 * c];
       mv = cnt[2 * c + 1];
     }
-    uint32_t ps = 0u, pm = 0u;
-    for (int j = t; j < b; j += kHintBlock) {
-      ps += cbt[2 * j];
-      pm += cbt[2 * j + 1];
-    }
-    uint32_t before_s = block_sum(ps, w);
-    uint32_t before_m = block_sum(pm, w);
+    uint64_t pp = 0u;  // (stayers << 32) | movers of the cell blocks before this one
+    for (int j = t; j < b; j += kHintBlock) pp += ((uint64_t)cbt[2 * j] << 32) | cbt[2 * j + 1];
+    uint64_t pt;
+    (void)block_exclusive_scan2(pp, w2, pt);
+    uint32_t before_s = (uint32_t)(pt >> 32);
+    uint32_t before_m = (uint32_t)pt;
     if (unsorted) {
       before_m += before_s;
       before_s = 0u;
@@ -563,9 +587,9 @@ __global__ __launch_bounds__(kHintBlock) void hint_scan_kernel(
       mv += st;
       st = 0u;
     }
-    uint32_t tot;
-    const uint32_t sx = block_exclusive_scan(st, w, tot);
-    const uint32_t mx = block_exclusive_scan(mv, w, tot);
+    uint64_t tot;
+    const uint64_t smx = block_exclusive_scan2(((uint64_t)st << 32) | mv, w2, tot);
+    const uint32_t sx = (uint32_t)(smx >> 32), mx = (uint32_t)smx;
     if (!valid) return;
     const uint32_t stay_before = before_s + sx, mover_before = before_m + mx;
     const uint32_t start = stay_before + mover_before, count = st + mv;
@@ -584,12 +608,13 @@ __global__ __launch_bounds__(kHintBlock) void hint_scan_kernel(
   const bool valid = i < n;
   const uint32_t k = valid ? hash[i] : 0u;  // (loaded before the prefix sum's barriers)
   const bool stay = valid && !unsorted && k == hint[i];
-  uint32_t pre = 0u;
+  // (stayers of the blocks before << 32) | this thread's stayer flag, in one scan
+  uint64_t pv = stay ? 1u : 0u;
   if (!unsorted)
-    for (int j = t; j < b; j += kHintBlock) pre += pblk[j];
-  const uint32_t before = block_sum(pre, w);
-  uint32_t tot;
-  const uint32_t S = before + block_exclusive_scan(stay ? 1u : 0u, w, tot);
+    for (int j = t; j < b; j += kHintBlock) pv += (uint64_t)pblk[j] << 32;
+  uint64_t tot;
+  const uint64_t ex = block_exclusive_scan2(pv, w2, tot);
+  const uint32_t S = (uint32_t)(tot >> 32) + (uint32_t)ex;
   if (!valid) return;
   ss[i] = (int32_t)(S | (stay ? 0x80000000u : 0u));
   if (!stay) {
