@@ -33,8 +33,8 @@ struct pmpm_context {
   // (allocated in words of 4 bytes)
   float *wrap_touch = nullptr;
   size_t wrap_touch_cap = 0;
-  // the nodes the typed-cubic wrap-around sums reach this pass: [0] count,
-  // [1] unused, [2..] node indices (each once)
+  // the nodes the typed-cubic wrap-around sums reach this pass (each once):
+  // kWrapListShards counts, then as many parts of node indices
   int32_t *wrap_list = nullptr;
   size_t wrap_list_cap = 0;
   // a byte per node: the components the last grid pass's NodeDomain
@@ -161,6 +161,7 @@ class TimedRegion {
 // be captured in a HIP graph).
 int zero_fill(void *ptr, int64_t n_words, hipStream_t s);
 constexpr int kZeroLists = 6;
+constexpr int kWrapListShards = 32;  // parts of the wrap-around node list (a power of two)
 struct ZeroLists {  // buffers of 32-bit words to clear in one launch
   uint32_t *p[kZeroLists];
   int64_t n[kZeroLists];

@@ -34,7 +34,7 @@ int pmpm::step_on(pmpm_context *ctx, const pmpm_params *prm, const pmpm_grid *gr
   // the wrap-around node flags and list counters (once allocated): every
   // step starts from an empty list
   z.add(ctx->wrap_touch, (int64_t)ctx->wrap_touch_cap);
-  z.add(ctx->wrap_list, ctx->wrap_list ? 2 : 0);
+  z.add(ctx->wrap_list, ctx->wrap_list ? kWrapListShards : 0);
   int rc = partition_clearing(ctx, prm->dim, grid, a->x, a->n, part, s, z);
   if (rc != PMPM_OK) return rc;
   if ((rc = do_p2g(ctx, *prm, *grid, *a, *part, *nodes, s, true)) != PMPM_OK) return rc;

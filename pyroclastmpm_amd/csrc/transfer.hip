@@ -1484,8 +1484,11 @@ __device__ inline void wrap_cell_sums(const pmpm_params &prm, const pmpm_grid &g
   int64_t lin;
   const bool active = lane < nlanes && offset_node(lane, sv, lin);
   // the node's flag (one byte of a 32-bit word); the first writer also
-  // appends the node to the gather's list (wlist[0] = count, entries from
-  // wlist[2]), so the list holds each reached node once
+  // appends the node to the gather's list, so the list holds each reached
+  // node once. The list has kWrapListShards parts (counts wlist[0..S), part
+  // p's entries from wlist[S + p cap]); a wave appends to part w mod S, so
+  // the appends of the wrap cells spread over S counters
+  const int cap = wrap_target_count<D>(g), shard = w & (kWrapListShards - 1);
   bool fresh = false;
   if (active) {
     const uint32_t bit = 1u << (8 * (int)(lin & 3));
@@ -1495,9 +1498,10 @@ __device__ inline void wrap_cell_sums(const pmpm_params &prm, const pmpm_grid &g
   if (fm) {  // wave-uniform
     const int first = __ffsll((unsigned long long)fm) - 1;
     int at = 0;
-    if (lane == first) at = atomicAdd(&wlist[0], __popcll(fm));
+    if (lane == first) at = atomicAdd(&wlist[shard], __popcll(fm));
     at = __shfl(at, first);
-    if (fresh) wlist[2 + at + __popcll(fm & ((1ull << lane) - 1ull))] = (int)lin;
+    const int idx = at + __popcll(fm & ((1ull << lane) - 1ull));
+    if (fresh && idx < cap) wlist[kWrapListShards + shard * cap + idx] = (int)lin;  // (at most cap nodes in all)
   }
   // the active offsets in order (s_list[rank] = offset)
   const uint64_t act = __ballot(active);
@@ -1930,11 +1934,23 @@ __global__ __launch_bounds__(256) void wrap_gather_list_kernel(pmpm_grid g, Tile
   const int lane = (int)threadIdx.x & 63;
   const int wv = (int)blockIdx.x * 4 + ((int)threadIdx.x >> 6), nw = (int)gridDim.x * 4;
   const int nn = g.num_cells_total;
-  const int count = __builtin_amdgcn_readfirstlane(wlist[0]);
+  // the list's parts: lane p < S holds part p's count; entry e of the whole
+  // list is in the part whose inclusive count first exceeds e
+  const int cap = wrap_target_count<D>(g);
+  const int cnt = lane < kWrapListShards ? min(wlist[lane], cap) : 0;
+  int incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += u;
+  }
+  const int count = __builtin_amdgcn_readlane(incl, 63);
   // grids below 3 cells on an axis have more images per node: generic path
   const bool small = g.num_cells[0] < 3 || g.num_cells[1] < 3 || (D > 2 && g.num_cells[2] < 3);
   for (int e = wv; e < count; e += nw) {  // wave-uniform
-    const int i = __builtin_amdgcn_readfirstlane(wlist[2 + e]);
+    const int part_e = __popcll(__ballot(lane < kWrapListShards && incl <= e));
+    const int before = __shfl(incl - cnt, part_e);
+    const int i = __builtin_amdgcn_readfirstlane(wlist[kWrapListShards + part_e * cap + (e - before)]);
     float acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) acc[k] = 0.f;
@@ -1980,7 +1996,8 @@ __global__ void grid_kernel(pmpm_params prm, pmpm_grid g, pmpm_nodes nodes, int 
                             int32_t *__restrict__ wlist) {
   const int nn = g.num_cells_total;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (wlist && i == 0) wlist[0] = 0;  // the wrap-around list of this pass is consumed
+  if (wlist && i == 0)  // the wrap-around list of this pass is consumed
+    for (int p2 = 0; p2 < kWrapListShards; ++p2) wlist[p2] = 0;
   if (i >= nn) return;
   float m = nodes.mass[i];
   float P[3] = {0.f, 0.f, 0.f}, Q[3] = {0.f, 0.f, 0.f};
@@ -2705,6 +2722,12 @@ static void partial_sizes(const pmpm_params &prm, const pmpm_grid &g, size_t *fl
     *floats = std::max(*floats, mt * mg.nv * (size_t)mg.N[0] * mg.N[1] * mg.N[2]);
   }
 }
+// Words of the wrap-around node list: the part counts, then kWrapListShards
+// parts of wrap_target_count entries each.
+static size_t wrap_node_list_words(const pmpm_params &prm, const pmpm_grid &g) {
+  const size_t cap = (size_t)(prm.dim == 3 ? wrap_target_count<3>(g) : (prm.dim == 2 ? wrap_target_count<2>(g) : 0));
+  return (size_t)kWrapListShards * (1 + cap);
+}
 static int reserve_tiles(pmpm_context *ctx, const pmpm_params &prm, const pmpm_grid &g) {
   size_t floats = 0, tiles = 0;
   partial_sizes(prm, g, &floats, &tiles);
@@ -2724,15 +2747,16 @@ static int reserve_tiles(pmpm_context *ctx, const pmpm_params &prm, const pmpm_g
     if ((rc = ensure_buffer(ctx->wrap_touch, ctx->wrap_touch_cap, words)) != PMPM_OK) return rc;
     PMPM_HIP_CHECK(hipMemset(ctx->wrap_touch, 0, words * sizeof(float)));
   }
-  // the gather's list: count, finished workgroups, one entry per node
-  const size_t lwords = (size_t)g.num_cells_total + 2;
+  // the gather's list: kWrapListShards counts, then as many parts of room
+  // for every node the wrap-around can reach
+  const size_t lwords = wrap_node_list_words(prm, g);
   if (lwords > ctx->wrap_list_cap) {
     float *lb = reinterpret_cast<float *>(ctx->wrap_list);
     size_t cap = ctx->wrap_list_cap;
     if ((rc = ensure_buffer(lb, cap, lwords)) != PMPM_OK) return rc;
     ctx->wrap_list = reinterpret_cast<int32_t *>(lb);
     ctx->wrap_list_cap = cap;
-    PMPM_HIP_CHECK(hipMemset(ctx->wrap_list, 0, 2 * sizeof(int32_t)));
+    PMPM_HIP_CHECK(hipMemset(ctx->wrap_list, 0, kWrapListShards * sizeof(int32_t)));
   }
   return PMPM_OK;
 }
@@ -2745,7 +2769,7 @@ void p2g_workspace(const pmpm_params &prm, const pmpm_grid &g, size_t *partial_f
   *wrap_cells = (size_t)(prm.dim == 3 ? wrap_cell_count<3>(g) : (prm.dim == 2 ? wrap_cell_count<2>(g) : 0));
   *wrap_floats = wrap_mode(prm, g) == 2 ? (size_t)wrap_rec_stride(1 + 2 * prm.dim) * *wrap_cells * 64 : 0;
   *wrap_touch_words = wrap_mode(prm, g) == 2 ? ((size_t)g.num_cells_total + 3) / 4 : 0;
-  *wrap_list_words = wrap_mode(prm, g) == 2 ? (size_t)g.num_cells_total + 2 : 0;
+  *wrap_list_words = wrap_mode(prm, g) == 2 ? wrap_node_list_words(prm, g) : 0;
   *clamp_words = ((size_t)g.num_cells_total + 3) / 4;
 }
 
