@@ -431,12 +431,26 @@ def test_adjoint_p2g_lane_layouts_agree(monkeypatch):
     chosen by launch size) give the same adjoint step (field-scaled relative
     error <= 1e-5: the layouts add the window's entries in different orders).
     PMPM_ADJOINT_P2G_LANES forces the layout."""
+    spec = S.column(h=0.04, per_axis=2, dt=1e-4, jitter=0.1, seed=3, musl=False)
+    _lane_layouts_agree(monkeypatch, spec, "3D DP column")
+
+
+def test_adjoint_p2g_lane_layouts_agree_long_runs(monkeypatch):
+    """The same on a column 38 cells long in x: the one-lane layout stages
+    the node records of runs of at most 32 x cells (a run ends at every
+    32-aligned x block), so here runs split and several rounds of staging
+    meet in one workgroup; the cells at x = 0 also take wrap-around entries."""
+    spec = S.column(h=0.04, per_axis=2, dt=1e-4, jitter=0.1, seed=4, musl=False, domain=(1.6, 0.24, 0.32),
+                    col=(1.5, 0.16, 0.2))
+    _lane_layouts_agree(monkeypatch, spec, "3D DP column 38 cells long")
+
+
+def _lane_layouts_agree(monkeypatch, spec, label):
     from pyroclastmpm_amd.diff import AdjointSet, DiffMPM, StepBuffers, ParticleSet
 
-    spec = S.column(h=0.04, per_axis=2, dt=1e-4, jitter=0.1, seed=3, musl=False)
     model = DiffMPM(3, spec.origin, spec.end, 0.04, spec.positions, 8, dt=1e-4)
     theta = (30.0, 20.0, 1e6)
-    _banner(f"adjoint P2G lanes 1 / 4 / 16: 3D DP column, n={model.n}, 12 warm-up steps")
+    _banner(f"adjoint P2G lanes 1 / 4 / 16: {label}, n={model.n}, 12 warm-up steps")
     state, _ = model.forward(theta, 12)
     prm = model.params(theta)
     n = model.n
