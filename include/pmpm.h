@@ -182,7 +182,11 @@ typedef struct pmpm_context pmpm_context;
 
 /* ---- context / workspace -------------------------------------------------
  * A context owns the scratch memory of the spatial sort for up to
- * max_particles particles and max_cells cells on the current device. */
+ * max_particles particles and max_cells cells on the current device (and the
+ * P2G, wrap-around and adjoint workspace, grown on first use). Its work is
+ * stream-ordered: calls on one context must be issued to one stream, or
+ * ordered between streams by the caller; overlapping calls on one context
+ * race on its workspace. Independent scenes take a context each. */
 int pmpm_context_create(pmpm_context **ctx);
 int pmpm_context_destroy(pmpm_context *ctx);
 int pmpm_context_reserve(pmpm_context *ctx, int32_t max_particles, int32_t max_cells);
